@@ -1,0 +1,325 @@
+"""Python host binding of libhodlrnd (MI355X / gfx950 HODLRnD hot path).
+
+Mirrors the reference's operator API (include/HDD_matrix.hpp:20-85,
+include/GMRES.hpp:44-50) over the C ABI declared in include/hodlrnd.h:
+
+    A = HODLRdD_matrix(kernel, points, x1, x2, nmax=..., dprime=..., eps_aca=...)
+    A.Assemble_matrix_operators()
+    y = A * x
+    x = A.solve(b)
+    A.print_matrix_statistics()
+
+Every compute call goes to the HIP library; there is no CPU fallback.  If
+libhodlrnd.so has not been built the import raises.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libhodlrnd.so")
+
+OK, E_ARG, E_DEVICE, E_NOMEM, E_RANK, E_NOCONV, E_BREAKDOWN, E_STATE = 0, -1, -2, -3, -4, -5, -6, -7
+KERNEL_LOG, KERNEL_INV_R, KERNEL_EXP = 0, 1, 2
+PREC_F64, PREC_F32 = 1, 2
+SOLVE_F64, SOLVE_MIXED = 0, 1
+
+
+class HodlrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("hodlrnd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class KernelDesc(C.Structure):
+    _fields_ = [("family", C.c_int32), ("ndim", C.c_int32), ("scale", C.c_double),
+                ("diag", C.c_double), ("weight", C.c_void_p)]
+
+
+class TreeInfo(C.Structure):
+    _fields_ = [("nlevels", C.c_int32), ("ndim", C.c_int32), ("npoints", C.c_int64),
+                ("nnodes", C.c_int64), ("nleaves", C.c_int64), ("max_leaf", C.c_int64),
+                ("n_admissible", C.c_int64), ("n_near", C.c_int64)]
+
+
+class OpStats(C.Structure):
+    _fields_ = [("n_flop_ref", C.c_double), ("max_rank", C.c_int64), ("e_near", C.c_double),
+                ("e_far", C.c_double), ("n_blocks", C.c_int64), ("n_dense", C.c_int64),
+                ("bytes_f64", C.c_double), ("bytes_f32", C.c_double),
+                ("matvec_bytes_f64", C.c_double), ("matvec_bytes_f32", C.c_double),
+                ("assemble_seconds", C.c_double), ("rank_seconds", C.c_double),
+                ("fill_seconds", C.c_double)]
+
+
+class GmresParams(C.Structure):
+    _fields_ = [("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
+                ("mode", C.c_int32), ("x0_is_b", C.c_int32), ("inner_tol", C.c_double),
+                ("max_outer", C.c_int32), ("verify", C.c_int32), ("hist_cap", C.c_int32),
+                ("hist", C.c_void_p)]
+
+
+class GmresInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("matvecs_f64", C.c_int32), ("matvecs_f32", C.c_int32),
+                ("outer", C.c_int32), ("converged", C.c_int32), ("hist_len", C.c_int32),
+                ("resid", C.c_double), ("true_resid", C.c_double), ("seconds", C.c_double)]
+
+
+SIGNATURES = {
+    "hodlr_version": (C.c_char_p, []),
+    "hodlr_ctx_create": (C.c_int, [C.c_int, C.c_void_p]),
+    "hodlr_ctx_destroy": (None, [C.c_void_p]),
+    "hodlr_last_error": (C.c_char_p, [C.c_void_p]),
+    "hodlr_tree_build": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hodlr_tree_get_info": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hodlr_tree_perm": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hodlr_tree_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hodlr_tree_lists": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "hodlr_tree_destroy": (None, [C.c_void_p]),
+    "hodlr_op_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                    C.c_void_p]),
+    "hodlr_op_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hodlr_op_block_ranks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "hodlr_op_destroy": (None, [C.c_void_p]),
+    "hodlr_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "hodlr_matvec_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "hodlr_gmres": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hodlr_gmres_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hodlr_matvec_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "hodlr_matvec_kernel_bytes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hodlr_device_alloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
+    "hodlr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hodlr_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "hodlr_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    "hodlr_to_tree_order": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "hodlr_from_tree_order": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+}
+
+
+def load_library(path=LIBPATH):
+    if not os.path.exists(path):
+        raise ImportError("libhodlrnd.so is not built (%s); run __graft_entry__.build()" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = load_library()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib.hodlr_ctx_create(device, C.byref(self.h))
+        if rc != OK:
+            raise HodlrError(rc, "context creation failed (device %d)" % device)
+
+    def check(self, rc, what=""):
+        if rc != OK:
+            raise HodlrError(rc, "%s: %s" % (what, lib.hodlr_last_error(self.h).decode()))
+
+    def close(self):
+        if self.h:
+            lib.hodlr_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Tree:
+    """2^d tree and lists (HDD_tree.hpp:23-74, HDD_node.hpp:150-180)."""
+
+    def __init__(self, points, x1, x2, nmax, dprime, ctx=None):
+        self.ctx = ctx or default_context()
+        self.points = _f64(points)
+        if self.points.ndim != 2:
+            raise ValueError("points must be n x ndim")
+        self.n, self.ndim = self.points.shape
+        lo, hi = _f64(x1), _f64(x2)
+        self.h = C.c_void_p()
+        self.ctx.check(lib.hodlr_tree_build(self.ctx.h, self.ndim, self.n, _p(self.points), _p(lo),
+                                            _p(hi), nmax, dprime, C.byref(self.h)), "tree_build")
+        self.info = TreeInfo()
+        lib.hodlr_tree_get_info(self.h, C.byref(self.info))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hodlr_tree_destroy(self.h)
+            self.h = None
+
+    @property
+    def nlevels(self):
+        return self.info.nlevels
+
+    def perm(self):
+        out = np.empty(self.n, dtype=np.int64)
+        lib.hodlr_tree_perm(self.h, _p(out))
+        return out
+
+    def nodes(self):
+        nn = self.info.nnodes
+        lo = np.empty(self.nlevels + 1, dtype=np.int64)
+        st = np.empty(nn, dtype=np.int64)
+        ct = np.empty(nn, dtype=np.int64)
+        lib.hodlr_tree_nodes(self.h, _p(lo), _p(st), _p(ct))
+        return lo, st, ct
+
+    def lists(self, which):
+        ptr = np.empty(self.info.nnodes + 1, dtype=np.int64)
+        lib.hodlr_tree_lists(self.h, which, _p(ptr), None)
+        idx = np.empty(max(int(ptr[-1]), 1), dtype=np.int64)
+        lib.hodlr_tree_lists(self.h, which, _p(ptr), _p(idx))
+        return ptr, idx[:int(ptr[-1])]
+
+
+class Kernel:
+    """Device-evaluable kernel: A_ij = diag if i == j else scale * f(r) * w_j."""
+
+    def __init__(self, family, ndim, scale, diag, weight=None):
+        self.family, self.ndim, self.scale, self.diag = family, ndim, scale, diag
+        self.weight = None if weight is None else _f64(weight)
+
+    def desc(self):
+        return KernelDesc(self.family, self.ndim, self.scale, self.diag,
+                          None if self.weight is None else self.weight.ctypes.data)
+
+
+# reference test kernels (user_kernel.hpp:10-179)
+def kernel_2d_test():
+    return Kernel(KERNEL_LOG, 2, -1.0 / (2 * math.pi), 0.0)
+
+
+def kernel_3d_test():
+    return Kernel(KERNEL_INV_R, 3, 1.0 / (4 * math.pi), 0.0)
+
+
+def kernel_4d_test(num_points, kii):
+    h = 1.0 / num_points
+    return Kernel(KERNEL_EXP, 4, h ** 4 * (-1.0 / (4 * math.pi)), kii)
+
+
+class Operator:
+    """Assembled HODLR operator (HDD_matrix.hpp:39-47, GMRES.hpp:160-303)."""
+
+    def __init__(self, tree, kernel, eps_aca, prec=PREC_F64 | PREC_F32):
+        self.tree, self.kernel, self.ctx = tree, kernel, tree.ctx
+        self.h = C.c_void_p()
+        kd = kernel.desc()
+        self.ctx.check(lib.hodlr_op_assemble(self.ctx.h, tree.h, C.byref(kd), eps_aca, prec,
+                                             C.byref(self.h)), "op_assemble")
+        self.n = tree.n
+        self.prec = prec
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.hodlr_op_destroy(self.h)
+            self.h = None
+
+    def stats(self):
+        s = OpStats()
+        lib.hodlr_op_get_stats(self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in OpStats._fields_}
+
+    def block_ranks(self):
+        nb = self.stats()["n_blocks"]
+        r = np.empty(max(nb, 1), dtype=np.int32)
+        d = np.empty(max(nb, 1), dtype=np.int32)
+        self.ctx.check(lib.hodlr_op_block_ranks(self.h, _p(r), _p(d), max(nb, 1)), "block_ranks")
+        return r[:nb], d[:nb]
+
+    def matvec(self, x, prec=PREC_F64):
+        x = np.asarray(x, dtype=np.float64)
+        single = x.ndim == 1
+        X = np.asfortranarray(x.reshape(self.n, -1))
+        Y = np.empty_like(X, order="F")
+        self.ctx.check(lib.hodlr_matvec(self.h, prec, X.shape[1], _p(X), _p(Y)), "matvec")
+        return Y[:, 0].copy() if single else Y
+
+    def gmres(self, b, x0=None, restart=100, max_iter=None, tol=1e-10, mode=SOLVE_F64,
+              inner_tol=1e-6, max_outer=20, verify=True, hist_cap=512):
+        b = _f64(b)
+        x = b.copy() if x0 is None else _f64(x0).copy()
+        hist = np.zeros(hist_cap)
+        p = GmresParams(restart, self.n if max_iter is None else max_iter, tol, mode,
+                        1 if x0 is None else 0, inner_tol, max_outer, 1 if verify else 0,
+                        hist_cap, hist.ctypes.data)
+        info = GmresInfo()
+        rc = lib.hodlr_gmres(self.h, _p(b), _p(x), C.byref(p), C.byref(info))
+        if rc not in (OK, E_NOCONV):
+            self.ctx.check(rc, "gmres")
+        out = {f: getattr(info, f) for f, _ in GmresInfo._fields_}
+        out["hist"] = hist[:info.hist_len].copy()
+        return x, out
+
+    def matvec_timing(self, prec=PREC_F64, nrhs=1, reps=10):
+        ms = np.zeros(3)
+        self.ctx.check(lib.hodlr_matvec_timing(self.h, prec, nrhs, reps, _p(ms)), "matvec_timing")
+        return ms
+
+    def kernel_bytes(self, prec=PREC_F64, nrhs=1):
+        b = np.zeros(3)
+        lib.hodlr_matvec_kernel_bytes(self.h, prec, nrhs, _p(b))
+        return b
+
+
+class HODLRdD_matrix:
+    """Reference-shaped operator object (HDD_matrix.hpp:10-89)."""
+
+    def __init__(self, kernel, points, x1, x2, nmax=64, dprime=0, eps_aca=1e-10, ctx=None,
+                 prec=PREC_F64 | PREC_F32):
+        self.kernel = kernel
+        self.eps_aca = eps_aca
+        self.prec = prec
+        self.tree = Tree(points, x1, x2, nmax, dprime, ctx)
+        self.N = self.tree.n
+        self.op = None
+
+    def get_size(self):
+        return self.N
+
+    def Assemble_matrix_operators(self):
+        self.op = Operator(self.tree, self.kernel, self.eps_aca, self.prec)
+
+    def __mul__(self, x):
+        return self.op.matvec(x)
+
+    def solve(self, b, tol=None, mode=SOLVE_F64, **kw):
+        """GMRES(100) from x0 = b to tol = eps_ACA (HDD_matrix.hpp:51-62)."""
+        x, info = self.op.gmres(b, restart=100, max_iter=self.N,
+                                tol=self.eps_aca if tol is None else tol, mode=mode, **kw)
+        self.last_info = info
+        return x
+
+    def print_matrix_statistics(self):
+        s = self.op.stats()
+        print("Memory (in GB) : %g" % (8e-9 * s["n_flop_ref"]))
+        print("Number of FLOP : %g" % s["n_flop_ref"])
+        print("Compression Ratio : %g" % (s["n_flop_ref"] / (float(self.N) ** 2)))
+        print("Maximum rank across the Tree : %d" % s["max_rank"])
+        print("Time to Initialize (in s) : %g" % s["assemble_seconds"])
+        return s

@@ -1,0 +1,498 @@
+// aca.hip -- batched adaptive cross approximation and dense fills (gfx950).
+//
+// One workgroup runs the partially pivoted ACA of one admissible block with
+// the reference's control flow (LowRank_matrix.hpp:64-377): first pivot row
+// 0, absolute pivot floor eps_ACA with up to ten retries (ends of the index
+// range while rank < 3, then alternately a random remaining index and the
+// midpoint of the widest gap of used indices), the squared-norm stop test,
+// and the dense fallback when the rank reaches min(m, n) - 1.  Every thread
+// runs the same control flow; block reductions broadcast pivots and norms.
+//
+// The kernel runs twice per block: a rank pass that only records the rank
+// (so the operator's exact footprint is known before it is allocated) and a
+// fill pass that repeats the identical computation and writes the factors
+// into their final panels.  Pivot-row/column residuals use explicit FMA in
+// the same order as the FP64 CPU restatement, so both pick the same pivots.
+#include <climits>
+
+#include "aca_layout.h"
+#include "device_common.h"
+
+namespace hodlr {
+
+namespace {
+
+template <int BS>
+struct Shared {
+    double v[BS / kWave], a[BS / kWave];
+    int i[BS / kWave];
+    double du[BS / kWave][kWave], dv[BS / kWave][kWave];
+    int64_t cnt[BS];
+    int bcast;
+};
+
+__device__ __forceinline__ bool better(double aa, int ai, double ba, int bi) {
+    return aa > ba || (aa == ba && ai < bi);
+}
+
+// argmax of |v| over the valid entries, lowest index on ties
+// (kernel_function.hpp:55-68 scans ascending with a strict >)
+template <int BS>
+__device__ void block_argmax(Shared<BS> &sh, double &val, double &av, int &idx) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double ov = __shfl_xor(val, o, kWave);
+        double oa = __shfl_xor(av, o, kWave);
+        int oi = __shfl_xor(idx, o, kWave);
+        if (better(oa, oi, av, idx)) { val = ov; av = oa; idx = oi; }
+    }
+    if constexpr (BS > kWave) {
+        const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+        if (lane == 0) { sh.v[wid] = val; sh.a[wid] = av; sh.i[wid] = idx; }
+        __syncthreads();
+        val = sh.v[0]; av = sh.a[0]; idx = sh.i[0];
+        for (int w = 1; w < BS / kWave; w++)
+            if (better(sh.a[w], sh.i[w], av, idx)) { val = sh.v[w]; av = sh.a[w]; idx = sh.i[w]; }
+        __syncthreads();
+    }
+}
+
+template <int BS>
+__device__ double block_sum(Shared<BS> &sh, double v) {
+    v = wave_sum(v);
+    if constexpr (BS > kWave) {
+        const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+        if (lane == 0) sh.v[wid] = v;
+        __syncthreads();
+        v = sh.v[0];
+        for (int w = 1; w < BS / kWave; w++) v += sh.v[w];
+        __syncthreads();
+    }
+    return v;
+}
+
+template <int BS>
+__device__ int block_min_int(Shared<BS> &sh, int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+    if constexpr (BS > kWave) {
+        const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+        if (lane == 0) sh.i[wid] = v;
+        __syncthreads();
+        v = sh.i[0];
+        for (int w = 1; w < BS / kWave; w++) v = min(v, sh.i[w]);
+        __syncthreads();
+    }
+    return v;
+}
+
+template <int BS>
+__device__ int block_max_int(Shared<BS> &sh, int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+    if constexpr (BS > kWave) {
+        const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+        if (lane == 0) sh.i[wid] = v;
+        __syncthreads();
+        v = sh.i[0];
+        for (int w = 1; w < BS / kWave; w++) v = max(v, sh.i[w]);
+        __syncthreads();
+    }
+    return v;
+}
+
+template <int BS>
+__device__ int first_unused(Shared<BS> &sh, const unsigned char *used, int len) {
+    int best = INT_MAX;
+    for (int i = threadIdx.x; i < len; i += BS)
+        if (!used[i]) { best = i; break; }
+    best = block_min_int(sh, best);
+    return best == INT_MAX ? 0 : best;
+}
+
+template <int BS>
+__device__ int last_unused(Shared<BS> &sh, const unsigned char *used, int len) {
+    int best = -1;
+    for (int i = len - 1 - (int)threadIdx.x; i >= 0; i -= BS)
+        if (!used[i]) { best = i; break; }
+    best = block_max_int(sh, best);
+    return best < 0 ? 0 : best;
+}
+
+// k-th (0-based) unused index in ascending order (std::advance over a set)
+template <int BS>
+__device__ int kth_unused(Shared<BS> &sh, const unsigned char *used, int len, int64_t k) {
+    const int tid = threadIdx.x;
+    const int chunk = (len + BS - 1) / BS;
+    const int b = tid * chunk, e = min(len, b + chunk);
+    int64_t c = 0;
+    for (int i = b; i < e; i++) c += used[i] ? 0 : 1;
+    sh.cnt[tid] = c;
+    __syncthreads();
+    if (tid == 0) {
+        int64_t run = 0;
+        int found = 0;
+        for (int t = 0; t < BS; t++) {
+            int64_t ct = sh.cnt[t];
+            if (run + ct > k) {
+                int64_t want = k - run;
+                int bb = t * chunk, ee = min(len, bb + chunk);
+                for (int i = bb; i < ee; i++) {
+                    if (used[i]) continue;
+                    if (want == 0) { found = i; break; }
+                    want--;
+                }
+                break;
+            }
+            run += ct;
+        }
+        sh.bcast = found;
+    }
+    __syncthreads();
+    int r = sh.bcast;
+    __syncthreads();
+    return r;
+}
+
+// midpoint of the widest gap of the used indices (LowRank_matrix.hpp:181-198)
+template <int BS>
+__device__ int mid_gap(Shared<BS> &sh, const int *ind, int cnt, int *tmp) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < cnt; i++) {
+            int v = ind[i], p = i;
+            while (p > 0 && tmp[p - 1] > v) { tmp[p] = tmp[p - 1]; p--; }
+            tmp[p] = v;
+        }
+        unsigned int mx = 0;
+        int at = 0;
+        for (int i = 0; i + 1 < cnt; i++) {
+            unsigned int d = (unsigned int)(tmp[i + 1] - tmp[i]);
+            if (d > mx) { at = i; mx = d; }
+        }
+        sh.bcast = tmp[at] + (int)(mx / 2);
+    }
+    __syncthreads();
+    int r = sh.bcast;
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ uint64_t splitmix64_next(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+template <int BS>
+__global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ blocks, KernelDev K,
+                                                 double tol, double *__restrict__ pool,
+                                                 int32_t *__restrict__ out_rank,
+                                                 int32_t *__restrict__ out_flags, int fill,
+                                                 const FillDest *__restrict__ dest,
+                                                 const int64_t *__restrict__ leaf_start,
+                                                 const int32_t *__restrict__ leaf_cnt,
+                                                 double *__restrict__ UP, double *__restrict__ VP) {
+    __shared__ Shared<BS> sh;
+    const AcaBlock B = blocks[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    const int m = B.m, n = B.n, cap = B.cap;
+    const int mn = m < n ? m : n;
+    double *U = pool + B.scratch;
+    double *V = U + (size_t)m * cap;
+    double *row = V + (size_t)n * cap;
+    double *col = row + n;
+    unsigned char *rused = reinterpret_cast<unsigned char *>(col + m);
+    unsigned char *cused = rused + m;
+    int *rind = reinterpret_cast<int *>(col + m + (m + n + 7) / 8);
+    int *cind = rind + aca_icap(cap);
+    int *tmp = cind + aca_icap(cap);
+
+    for (int i = tid; i < m; i += BS) rused[i] = 0;
+    for (int j = tid; j < n; j += BS) cused[j] = 0;
+    __syncthreads();
+    if (tid == 0) { rused[0] = 1; rind[0] = 0; }
+    __syncthreads();
+    int rrem = m - 1, crem = n, nri = 1, nci = 0;
+    int r = 0, pivot = 0, flags = 0;
+    double maxv = 0.0, mnorm = 0.0, rnorm = 0.0, cnorm = 0.0, gamma = 0.0;
+    uint64_t rng = B.seed;
+
+    // residual row ri of the block (LowRank_matrix.hpp:126-133); argmax over
+    // the columns not yet used as pivots
+    auto eval_row = [&](int ri) {
+        const int64_t pi = B.ord[B.row0 + ri];
+        double bv = 0.0, ba = -1.0;
+        int bi = INT_MAX;
+        for (int j = tid; j < n; j += BS) {
+            double v = entry(K, pi, B.ord[B.col0 + j]);
+            for (int q = 0; q < r; q++) v = fma(-U[(size_t)q * m + ri], V[(size_t)q * n + j], v);
+            row[j] = v;
+            if (!cused[j]) {
+                double a = fabs(v);
+                if (better(a, j, ba, bi)) { ba = a; bv = v; bi = j; }
+            }
+        }
+        block_argmax(sh, bv, ba, bi);
+        if (bi == INT_MAX) {  // no column left: index 0, as *begin() of an empty set
+            __syncthreads();
+            bi = 0;
+            bv = row[0];
+        }
+        maxv = bv;
+        pivot = bi;
+    };
+    // residual column cj (LowRank_matrix.hpp:237-243)
+    auto eval_col = [&](int cj) {
+        const int64_t pj = B.ord[B.col0 + cj];
+        double bv = 0.0, ba = -1.0;
+        int bi = INT_MAX;
+        for (int i = tid; i < m; i += BS) {
+            double v = entry(K, B.ord[B.row0 + i], pj);
+            for (int q = 0; q < r; q++) v = fma(-V[(size_t)q * n + cj], U[(size_t)q * m + i], v);
+            col[i] = v;
+            if (!rused[i]) {
+                double a = fabs(v);
+                if (better(a, i, ba, bi)) { ba = a; bv = v; bi = i; }
+            }
+        }
+        block_argmax(sh, bv, ba, bi);
+        if (bi == INT_MAX) {
+            __syncthreads();
+            bi = 0;
+            bv = col[0];
+        }
+        maxv = bv;
+        pivot = bi;
+    };
+
+    do {
+        __syncthreads();
+        eval_row(rind[nri - 1]);
+        int count = 0;
+        bool at_end = false, use_rand = true;
+        while (fabs(maxv) < tol && count < kMaxTries && crem > 0 && rrem > 0) {  // :144-216
+            nri--;
+            int nr;
+            if (r < 3) {
+                nr = at_end ? last_unused(sh, rused, m) : first_unused(sh, rused, m);
+                at_end = !at_end;
+            } else {
+                if (use_rand) nr = kth_unused(sh, rused, m, (int64_t)(splitmix64_next(rng) % (uint64_t)rrem));
+                else nr = mid_gap(sh, rind, nri, tmp);
+                use_rand = !use_rand;
+            }
+            __syncthreads();
+            const bool was_used = rused[nr] != 0;
+            __syncthreads();
+            if (tid == 0) {
+                rind[nri] = nr;
+                rused[nr] = 1;
+            }
+            nri++;
+            if (!was_used) rrem--;
+            __syncthreads();
+            eval_row(nr);
+            count++;
+        }
+        if (count == kMaxTries || crem == 0 || rrem == 0) break;  // :220-225
+        count = 0;
+        __syncthreads();
+        if (tid == 0) {
+            cind[nci] = pivot;
+            cused[pivot] = 1;
+        }
+        nci++;
+        crem--;
+        gamma = 1.0 / maxv;  // :233
+        __syncthreads();
+        eval_col(pivot);
+        while (fabs(maxv) < tol && count < kMaxTries && crem > 0 && rrem > 0) {  // :245-316
+            nci--;
+            int nc;
+            if (nci < 3) {
+                // :256-258: *remaining_col_ind.end() reads the set's size in libstdc++
+                nc = at_end ? crem : first_unused(sh, cused, n);
+                at_end = !at_end;
+            } else {
+                if (use_rand) nc = kth_unused(sh, cused, n, (int64_t)(splitmix64_next(rng) % (uint64_t)crem));
+                else nc = mid_gap(sh, cind, nci, tmp);
+                use_rand = !use_rand;
+            }
+            __syncthreads();
+            const bool was_used = cused[nc] != 0;
+            __syncthreads();
+            if (tid == 0) {
+                cind[nci] = nc;
+                cused[nc] = 1;
+            }
+            nci++;
+            if (!was_used) crem--;
+            __syncthreads();
+            eval_col(nc);
+            count++;
+        }
+        __syncthreads();
+        {  // :318-319
+            const bool was_used = rused[pivot] != 0;
+            __syncthreads();
+            if (tid == 0) {
+                rind[nri] = pivot;
+                rused[pivot] = 1;
+            }
+            nri++;
+            if (!was_used) rrem--;
+        }
+        if (r >= cap) {
+            if (cap < mn - 1) flags |= 2;  // rank budget exhausted
+            break;
+        }
+        // u = gamma col, v = row and the norm estimate (:322-338)
+        double rs = 0.0, cs = 0.0;
+        for (int j = tid; j < n; j += BS) {
+            double v = row[j];
+            V[(size_t)r * n + j] = v;
+            rs = rs + v * v;
+        }
+        for (int i = tid; i < m; i += BS) {
+            double c = col[i];
+            U[(size_t)r * m + i] = gamma * c;
+            cs = cs + c * c;
+        }
+        rs = block_sum(sh, rs);
+        cs = block_sum(sh, cs);
+        rnorm = sqrt(rs);
+        cnorm = sqrt(cs);
+        mnorm += fabs(gamma * gamma * rs * cs);
+        for (int q0 = 0; q0 < r; q0 += kWave) {
+            const int qn = min(kWave, r - q0);
+            for (int qq = 0; qq < qn; qq++) {
+                const double *uq = U + (size_t)(q0 + qq) * m, *ur = U + (size_t)r * m;
+                const double *vq = V + (size_t)(q0 + qq) * n, *vr = V + (size_t)r * n;
+                double du = 0.0, dv = 0.0;
+                for (int i = tid; i < m; i += BS) du = fma(uq[i], ur[i], du);
+                for (int j = tid; j < n; j += BS) dv = fma(vq[j], vr[j], dv);
+                du = wave_sum(du);
+                dv = wave_sum(dv);
+                if (lane == 0) { sh.du[wid][qq] = du; sh.dv[wid][qq] = dv; }
+            }
+            __syncthreads();
+            double contrib = 0.0;
+            if (tid < qn) {
+                double a = 0.0, b = 0.0;
+                for (int w = 0; w < BS / kWave; w++) { a += sh.du[w][tid]; b += sh.dv[w][tid]; }
+                contrib = 2.0 * fabs(a) * fabs(b);
+            }
+            __syncthreads();
+            mnorm += block_sum(sh, contrib);
+        }
+        r++;
+    } while ((double)(r * (m + n)) * rnorm * cnorm > fabs(maxv) * tol * mnorm && r < mn);  // :341-344
+
+    const bool dense = r >= mn - 1;  // :348
+    if (tid == 0) {
+        out_rank[B.id] = r;
+        out_flags[B.id] = flags | (dense ? 1 : 0);
+    }
+    if (!fill || dense || (flags & 2)) return;
+    __syncthreads();
+    const FillDest D = dest[B.id];
+    // U columns into the row node's leaf-blocked panel
+    for (int i = tid; i < m; i += BS) {
+        const int64_t p = B.ord[B.row0 + i];
+        const int64_t ls = leaf_start[p];
+        const int64_t nc = leaf_cnt[p];
+        const int64_t base = D.up_base + (int64_t)D.R * (ls - B.row0) + (p - ls);
+        for (int q = 0; q < r; q++) UP[base + (int64_t)(D.coff + q) * nc] = U[(size_t)q * m + i];
+    }
+    // V columns into the column node's row-major panel
+    for (int j = tid; j < n; j += BS) {
+        const int64_t base = D.vp_base + (int64_t)(B.ord[B.col0 + j] - B.col0) * D.T + D.voff;
+        for (int q = 0; q < r; q++) VP[base + q] = V[(size_t)q * n + j];
+    }
+}
+
+// dense-fallback blocks written into the row node's U panel
+__global__ __launch_bounds__(256) void dense_fill_kernel(const DenseBlock *__restrict__ blocks,
+                                                         KernelDev K,
+                                                         const int64_t *__restrict__ leaf_start,
+                                                         const int32_t *__restrict__ leaf_cnt,
+                                                         double *__restrict__ UP) {
+    const DenseBlock B = blocks[blockIdx.x];
+    const int64_t tot = (int64_t)B.m * B.n;
+    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int64_t i = e % B.m, q = e / B.m;
+        const int64_t p = B.row0 + i;
+        const int64_t ls = leaf_start[p], nc = leaf_cnt[p];
+        UP[B.up_base + (int64_t)B.R * (ls - B.row0) + (int64_t)(B.coff + q) * nc + (p - ls)] =
+            entry(K, p, B.col0 + q);
+    }
+}
+
+// near-field panel of one leaf: self block then neighbours (HDD_node.hpp:136-144)
+__global__ __launch_bounds__(256) void near_fill_kernel(const NearLeaf *__restrict__ leaves,
+                                                        const int64_t *__restrict__ colsrc,
+                                                        KernelDev K, double *__restrict__ NP) {
+    const NearLeaf L = leaves[blockIdx.x];
+    const int64_t tot = (int64_t)L.nc * L.ncols;
+    for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int64_t i = e % L.nc, q = e / L.nc;
+        NP[L.np_base + e] = entry(K, L.row0 + i, colsrc[L.colsrc + q]);
+    }
+}
+
+__global__ void convert_f64_f32_kernel(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+// ---- host launchers ----
+
+hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev &K, double tol,
+                      double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
+                      const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
+                      hipStream_t s) {
+    if (nblocks == 0) return hipSuccess;
+    dim3 g((unsigned)nblocks);
+    switch (bs) {
+    case 64:
+        aca_kernel<64><<<g, 64, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        break;
+    case 256:
+        aca_kernel<256><<<g, 256, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        break;
+    default:
+        aca_kernel<1024><<<g, 1024, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev &K, const int64_t *leaf_start,
+                             const int32_t *leaf_cnt, double *UP, hipStream_t s) {
+    if (nb == 0) return hipSuccess;
+    dense_fill_kernel<<<dim3((unsigned)nb), 256, 0, s>>>(b, K, leaf_start, leaf_cnt, UP);
+    return hipGetLastError();
+}
+
+hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev &K,
+                            double *NP, hipStream_t s) {
+    if (nl == 0) return hipSuccess;
+    near_fill_kernel<<<dim3((unsigned)nl), 256, 0, s>>>(l, colsrc, K, NP);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    int64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    convert_f64_f32_kernel<<<dim3((unsigned)g), 256, 0, s>>>(a, b, n);
+    return hipGetLastError();
+}
+
+}  // namespace hodlr

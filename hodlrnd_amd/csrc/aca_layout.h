@@ -1,0 +1,97 @@
+// aca_layout.h -- work descriptors shared by host planning code and kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace hodlr {
+
+constexpr int kMaxTries = 10;  // LowRank_matrix.hpp:117
+
+// one admissible block for the ACA kernel
+// Row i of the block is the i-th smallest point id of the row node (the
+// order of index_of_points, HDD_clusters.hpp:112-124), found at tree
+// position ord[row0 + i] of the block's level; columns likewise.
+struct AcaBlock {
+    int64_t row0, col0;   // first tree position of the row / column node
+    int64_t scratch;      // offset (doubles) of this block's scratch in the pool
+    const int32_t *ord;   // ascending-id order map of the block's level
+    int32_t m, n;
+    int32_t cap;          // rank capacity of the scratch
+    int32_t id;           // block id
+    uint64_t seed;        // retry stream (replaces rand(), LowRank_matrix.hpp:175,274)
+};
+
+// where a block's factors go in the operator (fill pass)
+struct FillDest {
+    int64_t up_base;      // U panel of the row node (elements)
+    int64_t vp_base;      // V panel of the column node (elements)
+    int32_t R, coff;      // U panel width and this block's first column
+    int32_t T, voff;      // V panel width and this block's first column
+};
+
+// a block stored dense (LowRank_matrix.hpp:348-363)
+struct DenseBlock {
+    int64_t row0, col0, up_base;
+    int32_t m, n, R, coff;
+};
+
+// one leaf's near-field panel (HDD_node.hpp:136-144)
+struct NearLeaf {
+    int64_t np_base;      // panel base (elements), nc x ncols column-major
+    int64_t row0;         // first tree position of the leaf
+    int64_t colsrc;       // offset into the column-source table
+    int32_t nc, ncols;
+};
+
+__host__ __device__ inline int64_t aca_icap(int cap) { return (int64_t)cap + 8; }
+
+// doubles of scratch one block needs: U, V, row, col, used flags, pivot lists
+__host__ __device__ inline int64_t aca_scratch_doubles(int m, int n, int cap) {
+    int64_t mn = (int64_t)m + n;
+    return mn * cap + mn + (mn + 7) / 8 + (3 * aca_icap(cap) + 1) / 2 + 2;
+}
+
+// ---- matvec work ----
+
+// V-panel projection tile: t[cols] (+)= VP[rows j0..j1, c0..c0+w)^T x[b0+j0..]
+struct ProjItem {
+    int64_t vp;           // element offset of (row j0, column c0) in the V arena
+    int64_t x0;           // tree position matching row j0
+    int64_t out;          // slot: >= 0 partial-sum slot, < 0 direct (-(tile index) - 1)
+    int32_t ld;           // panel width T_b
+    int32_t rows;
+    int32_t w;            // tile width (<= 64)
+    int32_t tcol;         // first global V column of the tile (into tidx)
+};
+
+// reduction of partial projection sums of one tile
+struct ProjReduce {
+    int64_t slot0;
+    int32_t nchunks, w, tcol, pad;
+};
+
+// dense-fallback block: t[toff..toff+n) = x[x0..x0+n)
+struct TCopy {
+    int64_t toff, x0;
+    int32_t n, pad;
+};
+
+// one row tile of one leaf for the panel kernel
+struct PanelItem {
+    int64_t y0;           // first tree position of the tile's rows
+    int64_t seg0;         // first segment
+    int32_t nseg;
+    int32_t h;            // rows in this tile (<= 64)
+    int32_t ld;           // leaf size (column stride of every segment)
+    int32_t r0;           // row offset of the tile inside the leaf
+};
+
+// one contiguous ld x len column-major matrix times a slice of x or t
+struct Segment {
+    int64_t mat;          // element offset in the arena (row 0 of the leaf)
+    int64_t vec;          // < 0: x[-vec-1 ..]; >= 0: t[vec ..]
+    int32_t len;
+    int32_t arena;        // 0 near-field arena, 1 U arena
+};
+
+}  // namespace hodlr

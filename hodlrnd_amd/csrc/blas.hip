@@ -1,0 +1,140 @@
+// blas.hip -- FP64 vector kernels for the device GMRES (GMRES.hpp:160-303).
+//
+// Arnoldi uses classical Gram-Schmidt with one re-orthogonalisation (CGS2):
+// two multi-dot passes h = V^T w and two updates w -= V h per iteration
+// instead of the reference's i+1 dependent dot/axpy pairs (modified
+// Gram-Schmidt, GMRES.hpp:236-240).  Dot products reduce in a fixed order:
+// per-chunk workgroup sums, then one pass over the chunks.
+#include "device_common.h"
+
+namespace hodlr {
+
+constexpr int kMaxBasis = 128;
+
+// partial[p * nv + k] = sum over chunk p of V[k * ldv + i] * w[i]
+__global__ __launch_bounds__(256) void mdot_kernel(const double *__restrict__ V, int64_t ldv, int nv,
+                                                   const double *__restrict__ w, int64_t n, int64_t chunk,
+                                                   double *__restrict__ partial) {
+    __shared__ double red[4][kMaxBasis];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t b = (int64_t)blockIdx.x * chunk, e = min(n, b + chunk);
+    for (int k = 0; k < nv; k++) {
+        const double *vk = V + (int64_t)k * ldv;
+        double s = 0.0;
+        for (int64_t i = b + tid; i < e; i += 256) s = fma(vk[i], w[i], s);
+        s = wave_sum(s);
+        if (lane == 0) red[wid][k] = s;
+    }
+    __syncthreads();
+    for (int k = tid; k < nv; k += 256)
+        partial[(int64_t)blockIdx.x * nv + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+}
+
+__global__ void mdot_finish_kernel(const double *__restrict__ partial, int np, int nv,
+                                   double *__restrict__ out, int accumulate) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nv) return;
+    double s = 0.0;
+    for (int p = 0; p < np; p++) s += partial[(int64_t)p * nv + k];
+    out[k] = accumulate ? out[k] + s : s;
+}
+
+// w[i] += sign * sum_k h[k] V[k * ldv + i]
+__global__ __launch_bounds__(256) void gemv_kernel(const double *__restrict__ V, int64_t ldv, int nv,
+                                                   const double *__restrict__ h, double sign,
+                                                   double *__restrict__ w, int64_t n) {
+    __shared__ double hs[kMaxBasis];
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) hs[k] = sign * h[k];
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double acc = w[i];
+        for (int k = 0; k < nv; k++) acc = fma(hs[k], V[(int64_t)k * ldv + i], acc);
+        w[i] = acc;
+    }
+}
+
+__global__ void scale_kernel(const double *__restrict__ src, double alpha, double *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] * alpha;
+}
+
+__global__ void sub_kernel(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c,
+                           int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c[i] = a[i] - b[i];
+}
+
+__global__ void add_kernel(double *__restrict__ a, const double *__restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = a[i] + b[i];
+}
+
+__global__ void to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+__global__ void to_f64_kernel(const float *__restrict__ a, double *__restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+// ---- launchers ----
+
+static inline unsigned grid_for(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 8192) g = 8192;
+    return (unsigned)g;
+}
+
+int64_t mdot_chunks(int64_t n) {
+    int64_t p = (n + 4095) / 4096;
+    if (p > 1024) p = 1024;
+    if (p < 1) p = 1;
+    return p;
+}
+
+hipError_t launch_mdot(const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *partial,
+                       double *out, int accumulate, hipStream_t s) {
+    if (nv > kMaxBasis) return hipErrorInvalidValue;
+    const int64_t np = mdot_chunks(n);
+    const int64_t chunk = (n + np - 1) / np;
+    mdot_kernel<<<dim3((unsigned)np), 256, 0, s>>>(V, ldv, nv, w, n, chunk, partial);
+    mdot_finish_kernel<<<dim3((unsigned)((nv + 63) / 64)), 64, 0, s>>>(partial, (int)np, nv, out, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemv(const double *V, int64_t ldv, int nv, const double *h, double sign, double *w, int64_t n,
+                       hipStream_t s) {
+    if (nv > kMaxBasis) return hipErrorInvalidValue;
+    gemv_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(V, ldv, nv, h, sign, w, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(const double *src, double alpha, double *dst, int64_t n, hipStream_t s) {
+    scale_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(src, alpha, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub(const double *a, const double *b, double *c, int64_t n, hipStream_t s) {
+    sub_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(a, b, c, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add(double *a, const double *b, int64_t n, hipStream_t s) {
+    add_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32(const double *a, float *b, int64_t n, hipStream_t s) {
+    to_f32_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f64(const float *a, double *b, int64_t n, hipStream_t s) {
+    to_f64_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(a, b, n);
+    return hipGetLastError();
+}
+
+}  // namespace hodlr

@@ -1,0 +1,368 @@
+// capi.cpp -- extern "C" entry points of libhodlrnd (include/hodlrnd.h).
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "op.h"
+#include "solver.h"
+
+using namespace hodlr;
+
+namespace {
+
+template <class F>
+int guard(hodlr_ctx *ctx, F &&f) {
+    try {
+        f();
+        return HODLR_OK;
+    } catch (const HipError &e) {
+        if (ctx) ctx->last_error = std::string(hipGetErrorString(e.code)) + " in " + e.where;
+        return HODLR_E_DEVICE;
+    } catch (const ArgError &e) {
+        if (ctx) ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return HODLR_E_NOMEM;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->last_error = e.what();
+        return HODLR_E_STATE;
+    }
+}
+
+int padded_rhs(int nrhs) {
+    static const int ks[] = {1, 2, 4, 8, 16, 32};
+    for (int k : ks)
+        if (k >= nrhs) return k;
+    return -1;
+}
+
+void set_device(hodlr_ctx *ctx) {
+    if (ctx->device < 0) throw ArgError{HODLR_E_STATE, "host-only context (device -1) cannot run device work"};
+    HIP_CHECK(hipSetDevice(ctx->device));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hodlr_version(void) { return "hodlrnd-mi355x 0.1 (gfx950)"; }
+
+int hodlr_ctx_create(int device, hodlr_ctx **out) {
+    if (!out) return HODLR_E_ARG;
+    *out = nullptr;
+    auto *ctx = new (std::nothrow) hodlr_ctx;
+    if (!ctx) return HODLR_E_NOMEM;
+    ctx->device = device;
+    int rc = guard(ctx, [&] {
+        if (device == -1) return;  // host-only context: trees and orderings, no device work
+        int ndev = 0;
+        HIP_CHECK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw ArgError{HODLR_E_ARG, "no such device"};
+        HIP_CHECK(hipSetDevice(device));
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    });
+    if (rc != HODLR_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return HODLR_OK;
+}
+
+void hodlr_ctx_destroy(hodlr_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->stream && ctx->device >= 0) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *hodlr_last_error(hodlr_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+int hodlr_tree_build(hodlr_ctx *ctx, int ndim, int64_t n, const double *pts, const double *lo,
+                     const double *hi, int nmax, int dprime, hodlr_tree **out) {
+    if (!ctx || !out) return HODLR_E_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        if (ndim < 1 || ndim > kMaxDim) throw ArgError{HODLR_E_ARG, "ndim must be in 1..8"};
+        if (n < 0 || (n > 0 && !pts) || !lo || !hi) throw ArgError{HODLR_E_ARG, "null points or box"};
+        if (nmax < 1) throw ArgError{HODLR_E_ARG, "nmax must be positive"};
+        if (n > INT32_MAX) throw ArgError{HODLR_E_ARG, "at most 2^31-1 points"};
+        auto *t = new hodlr_tree;
+        t->ctx = ctx;
+        t->ndim = ndim;
+        t->n = n;
+        t->nmax = nmax;
+        t->dprime = dprime;
+        t->pts.assign(pts, pts + n * ndim);
+        t->lo.assign(lo, lo + ndim);
+        t->hi.assign(hi, hi + ndim);
+        try {
+            build_tree(*t);
+        } catch (...) {
+            delete t;
+            throw;
+        }
+        *out = t;
+    });
+}
+
+int hodlr_tree_get_info(hodlr_tree *t, hodlr_tree_info *info) {
+    if (!t || !info) return HODLR_E_ARG;
+    std::memset(info, 0, sizeof(*info));
+    info->nlevels = t->nlevels;
+    info->ndim = t->ndim;
+    info->npoints = t->n;
+    info->nnodes = t->level_off.back();
+    const int L = t->nlevels - 1;
+    info->nleaves = t->level_size(L);
+    for (int64_t j = 0; j < info->nleaves; j++) {
+        const int64_t g = t->node(L, j);
+        if (t->count[g] > info->max_leaf) info->max_leaf = t->count[g];
+        if (!t->count[g]) continue;
+        for (int64_t a = t->nbr_ptr[g]; a < t->nbr_ptr[g + 1]; a++)
+            if (t->count[t->node(L, t->nbr[a])]) info->n_near++;
+    }
+    for (int l = 0; l <= L; l++)
+        for (int64_t j = 0; j < t->level_size(l); j++) {
+            const int64_t g = t->node(l, j);
+            if (!t->count[g]) continue;
+            for (int64_t a = t->il_ptr[g]; a < t->il_ptr[g + 1]; a++)
+                if (t->count[t->node(l, t->il[a])]) info->n_admissible++;
+        }
+    return HODLR_OK;
+}
+
+int hodlr_tree_perm(hodlr_tree *t, int64_t *perm) {
+    if (!t || !perm) return HODLR_E_ARG;
+    std::memcpy(perm, t->perm.data(), sizeof(int64_t) * t->n);
+    return HODLR_OK;
+}
+
+int hodlr_tree_nodes(hodlr_tree *t, int64_t *level_offset, int64_t *start, int64_t *count) {
+    if (!t) return HODLR_E_ARG;
+    if (level_offset) std::memcpy(level_offset, t->level_off.data(), sizeof(int64_t) * t->level_off.size());
+    if (start) std::memcpy(start, t->start.data(), sizeof(int64_t) * t->start.size());
+    if (count) std::memcpy(count, t->count.data(), sizeof(int64_t) * t->count.size());
+    return HODLR_OK;
+}
+
+int hodlr_tree_lists(hodlr_tree *t, int which, int64_t *ptr, int64_t *idx) {
+    if (!t || !ptr) return HODLR_E_ARG;
+    const auto &p = which ? t->il_ptr : t->nbr_ptr;
+    const auto &v = which ? t->il : t->nbr;
+    std::memcpy(ptr, p.data(), sizeof(int64_t) * p.size());
+    if (idx && !v.empty()) std::memcpy(idx, v.data(), sizeof(int64_t) * v.size());
+    return HODLR_OK;
+}
+
+void hodlr_tree_destroy(hodlr_tree *t) { delete t; }
+
+int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k, double eps_aca, int prec_flags,
+                      hodlr_op **out) {
+    if (!ctx || !t || !k || !out) return HODLR_E_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        if (k->ndim != t->ndim) throw ArgError{HODLR_E_ARG, "kernel ndim differs from the tree's"};
+        if (k->family < 0 || k->family > 2) throw ArgError{HODLR_E_ARG, "unknown kernel family"};
+        if (!(eps_aca > 0.0 && eps_aca < 1.0)) throw ArgError{HODLR_E_ARG, "eps_aca must be in (0, 1)"};
+        set_device(ctx);
+        auto *op = new hodlr_op;
+        op->ctx = ctx;
+        op->tree = t;
+        op->kd = *k;
+        op->eps = eps_aca;
+        op->prec_flags = prec_flags | HODLR_PREC_F64;
+        try {
+            assemble(*op);
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        op->kd.weight = nullptr;  // the caller's array is not referenced after assembly
+        *out = op;
+    });
+}
+
+int hodlr_op_get_stats(hodlr_op *op, hodlr_op_stats *s) {
+    if (!op || !s) return HODLR_E_ARG;
+    *s = op->stats;
+    return HODLR_OK;
+}
+
+int hodlr_op_block_ranks(hodlr_op *op, int32_t *rank, int32_t *dense, int64_t cap) {
+    if (!op) return HODLR_E_ARG;
+    if (cap < op->nblk) return (int)HODLR_E_ARG;
+    for (int64_t b = 0; b < op->nblk; b++) {
+        const bool dn = op->flags[b] & 1;
+        if (rank) rank[b] = dn ? std::min(op->b_m[b], op->b_n[b]) : op->rank[b];
+        if (dense) dense[b] = dn;
+    }
+    return HODLR_OK;
+}
+
+void hodlr_op_destroy(hodlr_op *op) {
+    if (!op) return;
+    if (op->ctx->device >= 0) (void)hipSetDevice(op->ctx->device);
+    delete op;
+}
+
+int hodlr_to_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst) {
+    if (!t || !src || !dst || nrhs < 1) return HODLR_E_ARG;
+    for (int64_t p = 0; p < t->n; p++)
+        for (int c = 0; c < nrhs; c++) dst[p * nrhs + c] = src[(int64_t)c * t->n + t->perm[p]];
+    return HODLR_OK;
+}
+
+int hodlr_from_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst) {
+    if (!t || !src || !dst || nrhs < 1) return HODLR_E_ARG;
+    for (int64_t p = 0; p < t->n; p++)
+        for (int c = 0; c < nrhs; c++) dst[(int64_t)c * t->n + t->perm[p]] = src[p * nrhs + c];
+    return HODLR_OK;
+}
+
+int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y) {
+    if (!op || !x || !y || nrhs < 1) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        const int K = padded_rhs(nrhs);
+        if (K < 0 || (prec == HODLR_PREC_F64 && K > 16)) throw ArgError{HODLR_E_ARG, "too many right-hand sides"};
+        if (prec != HODLR_PREC_F64 && prec != HODLR_PREC_F32) throw ArgError{HODLR_E_ARG, "bad precision"};
+        set_device(op->ctx);
+        const int64_t n = op->N;
+        hipStream_t s = op->ctx->stream;
+        std::vector<double> xt((size_t)n * K, 0.0), yt((size_t)n * K);
+        const auto &perm = op->tree->perm;
+        for (int64_t p = 0; p < n; p++)
+            for (int c = 0; c < nrhs; c++) xt[p * K + c] = x[(int64_t)c * n + perm[p]];
+        DevBuf dx, dy;
+        if (prec == HODLR_PREC_F64) {
+            upload(dx, xt);
+            dy.alloc(sizeof(double) * n * K);
+            op->matvec_dev(prec, K, dx.p, dy.p, s);
+            HIP_CHECK(hipStreamSynchronize(s));
+            HIP_CHECK(hipMemcpy(yt.data(), dy.p, sizeof(double) * n * K, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> xf(xt.begin(), xt.end()), yf((size_t)n * K);
+            upload(dx, xf);
+            dy.alloc(sizeof(float) * n * K);
+            op->matvec_dev(prec, K, dx.p, dy.p, s);
+            HIP_CHECK(hipStreamSynchronize(s));
+            HIP_CHECK(hipMemcpy(yf.data(), dy.p, sizeof(float) * n * K, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < yf.size(); i++) yt[i] = yf[i];
+        }
+        for (int64_t p = 0; p < n; p++)
+            for (int c = 0; c < nrhs; c++) y[(int64_t)c * n + perm[p]] = yt[p * K + c];
+    });
+}
+
+int hodlr_matvec_device(hodlr_op *op, int prec, int nrhs, const void *x, void *y, void *stream) {
+    if (!op || !x || !y) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        if (padded_rhs(nrhs) != nrhs) throw ArgError{HODLR_E_ARG, "nrhs must be 1, 2, 4, 8, 16 or 32"};
+        if (prec == HODLR_PREC_F64 && nrhs > 16) throw ArgError{HODLR_E_ARG, "FP64 supports at most 16 RHS"};
+        if (prec != HODLR_PREC_F64 && prec != HODLR_PREC_F32) throw ArgError{HODLR_E_ARG, "bad precision"};
+        set_device(op->ctx);
+        op->matvec_dev(prec, nrhs, x, y, stream ? static_cast<hipStream_t>(stream) : op->ctx->stream);
+    });
+}
+
+int hodlr_gmres_device(hodlr_op *op, const double *b, double *x, const hodlr_gmres_params *params,
+                       hodlr_gmres_info *info) {
+    if (!op || !b || !x || !params || !info) return HODLR_E_ARG;
+    int rc = guard(op->ctx, [&] {
+        set_device(op->ctx);
+        solve(*op, b, x, *params, *info);
+    });
+    if (rc == HODLR_OK && !info->converged) return HODLR_E_NOCONV;
+    return rc;
+}
+
+int hodlr_gmres(hodlr_op *op, const double *b, double *x, const hodlr_gmres_params *params,
+                hodlr_gmres_info *info) {
+    if (!op || !b || !x || !params || !info) return HODLR_E_ARG;
+    int rc = guard(op->ctx, [&] {
+        set_device(op->ctx);
+        const int64_t n = op->N;
+        std::vector<double> bt(n), xt(n);
+        const auto &perm = op->tree->perm;
+        for (int64_t p = 0; p < n; p++) {
+            bt[p] = b[perm[p]];
+            xt[p] = x[perm[p]];
+        }
+        DevBuf db, dx;
+        upload(db, bt);
+        upload(dx, xt);
+        solve(*op, db.as<double>(), dx.as<double>(), *params, *info);
+        HIP_CHECK(hipMemcpy(xt.data(), dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < n; p++) x[perm[p]] = xt[p];
+    });
+    if (rc == HODLR_OK && !info->converged) return HODLR_E_NOCONV;
+    return rc;
+}
+
+int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes) {
+    if (!op || !bytes) return HODLR_E_ARG;
+    const double s = prec == HODLR_PREC_F32 ? 4.0 : 8.0, k = nrhs, n = (double)op->N;
+    bytes[0] = s * (op->e_near + op->e_far) + 2.0 * k * s * n;
+    bytes[1] = s * (double)op->n_vp + k * s * (n + (double)op->n_t);
+    bytes[2] = s * (double)(op->n_np + op->n_up) + k * s * (n + (double)op->n_t) + k * s * n;
+    return HODLR_OK;
+}
+
+int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms) {
+    if (!op || !ms || reps < 1) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        if (padded_rhs(nrhs) != nrhs) throw ArgError{HODLR_E_ARG, "nrhs must be 1, 2, 4, 8, 16 or 32"};
+        set_device(op->ctx);
+        hipStream_t s = op->ctx->stream;
+        const int64_t n = op->N;
+        const size_t es = prec == HODLR_PREC_F32 ? sizeof(float) : sizeof(double);
+        DevBuf dx, dy;
+        dx.alloc(es * n * nrhs);
+        dy.alloc(es * n * nrhs);
+        HIP_CHECK(hipMemset(dx.p, 0, es * n * nrhs));
+        std::vector<hipEvent_t> ev((size_t)3 * reps);
+        for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+        op->matvec_dev(prec, nrhs, dx.p, dy.p, s);  // warm-up
+        for (int r = 0; r < reps; r++) op->matvec_dev(prec, nrhs, dx.p, dy.p, s, &ev[(size_t)3 * r]);
+        HIP_CHECK(hipStreamSynchronize(s));
+        double tot = 0, a = 0, b = 0;
+        for (int r = 0; r < reps; r++) {
+            float t01 = 0, t12 = 0;
+            HIP_CHECK(hipEventElapsedTime(&t01, ev[3 * r], ev[3 * r + 1]));
+            HIP_CHECK(hipEventElapsedTime(&t12, ev[3 * r + 1], ev[3 * r + 2]));
+            a += t01;
+            b += t12;
+            tot += t01 + t12;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        ms[0] = tot / reps;
+        ms[1] = a / reps;
+        ms[2] = b / reps;
+    });
+}
+
+int hodlr_device_alloc(hodlr_ctx *ctx, int64_t bytes, void **ptr) {
+    if (!ctx || !ptr || bytes < 0) return HODLR_E_ARG;
+    return guard(ctx, [&] {
+        set_device(ctx);
+        if (hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 8) != hipSuccess)
+            throw ArgError{HODLR_E_NOMEM, "hipMalloc failed"};
+    });
+}
+
+int hodlr_device_free(hodlr_ctx *ctx, void *ptr) {
+    if (!ctx) return HODLR_E_ARG;
+    return guard(ctx, [&] { HIP_CHECK(hipFree(ptr)); });
+}
+
+int hodlr_copy_to_device(hodlr_ctx *ctx, void *dst, const void *src, int64_t bytes) {
+    if (!ctx || !dst || !src || bytes < 0) return HODLR_E_ARG;
+    return guard(ctx, [&] { HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice)); });
+}
+
+int hodlr_copy_to_host(hodlr_ctx *ctx, void *dst, const void *src, int64_t bytes) {
+    if (!ctx || !dst || !src || bytes < 0) return HODLR_E_ARG;
+    return guard(ctx, [&] { HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost)); });
+}
+
+}  // extern "C"

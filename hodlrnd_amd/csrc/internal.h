@@ -1,0 +1,93 @@
+// internal.h -- shared declarations of libhodlrnd (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hodlrnd.h"
+
+namespace hodlr {
+
+constexpr int kMaxDim = 8;
+
+struct HipError {
+    hipError_t code;
+    std::string where;
+};
+struct ArgError {
+    int code;
+    std::string msg;
+};
+
+#define HIP_CHECK(expr)                                                             \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            throw ::hodlr::HipError{e_, std::string(#expr) + " @" + __FILE__ + ":" + \
+                                            std::to_string(__LINE__)};              \
+    } while (0)
+
+// Device buffer owned by an object; freed in the destructor.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void alloc(size_t nb) {
+        release();
+        if (nb == 0) nb = 8;
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw ArgError{HODLR_E_NOMEM, "hipMalloc of " + std::to_string(nb) + " bytes failed"};
+        }
+        bytes = nb;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <class T>
+void upload(DevBuf &b, const std::vector<T> &v) {
+    b.alloc(v.size() * sizeof(T));
+    if (!v.empty()) HIP_CHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace hodlr
+
+struct hodlr_ctx {
+    int device = 0;
+    std::string last_error;
+    hipStream_t stream = nullptr;
+};
+
+// Uniform-depth 2^d tree (HDD_tree.hpp:23-74).  Nodes are numbered level by
+// level; node g = level_off[l] + j is node j of level l, whose children are
+// nodes 2^d j .. 2^d j + 2^d - 1 of level l + 1.  Every node owns the range
+// [start, start + count) of the leaf-order permutation.
+struct hodlr_tree {
+    hodlr_ctx *ctx = nullptr;
+    int ndim = 0, nlevels = 0, nmax = 0, dprime = 0;
+    int64_t n = 0;
+    std::vector<double> pts;          // point order, n x ndim
+    std::vector<int64_t> perm;        // tree position -> point id
+    std::vector<int64_t> level_off;   // nlevels + 1
+    std::vector<int64_t> start, count;
+    std::vector<double> lo, hi, len, centre;  // per node: box, max side, centre
+    std::vector<int64_t> nbr_ptr, nbr, il_ptr, il;
+    int64_t node(int l, int64_t j) const { return level_off[l] + j; }
+    int64_t level_size(int l) const { return level_off[l + 1] - level_off[l]; }
+};
+
+namespace hodlr {
+void build_tree(hodlr_tree &t);
+}

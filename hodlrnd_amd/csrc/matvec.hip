@@ -1,0 +1,207 @@
+// matvec.hip -- the fast HODLR matvec (gfx950), FP64 or FP32, 1..32 RHS.
+//
+// The reference applies every node's blocks one GEMV at a time
+// (HDD_tree.hpp:86-107, HDD_node.hpp:223-254, LowRank_matrix.hpp:40-63).
+// Here the operator is stored in three streaming arenas and applied by
+// three launches, each reading its arena exactly once:
+//   proj:   t = V^T x for every compressed block.  The V factors of all
+//           blocks that share a column node b form one row-major panel
+//           (n_b x T_b); a wave owns a 64-column tile and walks its rows, so
+//           each step is one coalesced 64-element read and x_b[j] is a
+//           broadcast.  Tall tiles are cut into row chunks whose partial
+//           sums are added by a second small pass (deterministic order).
+//   tcopy:  dense-fallback blocks take t = x_b (their U panel holds K).
+//   panel:  y = sum over segments of M x_seg, one workgroup per leaf row
+//           tile.  A leaf's segments are its near-field blocks (self and
+//           neighbours) and one slice of the U panel of each ancestor,
+//           all stored column-major with leading dimension = leaf size, so
+//           G = 64/h column groups of h lanes read G*h contiguous elements
+//           per step.  Each leaf row is written once (no atomics).
+#include "aca_layout.h"
+#include "device_common.h"
+
+namespace hodlr {
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ items, int64_t nitems,
+                                                   const T *__restrict__ VP, const T *__restrict__ x,
+                                                   T *__restrict__ t, T *__restrict__ part,
+                                                   const int64_t *__restrict__ tidx) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int64_t it = xcd_remap(blockIdx.x, gridDim.x) * 4 + wid;
+    if (it >= nitems) return;
+    const ProjItem P = items[it];
+    if (lane >= P.w) return;
+    T acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = T(0);
+    const T *vp = VP + P.vp + lane;
+    const T *xv = x + P.x0 * K;
+    const int64_t ld = P.ld;
+    int j = 0;
+    for (; j + 4 <= P.rows; j += 4) {
+        const T a0 = vp[(j + 0) * ld], a1 = vp[(j + 1) * ld];
+        const T a2 = vp[(j + 2) * ld], a3 = vp[(j + 3) * ld];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            acc[k] = fma(a0, xv[(j + 0) * K + k], acc[k]);
+            acc[k] = fma(a1, xv[(j + 1) * K + k], acc[k]);
+            acc[k] = fma(a2, xv[(j + 2) * K + k], acc[k]);
+            acc[k] = fma(a3, xv[(j + 3) * K + k], acc[k]);
+        }
+    }
+    for (; j < P.rows; j++) {
+        const T a = vp[j * ld];
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] = fma(a, xv[j * K + k], acc[k]);
+    }
+    if (P.out >= 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) part[(P.out * kWave + lane) * K + k] = acc[k];
+    } else {
+        const int64_t ti = tidx[P.tcol + lane];
+#pragma unroll
+        for (int k = 0; k < K; k++) t[ti * K + k] = acc[k];
+    }
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(64) void proj_reduce_kernel(const ProjReduce *__restrict__ items,
+                                                         const T *__restrict__ part, T *__restrict__ t,
+                                                         const int64_t *__restrict__ tidx) {
+    const ProjReduce R = items[blockIdx.x];
+    const int lane = threadIdx.x;
+    if (lane >= R.w) return;
+    T acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = T(0);
+    for (int c = 0; c < R.nchunks; c++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += part[((R.slot0 + c) * kWave + lane) * K + k];
+    const int64_t ti = tidx[R.tcol + lane];
+#pragma unroll
+    for (int k = 0; k < K; k++) t[ti * K + k] = acc[k];
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void tcopy_kernel(const TCopy *__restrict__ items, int64_t nitems,
+                                                    const T *__restrict__ x, T *__restrict__ t) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int64_t it = (int64_t)blockIdx.x * 4 + wid;
+    if (it >= nitems) return;
+    const TCopy C = items[it];
+    const int64_t tot = (int64_t)C.n * K;
+    for (int64_t e = lane; e < tot; e += kWave) t[C.toff * K + e] = x[C.x0 * K + e];
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void panel_kernel(const PanelItem *__restrict__ items,
+                                                    const Segment *__restrict__ segs,
+                                                    const T *__restrict__ NP, const T *__restrict__ UP,
+                                                    const T *__restrict__ x, const T *__restrict__ t,
+                                                    T *__restrict__ y) {
+    constexpr int NW = 4;
+    __shared__ T red[NW][kWave][K];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const int h = P.h;
+    const int G = kWave / h;
+    const int o = lane % h, g = lane / h;
+    const bool active = g < G;
+    const int gg = wid * G + g, GG = NW * G;
+    const int64_t ld = P.ld;
+    T acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = T(0);
+    for (int s = 0; s < P.nseg; s++) {
+        const Segment S = segs[P.seg0 + s];
+        const T *M = (S.arena ? UP : NP) + S.mat + P.r0 + o;
+        const T *v = S.vec < 0 ? x + (-S.vec - 1) * K : t + S.vec * K;
+        if (!active) continue;
+        int q = gg;
+        const int len = S.len;
+        for (; q + 3 * GG < len; q += 4 * GG) {
+            const T a0 = M[(int64_t)q * ld];
+            const T a1 = M[(int64_t)(q + GG) * ld];
+            const T a2 = M[(int64_t)(q + 2 * GG) * ld];
+            const T a3 = M[(int64_t)(q + 3 * GG) * ld];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                acc[k] = fma(a0, v[(int64_t)q * K + k], acc[k]);
+                acc[k] = fma(a1, v[(int64_t)(q + GG) * K + k], acc[k]);
+                acc[k] = fma(a2, v[(int64_t)(q + 2 * GG) * K + k], acc[k]);
+                acc[k] = fma(a3, v[(int64_t)(q + 3 * GG) * K + k], acc[k]);
+            }
+        }
+        for (; q < len; q += GG) {
+            const T a = M[(int64_t)q * ld];
+#pragma unroll
+            for (int k = 0; k < K; k++) acc[k] = fma(a, v[(int64_t)q * K + k], acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) red[wid][lane][k] = active ? acc[k] : T(0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < h * K; e += blockDim.x) {
+        const int row = e / K, k = e % K;
+        T sum = T(0);
+        for (int w = 0; w < NW; w++)
+            for (int gi = 0; gi < G; gi++) sum += red[w][gi * h + row][k];
+        y[(P.y0 + row) * K + k] = sum;
+    }
+}
+
+template <typename T, int K>
+hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
+                           const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
+                           const Segment *segs, const T *VP, const T *NP, const T *UP,
+                           const T *x, T *t, T *part, const int64_t *tidx, T *y,
+                           hipEvent_t *ev, hipStream_t s) {
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (npi) proj_kernel<T, K><<<dim3((unsigned)((npi + 3) / 4)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+    if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 64, 0, s>>>(pr, part, t, tidx);
+    if (ntc) tcopy_kernel<T, K><<<dim3((unsigned)((ntc + 3) / 4)), 256, 0, s>>>(tc, ntc, x, t);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    if (npa) panel_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
+                         const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
+                         const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
+                         T *t, T *part, const int64_t *tidx, T *y, hipEvent_t *ev, hipStream_t s) {
+#define HODLR_MV(KK)                                                                                 \
+    case KK:                                                                                         \
+        return launch_matvec_k<T, KK>(pi, npi, pr, npr, tc, ntc, pa, npa, segs, VP, NP, UP, x, t, \
+                                      part, tidx, y, ev, s);
+    switch (nrhs) {
+        HODLR_MV(1)
+        HODLR_MV(2)
+        HODLR_MV(4)
+        HODLR_MV(8)
+        HODLR_MV(16)
+    case 32:  // the panel kernel's reduction buffer is 4*64*K*sizeof(T) bytes of LDS
+        if constexpr (sizeof(T) == 4)
+            return launch_matvec_k<T, 32>(pi, npi, pr, npr, tc, ntc, pa, npa, segs, VP, NP, UP, x, t,
+                                          part, tidx, y, ev, s);
+        return hipErrorInvalidValue;
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef HODLR_MV
+}
+
+template hipError_t launch_matvec<double>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
+                                          const TCopy *, int64_t, const PanelItem *, int64_t,
+                                          const Segment *, const double *, const double *, const double *,
+                                          const double *, double *, double *, const int64_t *, double *,
+                                          hipEvent_t *, hipStream_t);
+template hipError_t launch_matvec<float>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
+                                         const TCopy *, int64_t, const PanelItem *, int64_t,
+                                         const Segment *, const float *, const float *, const float *,
+                                         const float *, float *, float *, const int64_t *, float *,
+                                         hipEvent_t *, hipStream_t);
+
+}  // namespace hodlr

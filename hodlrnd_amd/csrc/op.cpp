@@ -1,0 +1,491 @@
+// op.cpp -- operator assembly and the matvec driver (host side).
+//
+// Assembly (HDD_tree.hpp:76-83, HDD_node.hpp:118-147) runs in four steps:
+//   1. enumerate the admissible blocks in the reference's order (level,
+//      node, interaction-list index), skipping empty nodes;
+//   2. ACA rank pass over every block (batched, chunked scratch);
+//   3. lay out the arenas from the exact ranks:
+//        near field  NP: per leaf an nc x (sum of neighbour sizes) panel,
+//        U panels    UP: per row node a the m_a x R_a matrix [U_ab1 | U_ab2 ..]
+//                        (dense-fallback blocks contribute K_ab), stored
+//                        leaf-blocked: each leaf's rows form a contiguous
+//                        nc x R_a column-major slice;
+//        V panels    VP: per column node b the n_b x T_b row-major matrix of
+//                        the V factors of every compressed block (., b);
+//   4. fill pass: near field, dense fallbacks, and the ACA again, writing
+//      factors into place.
+#include "op.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace hodlr {
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int aca_block_size(int m, int n) {
+    const int mx = m > n ? m : n;
+    if (mx <= 128) return 64;
+    if (mx <= 4096) return 256;
+    return 1024;
+}
+
+uint64_t block_seed(int level, int64_t row_node, int64_t col_node) {
+    uint64_t s = 0x243F6A8885A308D3ULL ^ ((uint64_t)(level + 1) * 0x9E3779B97F4A7C15ULL);
+    s ^= (uint64_t)row_node * 0xBF58476D1CE4E5B9ULL;
+    s ^= (uint64_t)col_node * 0x94D049BB133111EBULL;
+    return s;
+}
+
+struct AcaRun {
+    std::vector<AcaBlock> blocks;   // one size class
+    int bs = 64;
+};
+
+// Runs one ACA pass over `ids` (block ids) with per-block caps, chunked so
+// that the scratch pool stays within `budget` doubles.
+void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vector<int32_t> &cap, int fill,
+                  const KernelDev &K, DevBuf &pool, int64_t budget, DevBuf &d_rank, DevBuf &d_flags,
+                  const DevBuf *d_dest, hipStream_t s) {
+    const hodlr_tree &t = *op.tree;
+    // group by workgroup size, large blocks first inside a class
+    std::vector<int64_t> order(ids);
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        int sa = aca_block_size(op.b_m[a], op.b_n[a]), sb = aca_block_size(op.b_m[b], op.b_n[b]);
+        if (sa != sb) return sa > sb;
+        return (int64_t)op.b_m[a] + op.b_n[a] > (int64_t)op.b_m[b] + op.b_n[b];
+    });
+    DevBuf d_blocks;
+    size_t i = 0;
+    std::vector<AcaBlock> chunk;
+    while (i < order.size()) {
+        chunk.clear();
+        const int bs = aca_block_size(op.b_m[order[i]], op.b_n[order[i]]);
+        int64_t used = 0;
+        while (i < order.size()) {
+            const int64_t b = order[i];
+            if (aca_block_size(op.b_m[b], op.b_n[b]) != bs) break;
+            const int64_t need = aca_scratch_doubles(op.b_m[b], op.b_n[b], cap[b]);
+            if (!chunk.empty() && used + need > budget) break;
+            AcaBlock a;
+            a.row0 = t.start[op.b_row[b]];
+            a.col0 = t.start[op.b_col[b]];
+            a.scratch = used;
+            a.ord = op.ord.as<int32_t>() + (int64_t)op.b_level[b] * t.n;
+            a.m = op.b_m[b];
+            a.n = op.b_n[b];
+            a.cap = cap[b];
+            a.id = (int32_t)b;
+            a.seed = block_seed(op.b_level[b], op.b_row[b] - t.level_off[op.b_level[b]],
+                                op.b_col[b] - t.level_off[op.b_level[b]]);
+            chunk.push_back(a);
+            used += need;
+            i++;
+        }
+        if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double));
+        upload(d_blocks, chunk);
+        HIP_CHECK(launch_aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), K, op.eps, pool.as<double>(),
+                             d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
+                             d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
+                             op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(), s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+}
+
+}  // namespace
+
+void assemble(hodlr_op &op) {
+    const double t_start = now_s();
+    hodlr_tree &t = *op.tree;
+    hipStream_t s = op.ctx->stream;
+    const int d = t.ndim;
+    const int L = t.nlevels - 1;  // leaf level
+    const int64_t N = t.n;
+    op.N = N;
+
+    // ---- tree-order inputs ----
+    {
+        std::vector<double> X((size_t)N * d), W;
+        for (int64_t p = 0; p < N; p++)
+            for (int k = 0; k < d; k++) X[p * d + k] = t.pts[t.perm[p] * d + k];
+        upload(op.X, X);
+        if (op.kd.weight) {
+            W.resize(N);
+            for (int64_t p = 0; p < N; p++) W[p] = op.kd.weight[t.perm[p]];
+            upload(op.W, W);
+        }
+        std::vector<int64_t> ls(N > 0 ? N : 1, 0);
+        std::vector<int32_t> lc(N > 0 ? N : 1, 0);
+        for (int64_t j = 0; j < t.level_size(L); j++) {
+            const int64_t g = t.node(L, j);
+            for (int64_t p = t.start[g]; p < t.start[g] + t.count[g]; p++) {
+                ls[p] = t.start[g];
+                lc[p] = (int32_t)t.count[g];
+            }
+        }
+        upload(op.leaf_start, ls);
+        upload(op.leaf_cnt, lc);
+        // per level: each node's tree positions sorted by point id
+        std::vector<int32_t> ord((size_t)t.nlevels * (N > 0 ? N : 1));
+        for (int l = 0; l <= L; l++) {
+            int32_t *o = ord.data() + (size_t)l * N;
+            for (int64_t p = 0; p < N; p++) o[p] = (int32_t)p;
+            if (l == L) continue;  // leaves are already ascending
+            for (int64_t j = 0; j < t.level_size(l); j++) {
+                const int64_t g = t.node(l, j);
+                std::sort(o + t.start[g], o + t.start[g] + t.count[g],
+                          [&](int32_t a, int32_t b) { return t.perm[a] < t.perm[b]; });
+            }
+        }
+        upload(op.ord, ord);
+    }
+    KernelDev K;
+    K.family = op.kd.family;
+    K.ndim = d;
+    K.scale = op.kd.scale;
+    K.diag = op.kd.diag;
+    K.X = op.X.as<double>();
+    K.W = op.kd.weight ? op.W.as<double>() : nullptr;
+
+    // ---- 1. blocks in reference order ----
+    for (int l = 0; l <= L; l++)
+        for (int64_t j = 0; j < t.level_size(l); j++) {
+            const int64_t g = t.node(l, j);
+            if (!t.count[g]) continue;
+            for (int64_t a = t.il_ptr[g]; a < t.il_ptr[g + 1]; a++) {
+                const int64_t gb = t.node(l, t.il[a]);
+                if (!t.count[gb]) continue;
+                op.b_level.push_back(l);
+                op.b_row.push_back(g);
+                op.b_col.push_back(gb);
+                op.b_m.push_back((int32_t)t.count[g]);
+                op.b_n.push_back((int32_t)t.count[gb]);
+            }
+        }
+    op.nblk = (int64_t)op.b_level.size();
+    const int64_t nb = op.nblk;
+
+    // ---- 2. rank pass ----
+    const double t_rank0 = now_s();
+    DevBuf d_rank, d_flags, pool;
+    d_rank.alloc(sizeof(int32_t) * (nb + 1));
+    d_flags.alloc(sizeof(int32_t) * (nb + 1));
+    size_t free_b = 0, total_b = 0;
+    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const int64_t budget = std::max<int64_t>((int64_t)(free_b / 16 / sizeof(double)), (int64_t)1 << 24);
+    std::vector<int32_t> cap(nb);
+    std::vector<int64_t> todo(nb);
+    for (int64_t b = 0; b < nb; b++) {
+        const int mn = std::min(op.b_m[b], op.b_n[b]);
+        cap[b] = std::min(mn, 128);
+        todo[b] = b;
+    }
+    op.rank.assign(nb, 0);
+    op.flags.assign(nb, 0);
+    for (int round = 0; !todo.empty(); round++) {
+        run_aca_pass(op, todo, cap, 0, K, pool, budget, d_rank, d_flags, nullptr, s);
+        std::vector<int32_t> rk(nb), fl(nb);
+        if (nb) {
+            HIP_CHECK(hipMemcpy(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpy(fl.data(), d_flags.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+        }
+        std::vector<int64_t> again;
+        for (int64_t b : todo) {
+            op.rank[b] = rk[b];
+            op.flags[b] = fl[b];
+            if (fl[b] & 2) {  // rank budget exhausted: retry with a larger cap
+                const int mn = std::min(op.b_m[b], op.b_n[b]);
+                if (cap[b] >= mn) throw ArgError{HODLR_E_RANK, "ACA rank budget exhausted"};
+                cap[b] = std::min<int64_t>(mn, (int64_t)cap[b] * 4);
+                again.push_back(b);
+            }
+        }
+        todo.swap(again);
+    }
+    const double t_rank1 = now_s();
+
+    // ---- 3. layout ----
+    const int64_t nn = t.level_off.back();
+    std::vector<int64_t> R(nn, 0), Tb(nn, 0), up_base(nn, 0), t_base(nn, 0), vp_base(nn, 0), vcol_base(nn, 0);
+    std::vector<int32_t> width(nb), coff(nb), voff(nb);
+    std::vector<int64_t> first_blk(nn + 1, -1);
+    for (int64_t b = 0; b < nb; b++) {
+        const bool dense = op.flags[b] & 1;
+        width[b] = dense ? op.b_n[b] : op.rank[b];
+        const int64_t a = op.b_row[b];
+        coff[b] = (int32_t)R[a];
+        R[a] += width[b];
+        if (!dense) {
+            voff[b] = (int32_t)Tb[op.b_col[b]];
+            Tb[op.b_col[b]] += op.rank[b];
+        }
+    }
+    op.n_up = op.n_vp = op.n_t = 0;
+    int64_t nvcol = 0;
+    op.e_u = op.e_v = 0;
+    for (int64_t g = 0; g < nn; g++) {
+        up_base[g] = op.n_up;
+        op.n_up += t.count[g] * R[g];
+        t_base[g] = op.n_t;
+        op.n_t += R[g];
+        vp_base[g] = op.n_vp;
+        op.n_vp += t.count[g] * Tb[g];
+        vcol_base[g] = nvcol;
+        nvcol += Tb[g];
+    }
+    op.e_u = (double)op.n_up;
+    op.e_v = (double)op.n_vp;
+
+    // near field per non-empty leaf
+    std::vector<NearLeaf> near;
+    std::vector<int64_t> colsrc;
+    std::vector<Segment> segs;
+    std::vector<PanelItem> panel;
+    op.n_np = 0;
+    op.e_near = 0;
+    for (int64_t j = 0; j < t.level_size(L); j++) {
+        const int64_t g = t.node(L, j);
+        const int64_t nc = t.count[g];
+        if (!nc) continue;
+        NearLeaf nl;
+        nl.np_base = op.n_np;
+        nl.row0 = t.start[g];
+        nl.colsrc = (int64_t)colsrc.size();
+        nl.nc = (int32_t)nc;
+        const int64_t seg0 = (int64_t)segs.size();
+        int64_t ncols = 0;
+        auto add_near = [&](int64_t gb) {
+            const int64_t c = t.count[gb];
+            if (!c) return;
+            Segment sg;
+            sg.mat = op.n_np + ncols * nc;
+            sg.vec = -t.start[gb] - 1;
+            sg.len = (int32_t)c;
+            sg.arena = 0;
+            segs.push_back(sg);
+            for (int64_t p = t.start[gb]; p < t.start[gb] + c; p++) colsrc.push_back(p);
+            ncols += c;
+        };
+        add_near(g);
+        for (int64_t a = t.nbr_ptr[g]; a < t.nbr_ptr[g + 1]; a++) add_near(t.node(L, t.nbr[a]));
+        nl.ncols = (int32_t)ncols;
+        near.push_back(nl);
+        op.n_np += nc * ncols;
+        // far field: one U-panel slice per ancestor (and the leaf itself)
+        for (int l = 1; l <= L; l++) {
+            const int64_t ja = j >> (d * (L - l));
+            const int64_t ga = t.node(l, ja);
+            if (!R[ga]) continue;
+            Segment sg;
+            sg.mat = up_base[ga] + R[ga] * (t.start[g] - t.start[ga]);
+            sg.vec = t_base[ga];
+            sg.len = (int32_t)R[ga];
+            sg.arena = 1;
+            segs.push_back(sg);
+        }
+        const int64_t nseg = (int64_t)segs.size() - seg0;
+        for (int64_t r0 = 0; r0 < nc; r0 += 64) {
+            PanelItem pi;
+            pi.y0 = t.start[g] + r0;
+            pi.seg0 = seg0;
+            pi.nseg = (int32_t)nseg;
+            pi.h = (int32_t)std::min<int64_t>(64, nc - r0);
+            pi.ld = (int32_t)nc;
+            pi.r0 = (int32_t)r0;
+            panel.push_back(pi);
+        }
+    }
+    op.e_near = (double)op.n_np;
+    // root as a leaf (depth 0): the whole matrix is its near field, handled above
+
+    // projection tiles over the V panels; t index of every V column
+    std::vector<int64_t> tidx(nvcol > 0 ? nvcol : 1, 0);
+    std::vector<TCopy> tcopy;
+    std::vector<DenseBlock> dense;
+    std::vector<FillDest> dest(nb > 0 ? nb : 1);
+    op.t_lowrank = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        const int64_t a = op.b_row[b], c = op.b_col[b];
+        FillDest fd;
+        fd.up_base = up_base[a];
+        fd.R = (int32_t)R[a];
+        fd.coff = coff[b];
+        fd.vp_base = vp_base[c];
+        fd.T = (int32_t)Tb[c];
+        fd.voff = voff[b];
+        dest[b] = fd;
+        if (op.flags[b] & 1) {
+            TCopy tc;
+            tc.toff = t_base[a] + coff[b];
+            tc.x0 = t.start[c];
+            tc.n = op.b_n[b];
+            tc.pad = 0;
+            tcopy.push_back(tc);
+            DenseBlock db;
+            db.row0 = t.start[a];
+            db.col0 = t.start[c];
+            db.up_base = up_base[a];
+            db.m = op.b_m[b];
+            db.n = op.b_n[b];
+            db.R = (int32_t)R[a];
+            db.coff = coff[b];
+            dense.push_back(db);
+        } else {
+            for (int k = 0; k < op.rank[b]; k++) tidx[vcol_base[c] + voff[b] + k] = t_base[a] + coff[b] + k;
+            op.t_lowrank += op.rank[b];
+        }
+    }
+    std::vector<ProjItem> proj;
+    std::vector<ProjReduce> red;
+    const int64_t kChunk = 2048;
+    int64_t slots = 0;
+    for (int64_t g = 0; g < nn; g++) {
+        if (!Tb[g]) continue;
+        const int64_t nr = t.count[g];
+        for (int64_t c0 = 0; c0 < Tb[g]; c0 += 64) {
+            const int w = (int)std::min<int64_t>(64, Tb[g] - c0);
+            const int64_t nch = (nr + kChunk - 1) / kChunk;
+            if (nch > 1) {
+                ProjReduce pr;
+                pr.slot0 = slots;
+                pr.nchunks = (int32_t)nch;
+                pr.w = w;
+                pr.tcol = (int32_t)(vcol_base[g] + c0);
+                pr.pad = 0;
+                red.push_back(pr);
+            }
+            for (int64_t ch = 0; ch < nch; ch++) {
+                const int64_t j0 = ch * kChunk, j1 = std::min(nr, j0 + kChunk);
+                ProjItem it;
+                it.vp = vp_base[g] + j0 * Tb[g] + c0;
+                it.x0 = t.start[g] + j0;
+                it.ld = (int32_t)Tb[g];
+                it.rows = (int32_t)(j1 - j0);
+                it.w = w;
+                it.tcol = (int32_t)(vcol_base[g] + c0);
+                it.out = nch > 1 ? slots + ch : -1;
+                proj.push_back(it);
+            }
+            if (nch > 1) slots += nch;
+        }
+    }
+    if (nvcol > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many V columns for 32-bit tile offsets"};
+    op.n_slots = slots;
+
+    // ---- 4. allocate and fill ----
+    const double t_fill0 = now_s();
+    op.NP64.alloc(sizeof(double) * op.n_np);
+    op.UP64.alloc(sizeof(double) * op.n_up);
+    op.VP64.alloc(sizeof(double) * op.n_vp);
+    {
+        DevBuf d_near, d_colsrc, d_dense, d_dest;
+        upload(d_near, near);
+        upload(d_colsrc, colsrc);
+        upload(d_dense, dense);
+        upload(d_dest, dest);
+        HIP_CHECK(launch_near_fill(d_near.as<NearLeaf>(), (int64_t)near.size(), d_colsrc.as<int64_t>(), K,
+                                   op.NP64.as<double>(), s));
+        HIP_CHECK(launch_dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), K, op.leaf_start.as<int64_t>(),
+                                    op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        std::vector<int64_t> lowrank;
+        for (int64_t b = 0; b < nb; b++)
+            if (!(op.flags[b] & 1)) lowrank.push_back(b);
+        // the fill pass repeats the rank pass with the same caps
+        HIP_CHECK(hipMemset(d_rank.p, 0xff, sizeof(int32_t) * (nb + 1)));
+        run_aca_pass(op, lowrank, cap, 1, K, pool, budget, d_rank, d_flags, &d_dest, s);
+        std::vector<int32_t> rk(nb > 0 ? nb : 1);
+        if (nb) HIP_CHECK(hipMemcpy(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+        for (int64_t b : lowrank)
+            if (rk[b] != op.rank[b])
+                throw ArgError{HODLR_E_STATE, "ACA fill pass disagrees with the rank pass at block " +
+                                                  std::to_string(b)};
+    }
+    pool.release();
+    if (op.prec_flags & HODLR_PREC_F32) {
+        op.NP32.alloc(sizeof(float) * op.n_np);
+        op.UP32.alloc(sizeof(float) * op.n_up);
+        op.VP32.alloc(sizeof(float) * op.n_vp);
+        HIP_CHECK(launch_convert(op.NP64.as<double>(), op.NP32.as<float>(), op.n_np, s));
+        HIP_CHECK(launch_convert(op.UP64.as<double>(), op.UP32.as<float>(), op.n_up, s));
+        HIP_CHECK(launch_convert(op.VP64.as<double>(), op.VP32.as<float>(), op.n_vp, s));
+    }
+    upload(op.proj, proj);
+    upload(op.red, red);
+    upload(op.tcopy, tcopy);
+    upload(op.panel, panel);
+    upload(op.segs, segs);
+    upload(op.tidx, tidx);
+    op.n_proj = (int64_t)proj.size();
+    op.n_red = (int64_t)red.size();
+    op.n_tcopy = (int64_t)tcopy.size();
+    op.n_panel = (int64_t)panel.size();
+    op.n_seg = (int64_t)segs.size();
+    HIP_CHECK(hipStreamSynchronize(s));
+    const double t_end = now_s();
+
+    // ---- statistics (HDD_node.hpp:183-220, HDD_matrix.hpp:69-85) ----
+    hodlr_op_stats &st = op.stats;
+    std::memset(&st, 0, sizeof(st));
+    st.e_near = op.e_near;
+    st.n_flop_ref = op.e_near;
+    for (int64_t b = 0; b < nb; b++) {
+        const bool dn = op.flags[b] & 1;
+        const int64_t mn = std::min(op.b_m[b], op.b_n[b]);
+        const int64_t rr = dn ? mn : op.rank[b];
+        st.n_flop_ref += (double)rr * (double)(op.b_m[b] + op.b_n[b]);
+        st.max_rank = std::max<int64_t>(st.max_rank, rr);
+        st.e_far += dn ? (double)op.b_m[b] * op.b_n[b] : (double)op.rank[b] * (op.b_m[b] + op.b_n[b]);
+        st.n_dense += dn;
+    }
+    st.n_blocks = nb;
+    op.e_far = st.e_far;
+    st.bytes_f64 = 8.0 * (double)(op.n_np + op.n_up + op.n_vp);
+    st.bytes_f32 = (op.prec_flags & HODLR_PREC_F32) ? 4.0 * (double)(op.n_np + op.n_up + op.n_vp) : 0.0;
+    st.matvec_bytes_f64 = 8.0 * (st.e_near + st.e_far) + 2.0 * 8.0 * (double)N;
+    st.matvec_bytes_f32 = 4.0 * (st.e_near + st.e_far) + 2.0 * 4.0 * (double)N;
+    st.rank_seconds = t_rank1 - t_rank0;
+    st.fill_seconds = t_end - t_fill0;
+    st.assemble_seconds = t_end - t_start;
+}
+
+}  // namespace hodlr
+
+void hodlr_op::ensure_work(int prec, int nrhs) {
+    using namespace hodlr;
+    if (prec == HODLR_PREC_F64) {
+        if (cap_rhs64 >= nrhs) return;
+        t64.alloc(sizeof(double) * (size_t)(n_t > 0 ? n_t : 1) * nrhs);
+        part64.alloc(sizeof(double) * (size_t)(n_slots > 0 ? n_slots : 1) * 64 * nrhs);
+        cap_rhs64 = nrhs;
+    } else {
+        if (cap_rhs32 >= nrhs) return;
+        t32.alloc(sizeof(float) * (size_t)(n_t > 0 ? n_t : 1) * nrhs);
+        part32.alloc(sizeof(float) * (size_t)(n_slots > 0 ? n_slots : 1) * 64 * nrhs);
+        cap_rhs32 = nrhs;
+    }
+}
+
+void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev) {
+    using namespace hodlr;
+    ensure_work(prec, nrhs);
+    if (prec == HODLR_PREC_F64) {
+        HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
+                                        tcopy.as<TCopy>(), n_tcopy, panel.as<PanelItem>(), n_panel,
+                                        segs.as<Segment>(), VP64.as<double>(), NP64.as<double>(),
+                                        UP64.as<double>(), static_cast<const double *>(x), t64.as<double>(),
+                                        part64.as<double>(), tidx.as<int64_t>(), static_cast<double *>(y), ev, s));
+    } else {
+        if (!(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
+        HIP_CHECK(launch_matvec<float>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
+                                       tcopy.as<TCopy>(), n_tcopy, panel.as<PanelItem>(), n_panel,
+                                       segs.as<Segment>(), VP32.as<float>(), NP32.as<float>(), UP32.as<float>(),
+                                       static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
+                                       tidx.as<int64_t>(), static_cast<float *>(y), ev, s));
+    }
+}

@@ -1,0 +1,78 @@
+// op.h -- the assembled HODLR operator (host view of device arenas).
+#pragma once
+
+#include <functional>
+
+#include "aca_layout.h"
+#include "device_common.h"
+#include "internal.h"
+
+namespace hodlr {
+
+hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev &K, double tol,
+                      double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
+                      const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
+                      hipStream_t s);
+hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev &K,
+                             const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, hipStream_t s);
+hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev &K,
+                            double *NP, hipStream_t s);
+hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s);
+template <typename T>
+hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
+                         const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
+                         const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
+                         T *t, T *part, const int64_t *tidx, T *y, hipEvent_t *ev, hipStream_t s);
+
+int64_t mdot_chunks(int64_t n);
+hipError_t launch_mdot(const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *partial,
+                       double *out, int accumulate, hipStream_t s);
+hipError_t launch_gemv(const double *V, int64_t ldv, int nv, const double *h, double sign, double *w,
+                       int64_t n, hipStream_t s);
+hipError_t launch_scale(const double *src, double alpha, double *dst, int64_t n, hipStream_t s);
+hipError_t launch_sub(const double *a, const double *b, double *c, int64_t n, hipStream_t s);
+hipError_t launch_add(double *a, const double *b, int64_t n, hipStream_t s);
+hipError_t launch_to_f32(const double *a, float *b, int64_t n, hipStream_t s);
+hipError_t launch_to_f64(const float *a, double *b, int64_t n, hipStream_t s);
+
+}  // namespace hodlr
+
+struct hodlr_op {
+    hodlr_ctx *ctx = nullptr;
+    hodlr_tree *tree = nullptr;
+    hodlr_kernel_desc kd{};
+    double eps = 0.0;
+    int prec_flags = 0;
+    int64_t N = 0;
+
+    // tree-order coordinates, weights, per-position leaf start / size
+    hodlr::DevBuf X, W, leaf_start, leaf_cnt;
+    hodlr::DevBuf ord;   // nlevels x N: per node, tree positions in ascending point id
+
+    // blocks in reference order (level, node, interaction index)
+    int64_t nblk = 0;
+    std::vector<int32_t> b_level, b_m, b_n, rank, flags;
+    std::vector<int64_t> b_row, b_col;   // level-major node ids
+
+    // arenas: near field, U panels, V panels
+    int64_t n_np = 0, n_up = 0, n_vp = 0, n_t = 0, n_slots = 0;
+    hodlr::DevBuf NP64, UP64, VP64, NP32, UP32, VP32;
+
+    // matvec work lists
+    int64_t n_proj = 0, n_red = 0, n_tcopy = 0, n_panel = 0, n_seg = 0;
+    hodlr::DevBuf proj, red, tcopy, panel, segs, tidx;
+    double e_near = 0, e_far = 0, e_v = 0, e_u = 0, t_lowrank = 0;
+
+    // per-precision work buffers (grown on demand)
+    int cap_rhs64 = 0, cap_rhs32 = 0;
+    hodlr::DevBuf t64, part64, x64, y64, t32, part32, x32, y32;
+
+    hodlr_op_stats stats{};
+
+    void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr);
+    void ensure_work(int prec, int nrhs);
+};
+
+namespace hodlr {
+void assemble(hodlr_op &op);
+}

@@ -1,0 +1,248 @@
+// solver.cpp -- restarted GMRES on the device (GMRES.hpp:160-303) and the
+// FP32-inner / FP64-refined solve.
+//
+// Control flow follows iterSolver::GMRES: residual check `<= tol` before the
+// first cycle, `< tol` after every Arnoldi step and after every restart,
+// x updated by the back-solve of the rotated Hessenberg system
+// (GMRES.hpp:132-146), real Givens rotations (GMRES.hpp:83-129).  Vectors
+// stay on the device in tree order; per iteration the host receives the new
+// Hessenberg column (i + 2 doubles) and applies the rotations.
+// A zero subdiagonal (lucky breakdown) leaves the next basis vector zero
+// instead of dividing by zero (the reference has no guard, GMRES.hpp:242).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "op.h"
+#include "solver.h"
+
+namespace hodlr {
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void gen_rot(double dx, double dy, double &cs, double &sn) {
+    if (dy == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+    } else if (std::fabs(dy) > std::fabs(dx)) {
+        double temp = dx / dy;
+        sn = 1.0 / std::sqrt(1.0 + temp * temp);
+        cs = temp * sn;
+    } else {
+        double temp = dy / dx;
+        cs = 1.0 / std::sqrt(1.0 + temp * temp);
+        sn = temp * cs;
+    }
+}
+
+void app_rot(double &dx, double &dy, double cs, double sn) {
+    double temp = cs * dx + sn * dy;
+    dy = (-sn) * dx + cs * dy;
+    dx = temp;
+}
+
+}  // namespace
+
+Workspace::Workspace(int64_t n_, int restart_) : n(n_), restart(restart_) {
+    V.alloc(sizeof(double) * (size_t)n * (restart + 1));
+    w.alloc(sizeof(double) * n);
+    r.alloc(sizeof(double) * n);
+    partial.alloc(sizeof(double) * (size_t)mdot_chunks(n) * 130);
+    small.alloc(sizeof(double) * 512);
+    HIP_CHECK(hipHostMalloc(&hbuf, sizeof(double) * 512));
+}
+
+Workspace::~Workspace() {
+    if (hbuf) (void)hipHostFree(hbuf);
+}
+
+double Workspace::norm2(const double *v, hipStream_t s) {
+    HIP_CHECK(launch_mdot(v, n, 1, v, n, partial.as<double>(), small.as<double>(), 0, s));
+    HIP_CHECK(hipMemcpyAsync(hbuf, small.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    return std::sqrt(hbuf[0]);
+}
+
+int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int max_iter, double tol,
+                 bool x0_zero, hipStream_t s, GmresResult &res, std::vector<double> *hist) {
+    const int64_t n = ws.n;
+    const int m = ws.restart;
+    double *V = ws.V.as<double>();
+    double *w = ws.w.as<double>();
+    double *r = ws.r.as<double>();
+    double *hd = ws.small.as<double>();  // [0,128) h1, [128,256) h2
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), sv(m + 1), cs(m + 1), sn(m + 1);
+    auto Hc = [&](int row, int c) -> double & { return H[(size_t)c * (m + 1) + row]; };
+
+    const double normb0 = ws.norm2(b, s);
+    if (x0_zero) {
+        HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+        A(x, w);
+        res.matvecs++;
+        HIP_CHECK(launch_sub(b, w, r, n, s));
+    }
+    double beta = ws.norm2(r, s);
+    double normb = normb0 == 0.0 ? 1.0 : normb0;
+    double resid = beta / normb;
+    res.resid = resid;
+    if (resid <= tol) {  // GMRES.hpp:198-213
+        res.iterations = 0;
+        res.converged = true;
+        return 0;
+    }
+    int j = 1;
+    auto update = [&](int k) {  // GMRES.hpp:132-146
+        std::vector<double> y(sv.begin(), sv.begin() + k + 1);
+        for (int i = k; i >= 0; i--) {
+            y[i] /= Hc(i, i);
+            for (int q = i - 1; q >= 0; q--) y[q] -= Hc(q, i) * y[i];
+        }
+        std::memcpy(ws.hbuf, y.data(), sizeof(double) * (k + 1));
+        HIP_CHECK(hipMemcpyAsync(hd, ws.hbuf, sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
+        HIP_CHECK(launch_gemv(V, n, k + 1, hd, 1.0, x, n, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    };
+    while (j <= max_iter) {
+        HIP_CHECK(launch_scale(r, 1.0 / beta, V, n, s));
+        std::fill(sv.begin(), sv.end(), 0.0);
+        sv[0] = beta;
+        for (int i = 0; i < m && j <= max_iter; i++, j++) {
+            if (hist) hist->push_back(resid);
+            double *vi = V + (size_t)i * n;
+            A(vi, w);
+            res.matvecs++;
+            // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
+            HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd, 0, s));
+            HIP_CHECK(launch_gemv(V, n, i + 1, hd, -1.0, w, n, s));
+            HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd + 128, 0, s));
+            HIP_CHECK(launch_gemv(V, n, i + 1, hd + 128, -1.0, w, n, s));
+            HIP_CHECK(launch_mdot(w, n, 1, w, n, ws.partial.as<double>(), hd + 256, 0, s));
+            HIP_CHECK(hipMemcpyAsync(ws.hbuf, hd, sizeof(double) * 257, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            for (int k = 0; k <= i; k++) Hc(k, i) = ws.hbuf[k] + ws.hbuf[128 + k];
+            const double hn = std::sqrt(ws.hbuf[256]);
+            Hc(i + 1, i) = hn;
+            if (!std::isfinite(hn)) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
+            double *vn = V + (size_t)(i + 1) * n;
+            if (hn != 0.0) HIP_CHECK(launch_scale(w, 1.0 / hn, vn, n, s));
+            else HIP_CHECK(hipMemsetAsync(vn, 0, sizeof(double) * n, s));
+            for (int k = 0; k < i; k++) app_rot(Hc(k, i), Hc(k + 1, i), cs[k], sn[k]);
+            gen_rot(Hc(i, i), Hc(i + 1, i), cs[i], sn[i]);
+            app_rot(Hc(i, i), Hc(i + 1, i), cs[i], sn[i]);
+            app_rot(sv[i], sv[i + 1], cs[i], sn[i]);
+            resid = std::fabs(sv[i + 1]) / normb;
+            res.resid = resid;
+            if (resid < tol) {
+                update(i);
+                res.iterations = j;
+                res.converged = true;
+                return j;
+            }
+        }
+        update(m - 1);
+        A(x, w);
+        res.matvecs++;
+        HIP_CHECK(launch_sub(b, w, r, n, s));
+        beta = ws.norm2(r, s);
+        resid = beta / normb;
+        res.resid = resid;
+        if (resid < tol) {
+            res.iterations = j;
+            res.converged = true;
+            return j;
+        }
+    }
+    res.iterations = max_iter;
+    res.converged = false;
+    return max_iter;
+}
+
+void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p, hodlr_gmres_info &info) {
+    const double t0 = now_s();
+    hipStream_t s = op.ctx->stream;
+    const int64_t n = op.N;
+    const int restart = p.restart > 0 ? p.restart : 100;
+    if (restart + 1 > 128) throw ArgError{HODLR_E_ARG, "restart must be at most 127"};
+    const int max_iter = p.max_iter > 0 ? p.max_iter : (int)std::min<int64_t>(n, INT32_MAX);
+    Workspace ws(n, restart);
+    std::memset(&info, 0, sizeof(info));
+    std::vector<double> hist;
+    if (p.x0_is_b) HIP_CHECK(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+
+    Apply A64 = [&](const double *in, double *out) { op.matvec_dev(HODLR_PREC_F64, 1, in, out, s); };
+    GmresResult gr;
+    if (p.mode == HODLR_SOLVE_F64) {
+        gmres_device(A64, ws, b, x, max_iter, p.tol, false, s, gr, &hist);
+        info.iterations = gr.iterations;
+        info.matvecs_f64 = gr.matvecs;
+        info.converged = gr.converged;
+        info.resid = gr.resid;
+        info.true_resid = -1.0;
+        if (p.verify) {
+            A64(x, ws.w.as<double>());
+            HIP_CHECK(launch_sub(b, ws.w.as<double>(), ws.r.as<double>(), n, s));
+            double nb = ws.norm2(b, s);
+            info.true_resid = ws.norm2(ws.r.as<double>(), s) / (nb == 0.0 ? 1.0 : nb);
+        }
+    } else {
+        if (!(op.prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
+        op.ensure_work(HODLR_PREC_F32, 1);
+        DevBuf x32, y32, d, rr, w64;
+        x32.alloc(sizeof(float) * n);
+        y32.alloc(sizeof(float) * n);
+        d.alloc(sizeof(double) * n);
+        rr.alloc(sizeof(double) * n);
+        w64.alloc(sizeof(double) * n);
+        Apply A32 = [&](const double *in, double *out) {
+            HIP_CHECK(launch_to_f32(in, x32.as<float>(), n, s));
+            op.matvec_dev(HODLR_PREC_F32, 1, x32.as<float>(), y32.as<float>(), s);
+            HIP_CHECK(launch_to_f64(y32.as<float>(), out, n, s));
+        };
+        const double nb0 = ws.norm2(b, s);
+        const double nb = nb0 == 0.0 ? 1.0 : nb0;
+        // r = b - A64 x
+        A64(x, w64.as<double>());
+        info.matvecs_f64++;
+        HIP_CHECK(launch_sub(b, w64.as<double>(), rr.as<double>(), n, s));
+        double res = ws.norm2(rr.as<double>(), s) / nb;
+        hist.push_back(res);
+        const int max_outer = p.max_outer > 0 ? p.max_outer : 20;
+        const double inner_tol = p.inner_tol > 0 ? p.inner_tol : 1e-6;
+        int total = 0;
+        while (!(res < p.tol) && info.outer < max_outer && total < max_iter) {
+            GmresResult g;
+            g.matvecs = 0;
+            HIP_CHECK(hipMemsetAsync(d.p, 0, sizeof(double) * n, s));
+            gmres_device(A32, ws, rr.as<double>(), d.as<double>(), max_iter - total, inner_tol, true, s, g,
+                         nullptr);
+            total += g.iterations;
+            info.matvecs_f32 += g.matvecs;
+            HIP_CHECK(launch_add(x, d.as<double>(), n, s));
+            A64(x, w64.as<double>());
+            info.matvecs_f64++;
+            HIP_CHECK(launch_sub(b, w64.as<double>(), rr.as<double>(), n, s));
+            res = ws.norm2(rr.as<double>(), s) / nb;
+            hist.push_back(res);
+            info.outer++;
+            if (g.iterations == 0) break;  // no progress possible at this precision
+        }
+        info.iterations = total;
+        info.converged = res < p.tol;
+        info.resid = res;
+        info.true_resid = res;
+    }
+    HIP_CHECK(hipStreamSynchronize(s));
+    if (p.hist && p.hist_cap > 0) {
+        const int nh = (int)std::min<size_t>(hist.size(), (size_t)p.hist_cap);
+        std::memcpy(p.hist, hist.data(), sizeof(double) * nh);
+        info.hist_len = nh;
+    }
+    info.seconds = now_s() - t0;
+}
+
+}  // namespace hodlr

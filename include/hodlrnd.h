@@ -1,0 +1,198 @@
+/*
+ * hodlrnd.h -- C ABI of the MI355X (gfx950) HODLRnD hot path.
+ *
+ * The reference (kandapva/HODLRnD) exposes its operator as C++ templates
+ * (include/HDD_matrix.hpp, include/GMRES.hpp); it has no FFI.  Each entry
+ * point below replaces one of those calls, cited as file:line in the
+ * reference.  The reference-shaped C++ API over this ABI is
+ * include/hodlrnd/HODLRnD.hpp.
+ *
+ * Conventions: every function returns HODLR_OK (0) or a negative status;
+ * the message of the last failure on a context is hodlr_last_error(ctx).
+ * No exception crosses the ABI.  Vectors passed as host pointers are in the
+ * caller's point order (ptsnD order); device pointers are in tree order
+ * (see hodlr_tree_perm).  One operator may be used from one host thread at
+ * a time.
+ */
+#ifndef HODLRND_H
+#define HODLRND_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HODLR_OK = 0,
+    HODLR_E_ARG = -1,        /* invalid argument */
+    HODLR_E_DEVICE = -2,     /* HIP runtime error */
+    HODLR_E_NOMEM = -3,      /* device or host allocation failed */
+    HODLR_E_RANK = -4,       /* ACA exceeded its rank budget */
+    HODLR_E_NOCONV = -5,     /* solver stopped at max_iter */
+    HODLR_E_BREAKDOWN = -6,  /* non-finite value in the solver */
+    HODLR_E_STATE = -7       /* call not valid in the object's state */
+};
+
+/* kernel families: A_ij = diag if i == j, else scale * f(|x_i - x_j|) * w_j */
+enum {
+    HODLR_KERNEL_LOG = 0,  /* f(r) = log r     (user_kernel.hpp:46-49)   */
+    HODLR_KERNEL_INV_R = 1,/* f(r) = 1 / r     (user_kernel.hpp:101-104) */
+    HODLR_KERNEL_EXP = 2   /* f(r) = exp(-r)   (user_kernel.hpp:167-170) */
+};
+
+/* operator precisions built by hodlr_op_assemble */
+enum {
+    HODLR_PREC_F64 = 1,    /* FP64 factors (always built) */
+    HODLR_PREC_F32 = 2     /* an FP32 copy of every factor as well */
+};
+
+/* solver modes */
+enum {
+    HODLR_SOLVE_F64 = 0,   /* FP64 GMRES on the FP64 operator (GMRES.hpp:160-303) */
+    HODLR_SOLVE_MIXED = 1  /* FP64 iterative refinement around FP32-operator GMRES */
+};
+
+typedef struct hodlr_ctx hodlr_ctx;
+typedef struct hodlr_tree hodlr_tree;
+typedef struct hodlr_op hodlr_op;
+
+/* Replaces the user kernel's getMatrixEntry (user_kernel.hpp:50-58,105-113,
+ * 171-177) for device evaluation.  weight is nullable; when given it holds
+ * one value per point in point order. */
+typedef struct {
+    int32_t family;
+    int32_t ndim;
+    double scale;
+    double diag;
+    const double *weight;
+} hodlr_kernel_desc;
+
+typedef struct {
+    int32_t nlevels;       /* levels including the root (HDD_tree.hpp:73) */
+    int32_t ndim;
+    int64_t npoints;
+    int64_t nnodes;        /* all levels */
+    int64_t nleaves;
+    int64_t max_leaf;      /* largest leaf */
+    int64_t n_admissible;  /* interaction-list entries with both sides non-empty */
+    int64_t n_near;        /* neighbour-list entries at the leaves, both non-empty */
+} hodlr_tree_info;
+
+typedef struct {
+    double n_flop_ref;     /* HDD_node.hpp:183-202 */
+    int64_t max_rank;      /* HDD_node.hpp:205-220 */
+    double e_near;         /* near-field entries */
+    double e_far;          /* stored far-field entries */
+    int64_t n_blocks;      /* compressed blocks */
+    int64_t n_dense;       /* blocks stored dense (LowRank_matrix.hpp:348-363) */
+    double bytes_f64;      /* HBM bytes of the FP64 operator */
+    double bytes_f32;      /* HBM bytes of the FP32 copy (0 if not built) */
+    double matvec_bytes_f64; /* algorithmic bytes of one FP64 matvec, 1 RHS */
+    double matvec_bytes_f32;
+    double assemble_seconds;
+    double rank_seconds;   /* ACA rank pass */
+    double fill_seconds;   /* ACA fill pass + near field */
+} hodlr_op_stats;
+
+typedef struct {
+    int32_t restart;       /* GMRES.hpp:44 (reference solve uses 100) */
+    int32_t max_iter;      /* reference solve uses N */
+    double tol;            /* relative residual (reference solve uses eps_ACA) */
+    int32_t mode;          /* HODLR_SOLVE_F64 or HODLR_SOLVE_MIXED */
+    int32_t x0_is_b;       /* 1: initial guess x0 = b (HDD_matrix.hpp:52-53) */
+    double inner_tol;      /* mixed mode: relative tolerance of each FP32 solve */
+    int32_t max_outer;     /* mixed mode: refinement sweeps */
+    int32_t verify;        /* F64 mode: recompute ||b - A x|| / ||b|| at exit */
+    int32_t hist_cap;      /* length of hist (may be 0) */
+    double *hist;          /* residual history, one entry per iteration */
+} hodlr_gmres_params;
+
+typedef struct {
+    int32_t iterations;    /* return value of iterSolver::GMRES (GMRES.hpp:212,267,294,302) */
+    int32_t matvecs_f64;
+    int32_t matvecs_f32;
+    int32_t outer;         /* refinement sweeps (mixed mode) */
+    int32_t converged;
+    int32_t hist_len;
+    double resid;          /* relative residual estimate at exit */
+    double true_resid;     /* ||b - A64 x|| / ||b|| recomputed in FP64 */
+    double seconds;
+} hodlr_gmres_info;
+
+/* ---- context ---- */
+/* device -1 gives a host-only context: tree construction and ordering helpers
+ * work, everything that needs the GPU returns HODLR_E_STATE. */
+int hodlr_ctx_create(int device, hodlr_ctx **out);
+void hodlr_ctx_destroy(hodlr_ctx *ctx);
+const char *hodlr_last_error(hodlr_ctx *ctx);
+const char *hodlr_version(void);
+
+/* HDD_matrix.hpp:20-33 -> HDD_tree.hpp:23-74: 2^d tree and lists.
+ * pts is n x ndim row-major (ptsnD order); lo/hi the bounding box. */
+int hodlr_tree_build(hodlr_ctx *ctx, int ndim, int64_t n, const double *pts,
+                     const double *lo, const double *hi, int nmax, int dprime,
+                     hodlr_tree **out);
+int hodlr_tree_get_info(hodlr_tree *t, hodlr_tree_info *info);
+/* tree position -> point id (leaf order) */
+int hodlr_tree_perm(hodlr_tree *t, int64_t *perm);
+/* per node (level-major numbering): start in perm and point count */
+int hodlr_tree_nodes(hodlr_tree *t, int64_t *level_offset /* nlevels+1 */,
+                     int64_t *start, int64_t *count);
+/* HDD_node.hpp:150-180: which = 0 neighbours, 1 interaction list; lists are
+ * same-level node indices in reference order, CSR over the level-major nodes */
+int hodlr_tree_lists(hodlr_tree *t, int which, int64_t *ptr /* nnodes+1 */,
+                     int64_t *idx /* nullable: query the size via ptr */);
+void hodlr_tree_destroy(hodlr_tree *t);
+
+/* HDD_matrix.hpp:39-41 -> HDD_tree.hpp:76-83 -> HDD_node.hpp:118-147:
+ * batched ACA of every admissible block (LowRank_matrix.hpp:64-377) and
+ * the dense near field.  The tree must outlive the operator. */
+int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
+                      double eps_aca, int prec_flags, hodlr_op **out);
+/* HDD_matrix.hpp:69-85 */
+int hodlr_op_get_stats(hodlr_op *op, hodlr_op_stats *s);
+/* ranks and dense flags in block order (level, node, interaction index) */
+int hodlr_op_block_ranks(hodlr_op *op, int32_t *rank, int32_t *dense, int64_t cap);
+void hodlr_op_destroy(hodlr_op *op);
+
+/* HDD_matrix.hpp:43-47 -> HDD_tree.hpp:86-107: y = A x for nrhs columns
+ * (column-major n x nrhs), host buffers in point order. prec = HODLR_PREC_F64
+ * or HODLR_PREC_F32. */
+int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y);
+/* Same on device buffers in tree order, stream-ordered on `stream` (a
+ * hipStream_t, NULL for the default stream).  F64: x,y are double n x nrhs
+ * with the right-hand sides interleaved (element (i, c) at i*nrhs + c);
+ * F32: float, same layout. */
+int hodlr_matvec_device(hodlr_op *op, int prec, int nrhs, const void *x, void *y,
+                        void *stream);
+
+/* HDD_matrix.hpp:51-62 with GMRES.hpp:160-303: solve A x = b, host buffers
+ * in point order. x holds the initial guess unless params->x0_is_b. */
+int hodlr_gmres(hodlr_op *op, const double *b, double *x,
+                const hodlr_gmres_params *params, hodlr_gmres_info *info);
+/* Same with b and x on the device (tree order, double). */
+int hodlr_gmres_device(hodlr_op *op, const double *b, double *x,
+                       const hodlr_gmres_params *params, hodlr_gmres_info *info);
+
+/* ---- measurement ---- */
+/* Times `reps` back-to-back matvecs on the op's stream with HIP events per
+ * kernel; ms[0] = whole matvec, ms[1] = projection kernel (V^T x),
+ * ms[2] = panel kernel (near field + U t), averages per launch. */
+int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms);
+/* algorithmic HBM bytes of one launch of each matvec kernel (same order) */
+int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes);
+
+/* device memory helpers for callers that keep vectors resident */
+int hodlr_device_alloc(hodlr_ctx *ctx, int64_t bytes, void **ptr);
+int hodlr_device_free(hodlr_ctx *ctx, void *ptr);
+int hodlr_copy_to_device(hodlr_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int hodlr_copy_to_host(hodlr_ctx *ctx, void *dst, const void *src, int64_t bytes);
+/* point order <-> tree order for n x nrhs double vectors (host) */
+int hodlr_to_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst);
+int hodlr_from_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

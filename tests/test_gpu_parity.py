@@ -1,0 +1,189 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle.
+
+Bars:
+  * tree / lists: bit-exact (also checked on CPU in test_tree.py);
+  * ACA ranks and dense flags per block: identical to the oracle for the
+    1/r kernel (entries round identically on both sides); >= 99% for the
+    log / exp kernels, whose device log/exp may differ from libm by an ulp;
+  * FP64 matvec vs the oracle's HODLR matvec: 1e-12 relative;
+  * FP64 matvec vs the dense FP64 product: 100 * eps_ACA relative (the
+    HODLR approximation error, not a device error);
+  * FP32 matvec vs FP64 matvec: 1e-5 relative;
+  * GMRES: relative true residual below tol, solution vs the oracle's
+    GMRES solution within 100 * tol.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = pytest.importorskip("hodlrnd_amd")
+from oracle import pyoracle as O  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def problem(nd, n, seed, family=O.K_INV, scale=None, diag=1.0, weight=None):
+    pts = O.splitmix_uniform(seed, n * nd).reshape(n, nd)
+    scale = 1.0 / n if scale is None else scale
+    return pts, family, scale, diag, weight
+
+
+def build_pair(pts, family, scale, diag, weight, nmax, dprime, eps, prec=H.PREC_F64 | H.PREC_F32):
+    nd = pts.shape[1]
+    lo, hi = -np.ones(nd), np.ones(nd)
+    kern = H.Kernel(family, nd, scale, diag, weight)
+    A = H.HODLRdD_matrix(kern, pts, lo, hi, nmax=nmax, dprime=dprime, eps_aca=eps, prec=prec)
+    A.Assemble_matrix_operators()
+    T = O.Tree(pts, lo, hi, nmax, dprime)
+    ok = O.KernelDesc(family, nd, scale, diag, weight)
+    ref = O.Operator(T, ok, eps, sys_size=-1)
+    return A, ref, ok
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+CASES = [
+    # name, nd, n, nmax, dprime, eps, family
+    ("3d_inv_cfg1_small", 3, 4000, 64, 0, 1e-10, O.K_INV),
+    ("2d_inv", 2, 3000, 32, 0, 1e-10, O.K_INV),
+    ("3d_inv_hmatrix", 3, 3000, 32, -1, 1e-8, O.K_INV),
+    ("3d_exp", 3, 3000, 64, 0, 1e-8, O.K_EXP),
+    ("2d_log", 2, 2500, 32, 0, 1e-10, O.K_LOG),
+]
+
+
+@pytest.mark.parametrize("name,nd,n,nmax,dprime,eps,family", CASES)
+def test_matvec_parity(name, nd, n, nmax, dprime, eps, family):
+    say("case", name)
+    scale = -1.0 / (2 * math.pi) / n if family == O.K_LOG else 1.0 / n
+    pts, fam, sc, dg, w = problem(nd, n, 100 + n, family, scale)
+    A, ref, ok = build_pair(pts, fam, sc, dg, w, nmax, dprime, eps)
+    r_gpu, d_gpu = A.op.block_ranks()
+    r_ref, d_ref = ref.block_ranks()
+    assert len(r_gpu) == len(r_ref)
+    same = np.mean((r_gpu == r_ref) & (d_gpu == d_ref))
+    say(name, "blocks", len(r_gpu), "rank agreement", same)
+    if family == O.K_INV:
+        assert same == 1.0
+    else:
+        assert same >= 0.99
+    x = O.splitmix_uniform(7, n)
+    y = A * x
+    y_ref = ref.matvec(x)
+    e_ref = relerr(y, y_ref)
+    y_dense = O.dense_matvec(ok, pts, x)
+    e_dense = relerr(y, y_dense)
+    say(name, "vs oracle", e_ref, "vs dense", e_dense)
+    assert e_ref < (1e-12 if same == 1.0 else 100 * eps)
+    assert e_dense < 100 * eps
+    y32 = A.op.matvec(x, prec=H.PREC_F32)
+    assert relerr(y32, y) < 1e-5
+    st = A.op.stats()
+    so = ref.stats()
+    assert st["e_near"] == so["e_near"]
+    if family == O.K_INV:
+        assert st["e_far"] == so["e_far"] and st["n_flop_ref"] == so["n_flop_ref"]
+        assert st["max_rank"] == so["max_rank"] and st["n_dense"] == so["n_dense"]
+
+
+def test_multi_rhs_columns():
+    say("case multi_rhs")
+    pts, fam, sc, dg, w = problem(3, 3000, 5)
+    A, ref, ok = build_pair(pts, fam, sc, dg, w, 64, 0, 1e-8)
+    X = np.stack([O.splitmix_uniform(20 + c, 3000) for c in range(8)], axis=1)
+    Y = A.op.matvec(X)
+    for c in range(8):
+        y1 = A.op.matvec(X[:, c])
+        assert relerr(Y[:, c], y1) < 1e-14
+    Y32 = A.op.matvec(X[:, :5], prec=H.PREC_F32)  # padded to 8 columns internally
+    assert relerr(Y32, Y[:, :5]) < 1e-5
+
+
+def test_gmres_f64_vs_oracle():
+    say("case gmres_f64")
+    n = 4000
+    pts, fam, sc, dg, w = problem(3, n, 4100)
+    A, ref, ok = build_pair(pts, fam, sc, dg, w, 64, 0, 1e-10)
+    b = O.splitmix_uniform(3, n)
+    x = A.solve(b, tol=1e-10)
+    info = A.last_info
+    xr, ir = ref.gmres(b, tol=1e-10)
+    say("gpu", info["iterations"], info["true_resid"], "oracle", ir)
+    assert info["converged"]
+    assert info["true_resid"] < 1e-9
+    assert abs(info["iterations"] - ir["iterations"]) <= 1
+    assert relerr(x, xr) < 1e-8
+
+
+def test_gmres_mixed_reaches_fp64_residual():
+    say("case gmres_mixed")
+    n = 4000
+    pts, fam, sc, dg, w = problem(3, n, 4200)
+    A, ref, ok = build_pair(pts, fam, sc, dg, w, 64, 0, 1e-8)
+    b = O.splitmix_uniform(9, n)
+    x64 = A.solve(b, tol=1e-8)
+    i64 = A.last_info
+    xm = A.solve(b, tol=1e-8, mode=H.SOLVE_MIXED, inner_tol=1e-5)
+    im = A.last_info
+    say("f64", i64["iterations"], i64["true_resid"], "mixed", im["iterations"], im["outer"],
+        im["true_resid"])
+    assert im["converged"] and im["true_resid"] < 1e-8
+    r = b - A * xm
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
+    assert relerr(xm, x64) < 1e-7
+
+
+def test_kernel_4d_fixture(golden_dir):
+    """include/main.cpp flow on the shipped fixture: kernel_4d_test at
+    numPoints = 10 (N = 10^4, NDIM = 4, Nmax = 1000, d' = 1, eps = 1e-10);
+    the HODLR matvec of data/x_10000.bin against data/rhs_10000.bin."""
+    say("case kernel_4d_fixture")
+    import os
+    x = O.read_cereal_vec(os.path.join(golden_dir, "x_10000.bin"))
+    rhs = O.read_cereal_vec(os.path.join(golden_dir, "rhs_10000.bin"))
+    h = 0.1
+    kii = 1.0 + O.quadruple_integral([0] * 4, [h * 0.5] * 4, 32)
+    pts = O.tensor_grid(4, O.uniform_nodes(-1, 1, 10))
+    kern = H.kernel_4d_test(10, kii)
+    A = H.HODLRdD_matrix(kern, pts, -np.ones(4), np.ones(4), nmax=1000, dprime=1, eps_aca=1e-10)
+    A.Assemble_matrix_operators()
+    y = A * x
+    e = relerr(y, rhs)
+    say("4d fixture rel err", e)
+    assert e < 1e-8
+    xs = A.solve(y, tol=1e-10)
+    assert A.last_info["converged"]
+    assert relerr(xs, x) < 1e-8
+
+
+def test_depth_zero_dense_only():
+    say("case depth0")
+    pts, fam, sc, dg, w = problem(3, 50, 1)
+    A, ref, ok = build_pair(pts, fam, sc, dg, w, 64, 0, 1e-10)
+    x = O.splitmix_uniform(2, 50)
+    y = A * x
+    assert relerr(y, O.dense_matvec(ok, pts, x)) < 1e-14
+
+
+def test_empty_leaves_and_weights():
+    """Gauss-Legendre tensor grid in 4D (config-3 shape at 8^4 points):
+    many empty leaves, column weights w_j."""
+    say("case gl4d_weights")
+    gx, gw = O.gauss_legendre(8)
+    pts = O.tensor_grid(4, gx)
+    wts = O.tensor_weights(4, gw)
+    n = len(pts)
+    A, ref, ok = build_pair(pts, O.K_INV, 1.0, 1.0, wts, 16, 0, 1e-8)
+    x = O.splitmix_uniform(4, n)
+    y = A * x
+    assert relerr(y, ref.matvec(x)) < 1e-12
+    assert relerr(y, O.dense_matvec(ok, pts, x)) < 1e-6
