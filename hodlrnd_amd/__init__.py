@@ -90,6 +90,8 @@ SIGNATURES = {
     "hodlr_gmres_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hodlr_matvec_timing": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     "hodlr_matvec_kernel_bytes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hodlr_op_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "hodlr_op_profile_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "hodlr_device_alloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "hodlr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
@@ -149,6 +151,33 @@ def default_context():
     if _default_ctx is None:
         _default_ctx = Context(0)
     return _default_ctx
+
+
+class DeviceArray:
+    """Device buffer owned by a context (hodlr_device_alloc)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        ctx.check(lib.hodlr_device_alloc(ctx.h, self.nbytes, C.byref(self.ptr)), "device_alloc")
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        self.ctx.check(lib.hodlr_copy_to_device(self.ctx.h, self.ptr, _p(a), a.nbytes), "to_device")
+
+    def download(self, a):
+        self.ctx.check(lib.hodlr_copy_to_host(self.ctx.h, _p(a), self.ptr, a.nbytes), "to_host")
+        return a
+
+    def free(self):
+        if self.ptr:
+            lib.hodlr_device_free(self.ctx.h, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        self.free()
 
 
 class Tree:
@@ -261,7 +290,7 @@ class Operator:
         return Y[:, 0].copy() if single else Y
 
     def gmres(self, b, x0=None, restart=100, max_iter=None, tol=1e-10, mode=SOLVE_F64,
-              inner_tol=1e-6, max_outer=20, verify=True, hist_cap=512):
+              inner_tol=0.0, max_outer=20, verify=True, hist_cap=512):
         b = _f64(b)
         x = b.copy() if x0 is None else _f64(x0).copy()
         hist = np.zeros(hist_cap)
@@ -276,10 +305,33 @@ class Operator:
         out["hist"] = hist[:info.hist_len].copy()
         return x, out
 
+    def gmres_device(self, b_dev, x_dev, restart=100, max_iter=None, tol=1e-10, mode=SOLVE_F64,
+                     inner_tol=0.0, max_outer=20, verify=False, x0="b"):
+        """Solve with b and x resident on the device (tree order, double).
+        x0: "b" (the reference's choice), "zero", or "given" (x holds it)."""
+        x0code = {"b": 1, "zero": 2, "given": 0}[x0]
+        p = GmresParams(restart, self.n if max_iter is None else max_iter, tol, mode,
+                        x0code, inner_tol, max_outer, 1 if verify else 0, 0, None)
+        info = GmresInfo()
+        rc = lib.hodlr_gmres_device(self.h, b_dev, x_dev, C.byref(p), C.byref(info))
+        if rc not in (OK, E_NOCONV):
+            self.ctx.check(rc, "gmres_device")
+        return {f: getattr(info, f) for f, _ in GmresInfo._fields_}
+
     def matvec_timing(self, prec=PREC_F64, nrhs=1, reps=10):
         ms = np.zeros(3)
         self.ctx.check(lib.hodlr_matvec_timing(self.h, prec, nrhs, reps, _p(ms)), "matvec_timing")
         return ms
+
+    def profile(self, enable=True):
+        self.ctx.check(lib.hodlr_op_profile(self.h, 1 if enable else 0), "profile")
+
+    def profile_read(self, reset=True):
+        ms = np.zeros(4)
+        cnt = np.zeros(2, dtype=np.int64)
+        self.ctx.check(lib.hodlr_op_profile_read(self.h, _p(ms), _p(cnt), 1 if reset else 0),
+                       "profile_read")
+        return ms, cnt
 
     def kernel_bytes(self, prec=PREC_F64, nrhs=1):
         b = np.zeros(3)

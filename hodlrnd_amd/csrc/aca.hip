@@ -13,6 +13,7 @@
 // fill pass that repeats the identical computation and writes the factors
 // into their final panels.  Pivot-row/column residuals use explicit FMA in
 // the same order as the FP64 CPU restatement, so both pick the same pivots.
+#include <algorithm>
 #include <climits>
 
 #include "aca_layout.h"
@@ -409,10 +410,17 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
         const int64_t base = D.up_base + (int64_t)D.R * (ls - B.row0) + (p - ls);
         for (int q = 0; q < r; q++) UP[base + (int64_t)(D.coff + q) * nc] = U[(size_t)q * m + i];
     }
-    // V columns into the column node's row-major panel
+    // V columns into the column node's tile-major panel: tile i holds panel
+    // columns [256 i, 256 i + tw) as an n x tw row-major block
     for (int j = tid; j < n; j += BS) {
-        const int64_t base = D.vp_base + (int64_t)(B.ord[B.col0 + j] - B.col0) * D.T + D.voff;
-        for (int q = 0; q < r; q++) VP[base + q] = V[(size_t)q * n + j];
+        const int64_t row = B.ord[B.col0 + j] - B.col0;
+        for (int q = 0; q < r; q++) {
+            const int col = D.voff + q;
+            const int tile = col / kProjTile;
+            const int tw = min(kProjTile, D.T - tile * kProjTile);
+            VP[D.vp_base + (int64_t)tile * kProjTile * n + row * tw + (col - tile * kProjTile)] =
+                V[(size_t)q * n + j];
+        }
     }
 }
 
@@ -443,6 +451,21 @@ __global__ __launch_bounds__(256) void near_fill_kernel(const NearLeaf *__restri
         const int64_t i = e % L.nc, q = e / L.nc;
         NP[L.np_base + e] = entry(K, L.row0 + i, colsrc[L.colsrc + q]);
     }
+}
+
+// zero the diagonal of every leaf's self block (the first nc columns of its
+// near-field panel); the FP32 operator then holds only off-diagonal entries
+__global__ void zero_self_diag_kernel(const NearLeaf *__restrict__ leaves, int64_t nl, float *__restrict__ NP) {
+    for (int64_t l = blockIdx.x; l < nl; l += gridDim.x) {
+        const NearLeaf L = leaves[l];
+        for (int i = threadIdx.x; i < L.nc; i += blockDim.x) NP[L.np_base + (int64_t)i * L.nc + i] = 0.0f;
+    }
+}
+
+hipError_t launch_zero_self_diag(const NearLeaf *l, int64_t nl, float *NP, hipStream_t s) {
+    if (nl == 0) return hipSuccess;
+    zero_self_diag_kernel<<<dim3((unsigned)std::min<int64_t>(nl, 65536)), 64, 0, s>>>(l, nl, NP);
+    return hipGetLastError();
 }
 
 __global__ void convert_f64_f32_kernel(const double *__restrict__ a, float *__restrict__ b, int64_t n) {

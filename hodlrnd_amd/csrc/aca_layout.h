@@ -5,7 +5,9 @@
 
 namespace hodlr {
 
-constexpr int kMaxTries = 10;  // LowRank_matrix.hpp:117
+constexpr int kMaxTries = 10;     // LowRank_matrix.hpp:117
+constexpr int kProjTile = 256;    // columns of one V-panel tile (projection work item)
+constexpr int kProjChunk = 2048;  // rows of one projection work item
 
 // one admissible block for the ACA kernel
 // Row i of the block is the i-th smallest point id of the row node (the
@@ -24,9 +26,9 @@ struct AcaBlock {
 // where a block's factors go in the operator (fill pass)
 struct FillDest {
     int64_t up_base;      // U panel of the row node (elements)
-    int64_t vp_base;      // V panel of the column node (elements)
+    int64_t vp_base;      // V panel of the column node (elements, tile-major)
     int32_t R, coff;      // U panel width and this block's first column
-    int32_t T, voff;      // V panel width and this block's first column
+    int32_t T, voff;      // V panel width (padded to 4) and this block's first column
 };
 
 // a block stored dense (LowRank_matrix.hpp:348-363)
@@ -53,6 +55,7 @@ __host__ __device__ inline int64_t aca_scratch_doubles(int m, int n, int cap) {
 
 // ---- matvec work ----
 
+
 // V-panel projection tile: t[cols] (+)= VP[rows j0..j1, c0..c0+w)^T x[b0+j0..]
 struct ProjItem {
     int64_t vp;           // element offset of (row j0, column c0) in the V arena
@@ -60,7 +63,7 @@ struct ProjItem {
     int64_t out;          // slot: >= 0 partial-sum slot, < 0 direct (-(tile index) - 1)
     int32_t ld;           // panel width T_b
     int32_t rows;
-    int32_t w;            // tile width (<= 64)
+    int32_t w;            // tile width (<= kProjTile), padding columns excluded
     int32_t tcol;         // first global V column of the tile (into tidx)
 };
 

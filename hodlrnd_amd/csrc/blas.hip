@@ -30,13 +30,20 @@ __global__ __launch_bounds__(256) void mdot_kernel(const double *__restrict__ V,
         partial[(int64_t)blockIdx.x * nv + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
 }
 
-__global__ void mdot_finish_kernel(const double *__restrict__ partial, int np, int nv,
-                                   double *__restrict__ out, int accumulate) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nv) return;
+// one workgroup per output: out[k] = sum_p partial[p * nv + k], fixed order
+__global__ __launch_bounds__(256) void mdot_finish_kernel(const double *__restrict__ partial, int np, int nv,
+                                                          double *__restrict__ out, int accumulate) {
+    __shared__ double red[4];
+    const int k = blockIdx.x, tid = threadIdx.x;
     double s = 0.0;
-    for (int p = 0; p < np; p++) s += partial[(int64_t)p * nv + k];
-    out[k] = accumulate ? out[k] + s : s;
+    for (int p = tid; p < np; p += 256) s += partial[(int64_t)p * nv + k];
+    s = wave_sum(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        const double v = (red[0] + red[1]) + (red[2] + red[3]);
+        out[k] = accumulate ? out[k] + v : v;
+    }
 }
 
 // w[i] += sign * sum_k h[k] V[k * ldv + i]
@@ -79,6 +86,20 @@ __global__ void to_f64_kernel(const float *__restrict__ a, double *__restrict__ 
         b[i] = (double)a[i];
 }
 
+// out = (double) y + diag * x: the FP32 operator's result plus its diagonal
+// applied in FP64
+__global__ void f32_result_kernel(const float *__restrict__ y, const double *__restrict__ x, double diag,
+                                  double *__restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (double)y[i] + diag * x[i];
+}
+
+// y += diag * x in FP32 (the public FP32 matvec)
+__global__ void add_diag_f32_kernel(float *__restrict__ y, const float *__restrict__ x, float diag, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = fmaf(diag, x[i], y[i]);
+}
+
 // ---- launchers ----
 
 static inline unsigned grid_for(int64_t n) {
@@ -101,7 +122,7 @@ hipError_t launch_mdot(const double *V, int64_t ldv, int nv, const double *w, in
     const int64_t np = mdot_chunks(n);
     const int64_t chunk = (n + np - 1) / np;
     mdot_kernel<<<dim3((unsigned)np), 256, 0, s>>>(V, ldv, nv, w, n, chunk, partial);
-    mdot_finish_kernel<<<dim3((unsigned)((nv + 63) / 64)), 64, 0, s>>>(partial, (int)np, nv, out, accumulate);
+    mdot_finish_kernel<<<dim3((unsigned)nv), 256, 0, s>>>(partial, (int)np, nv, out, accumulate);
     return hipGetLastError();
 }
 
@@ -137,4 +158,15 @@ hipError_t launch_to_f64(const float *a, double *b, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+}  // namespace hodlr
+
+namespace hodlr {
+hipError_t launch_f32_result(const float *y, const double *x, double diag, double *out, int64_t n, hipStream_t s) {
+    f32_result_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(y, x, diag, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_add_diag_f32(float *y, const float *x, float diag, int64_t n, hipStream_t s) {
+    add_diag_f32_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(y, x, diag, n);
+    return hipGetLastError();
+}
 }  // namespace hodlr

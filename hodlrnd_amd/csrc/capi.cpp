@@ -303,7 +303,7 @@ int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes) {
     if (!op || !bytes) return HODLR_E_ARG;
     const double s = prec == HODLR_PREC_F32 ? 4.0 : 8.0, k = nrhs, n = (double)op->N;
     bytes[0] = s * (op->e_near + op->e_far) + 2.0 * k * s * n;
-    bytes[1] = s * (double)op->n_vp + k * s * (n + (double)op->n_t);
+    bytes[1] = s * op->e_v + k * s * (n + (double)op->n_t);
     bytes[2] = s * (double)(op->n_np + op->n_up) + k * s * (n + (double)op->n_t) + k * s * n;
     return HODLR_OK;
 }
@@ -338,6 +338,35 @@ int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms) 
         ms[0] = tot / reps;
         ms[1] = a / reps;
         ms[2] = b / reps;
+    });
+}
+
+int hodlr_op_profile(hodlr_op *op, int enable) {
+    if (!op) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        op->profile_collect();
+        op->profiling = enable != 0;
+    });
+}
+
+int hodlr_op_profile_read(hodlr_op *op, double *ms, int64_t *counts, int reset) {
+    if (!op) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        op->profile_collect();
+        if (ms) {
+            ms[0] = op->prof_ms[0][0];
+            ms[1] = op->prof_ms[0][1];
+            ms[2] = op->prof_ms[1][0];
+            ms[3] = op->prof_ms[1][1];
+        }
+        if (counts) {
+            counts[0] = op->prof_count[0];
+            counts[1] = op->prof_count[1];
+        }
+        if (reset) {
+            std::memset(op->prof_ms, 0, sizeof(op->prof_ms));
+            op->prof_count[0] = op->prof_count[1] = 0;
+        }
     });
 }
 

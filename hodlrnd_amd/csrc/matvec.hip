@@ -5,11 +5,13 @@
 // Here the operator is stored in three streaming arenas and applied by
 // three launches, each reading its arena exactly once:
 //   proj:   t = V^T x for every compressed block.  The V factors of all
-//           blocks that share a column node b form one row-major panel
-//           (n_b x T_b); a wave owns a 64-column tile and walks its rows, so
-//           each step is one coalesced 64-element read and x_b[j] is a
-//           broadcast.  Tall tiles are cut into row chunks whose partial
-//           sums are added by a second small pass (deterministic order).
+//           blocks that share a column node b form one panel (n_b x T_b,
+//           T_b padded to a multiple of 4) stored as 256-column tiles, each
+//           a contiguous row-major block; a workgroup streams one tile, each
+//           lane four adjacent columns (one vector load per row), and
+//           x_b[j] is a broadcast.  Tall tiles are cut
+//           into row chunks whose partial sums are added by a second small
+//           pass (deterministic order).
 //   tcopy:  dense-fallback blocks take t = x_b (their U panel holds K).
 //   panel:  y = sum over segments of M x_seg, one workgroup per leaf row
 //           tile.  A leaf's segments are its near-field blocks (self and
@@ -22,63 +24,111 @@
 
 namespace hodlr {
 
+// 4 consecutive elements, 16-byte (float) or 32-byte (double) aligned
+template <typename T>
+struct Vec4 {
+    T v[4];
+};
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ Vec4<T> load4(const T *p) {
+    Vec4<T> r;
+    if constexpr (sizeof(T) == 4) {
+        const f32x4 a = *reinterpret_cast<const f32x4 *>(p);
+        r.v[0] = a[0]; r.v[1] = a[1]; r.v[2] = a[2]; r.v[3] = a[3];
+    } else {
+        const f64x2 a = *reinterpret_cast<const f64x2 *>(p);
+        const f64x2 b = *reinterpret_cast<const f64x2 *>(p + 2);
+        r.v[0] = a[0]; r.v[1] = a[1]; r.v[2] = b[0]; r.v[3] = b[1];
+    }
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ T load_stream(const T *p) {
+    return *p;
+}
+
+// One workgroup per (V panel, 256-column tile, row chunk).  Lane l owns
+// columns 4l..4l+3 of the tile (one 16/32-byte load per row); the 4 waves
+// take rows w, w+4, ... and are summed through LDS at the end.  Panels are
+// padded to a multiple of 4 columns, so every row of a tile starts aligned.
 template <typename T, int K>
 __global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ items, int64_t nitems,
                                                    const T *__restrict__ VP, const T *__restrict__ x,
                                                    T *__restrict__ t, T *__restrict__ part,
                                                    const int64_t *__restrict__ tidx) {
+    constexpr int U = 4;  // rows in flight per wave per step
+    __shared__ T red[kProjTile][K];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const int64_t it = xcd_remap(blockIdx.x, gridDim.x) * 4 + wid;
-    if (it >= nitems) return;
-    const ProjItem P = items[it];
-    if (lane >= P.w) return;
-    T acc[K];
+    // items are sorted by size, largest first: the hardware's round-robin
+    // dispatch over XCDs then balances bytes per XCD
+    const ProjItem P = items[blockIdx.x];
+    const int c = lane * 4;
+    const bool live = c < P.w;
+    T acc[4][K];
 #pragma unroll
-    for (int k = 0; k < K; k++) acc[k] = T(0);
-    const T *vp = VP + P.vp + lane;
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[e][k] = T(0);
+    const T *base = VP + P.vp + c;
     const T *xv = x + P.x0 * K;
     const int64_t ld = P.ld;
-    int j = 0;
-    for (; j + 4 <= P.rows; j += 4) {
-        const T a0 = vp[(j + 0) * ld], a1 = vp[(j + 1) * ld];
-        const T a2 = vp[(j + 2) * ld], a3 = vp[(j + 3) * ld];
+    if (live) {
+        int j = wid;
+        for (; j + 4 * (U - 1) < P.rows; j += 4 * U) {
+            Vec4<T> a[U];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            acc[k] = fma(a0, xv[(j + 0) * K + k], acc[k]);
-            acc[k] = fma(a1, xv[(j + 1) * K + k], acc[k]);
-            acc[k] = fma(a2, xv[(j + 2) * K + k], acc[k]);
-            acc[k] = fma(a3, xv[(j + 3) * K + k], acc[k]);
+            for (int u = 0; u < U; u++) a[u] = load4(base + (int64_t)(j + 4 * u) * ld);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const T xj = xv[(int64_t)(j + 4 * u) * K + k];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) acc[e][k] = fma(a[u].v[e], xj, acc[e][k]);
+                }
+        }
+        for (; j < P.rows; j += 4) {
+            const Vec4<T> a = load4(base + (int64_t)j * ld);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const T xj = xv[(int64_t)j * K + k];
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[e][k] = fma(a.v[e], xj, acc[e][k]);
+            }
         }
     }
-    for (; j < P.rows; j++) {
-        const T a = vp[j * ld];
+    // waves add their partial sums into LDS one after another (fixed order)
+    for (int w = 0; w < 4; w++) {
+        if (wid == w)
 #pragma unroll
-        for (int k = 0; k < K; k++) acc[k] = fma(a, xv[j * K + k], acc[k]);
+            for (int e = 0; e < 4; e++)
+#pragma unroll
+                for (int k = 0; k < K; k++) red[c + e][k] = (w == 0 ? T(0) : red[c + e][k]) + acc[e][k];
+        __syncthreads();
     }
-    if (P.out >= 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) part[(P.out * kWave + lane) * K + k] = acc[k];
-    } else {
-        const int64_t ti = tidx[P.tcol + lane];
-#pragma unroll
-        for (int k = 0; k < K; k++) t[ti * K + k] = acc[k];
+    for (int q = threadIdx.x; q < P.w * K; q += blockDim.x) {
+        const int col = q / K, k = q % K;
+        if (P.out >= 0) part[(P.out * kProjTile + col) * K + k] = red[col][k];
+        else t[tidx[P.tcol + col] * K + k] = red[col][k];
     }
 }
 
 template <typename T, int K>
-__global__ __launch_bounds__(64) void proj_reduce_kernel(const ProjReduce *__restrict__ items,
-                                                         const T *__restrict__ part, T *__restrict__ t,
-                                                         const int64_t *__restrict__ tidx) {
+__global__ __launch_bounds__(256) void proj_reduce_kernel(const ProjReduce *__restrict__ items,
+                                                          const T *__restrict__ part, T *__restrict__ t,
+                                                          const int64_t *__restrict__ tidx) {
     const ProjReduce R = items[blockIdx.x];
-    const int lane = threadIdx.x;
-    if (lane >= R.w) return;
+    const int col = threadIdx.x;
+    if (col >= R.w) return;
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) acc[k] = T(0);
     for (int c = 0; c < R.nchunks; c++)
 #pragma unroll
-        for (int k = 0; k < K; k++) acc[k] += part[((R.slot0 + c) * kWave + lane) * K + k];
-    const int64_t ti = tidx[R.tcol + lane];
+        for (int k = 0; k < K; k++) acc[k] += part[((R.slot0 + c) * kProjTile + col) * K + k];
+    const int64_t ti = tidx[R.tcol + col];
 #pragma unroll
     for (int k = 0; k < K; k++) t[ti * K + k] = acc[k];
 }
@@ -118,25 +168,24 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelItem *__restrict_
         const T *M = (S.arena ? UP : NP) + S.mat + P.r0 + o;
         const T *v = S.vec < 0 ? x + (-S.vec - 1) * K : t + S.vec * K;
         if (!active) continue;
-        int q = gg;
         const int len = S.len;
-        for (; q + 3 * GG < len; q += 4 * GG) {
-            const T a0 = M[(int64_t)q * ld];
-            const T a1 = M[(int64_t)(q + GG) * ld];
-            const T a2 = M[(int64_t)(q + 2 * GG) * ld];
-            const T a3 = M[(int64_t)(q + 3 * GG) * ld];
+        // kU column steps per iteration, predicated, so short segments
+        // (near-field blocks of ~leaf-size width) still keep kU loads in flight
+        constexpr int kU = 8;
+        for (int q = gg; q < len; q += kU * GG) {
+            T a[kU], vv[kU][K];
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                acc[k] = fma(a0, v[(int64_t)q * K + k], acc[k]);
-                acc[k] = fma(a1, v[(int64_t)(q + GG) * K + k], acc[k]);
-                acc[k] = fma(a2, v[(int64_t)(q + 2 * GG) * K + k], acc[k]);
-                acc[k] = fma(a3, v[(int64_t)(q + 3 * GG) * K + k], acc[k]);
+            for (int u = 0; u < kU; u++) {
+                const int qu = q + u * GG;
+                const bool ok = qu < len;
+                a[u] = ok ? load_stream(M + (int64_t)qu * ld) : T(0);
+#pragma unroll
+                for (int k = 0; k < K; k++) vv[u][k] = ok ? v[(int64_t)qu * K + k] : T(0);
             }
-        }
-        for (; q < len; q += GG) {
-            const T a = M[(int64_t)q * ld];
 #pragma unroll
-            for (int k = 0; k < K; k++) acc[k] = fma(a, v[(int64_t)q * K + k], acc[k]);
+            for (int u = 0; u < kU; u++)
+#pragma unroll
+                for (int k = 0; k < K; k++) acc[k] = fma(a[u], vv[u][k], acc[k]);
         }
     }
 #pragma unroll
@@ -158,8 +207,8 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
                            const T *x, T *t, T *part, const int64_t *tidx, T *y,
                            hipEvent_t *ev, hipStream_t s) {
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (npi) proj_kernel<T, K><<<dim3((unsigned)((npi + 3) / 4)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
-    if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 64, 0, s>>>(pr, part, t, tidx);
+    if (npi) proj_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+    if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 256, 0, s>>>(pr, part, t, tidx);
     if (ntc) tcopy_kernel<T, K><<<dim3((unsigned)((ntc + 3) / 4)), 256, 0, s>>>(tc, ntc, x, t);
     if (ev) (void)hipEventRecord(ev[1], s);
     if (npa) panel_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);

@@ -10,14 +10,17 @@
 //                        (dense-fallback blocks contribute K_ab), stored
 //                        leaf-blocked: each leaf's rows form a contiguous
 //                        nc x R_a column-major slice;
-//        V panels    VP: per column node b the n_b x T_b row-major matrix of
-//                        the V factors of every compressed block (., b);
+//        V panels    VP: per column node b the n_b x T_b matrix of the V
+//                        factors of every compressed block (., b), stored as
+//                        256-column tiles, each a contiguous row-major block;
 //   4. fill pass: near field, dense fallbacks, and the ACA again, writing
 //      factors into place.
 #include "op.h"
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace hodlr {
@@ -26,6 +29,15 @@ namespace {
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// HODLR_VERBOSE=1 prints assembly progress to stderr
+bool verbose() {
+    static const bool v = [] {
+        const char *e = std::getenv("HODLR_VERBOSE");
+        return e && e[0] && e[0] != '0';
+    }();
+    return v;
 }
 
 int aca_block_size(int m, int n) {
@@ -94,6 +106,9 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
                              d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
                              op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
+        if (verbose())
+            std::fprintf(stderr, "[hodlr] aca %s pass: %zu/%zu blocks (workgroup %d, scratch %.2f GB)\n",
+                         fill ? "fill" : "rank", i, order.size(), bs, used * 8e-9);
     }
 }
 
@@ -209,6 +224,7 @@ void assemble(hodlr_op &op) {
     }
     const double t_rank1 = now_s();
 
+    if (verbose()) std::fprintf(stderr, "[hodlr] rank pass %.3f s, %lld blocks\n", t_rank1 - t_rank0, (long long)nb);
     // ---- 3. layout ----
     const int64_t nn = t.level_off.back();
     std::vector<int64_t> R(nn, 0), Tb(nn, 0), up_base(nn, 0), t_base(nn, 0), vp_base(nn, 0), vcol_base(nn, 0);
@@ -228,18 +244,23 @@ void assemble(hodlr_op &op) {
     op.n_up = op.n_vp = op.n_t = 0;
     int64_t nvcol = 0;
     op.e_u = op.e_v = 0;
+    // V panels are padded to a multiple of 4 columns so that every row of a
+    // projection tile starts on a 4-element boundary
+    std::vector<int64_t> Tpad(nn, 0);
     for (int64_t g = 0; g < nn; g++) {
+        Tpad[g] = (Tb[g] + 3) / 4 * 4;
         up_base[g] = op.n_up;
         op.n_up += t.count[g] * R[g];
         t_base[g] = op.n_t;
         op.n_t += R[g];
         vp_base[g] = op.n_vp;
-        op.n_vp += t.count[g] * Tb[g];
+        op.n_vp += t.count[g] * Tpad[g];
         vcol_base[g] = nvcol;
-        nvcol += Tb[g];
+        nvcol += Tpad[g];
     }
     op.e_u = (double)op.n_up;
-    op.e_v = (double)op.n_vp;
+    op.e_v = 0;
+    for (int64_t g = 0; g < nn; g++) op.e_v += (double)t.count[g] * Tb[g];
 
     // near field per non-empty leaf
     std::vector<NearLeaf> near;
@@ -316,7 +337,7 @@ void assemble(hodlr_op &op) {
         fd.R = (int32_t)R[a];
         fd.coff = coff[b];
         fd.vp_base = vp_base[c];
-        fd.T = (int32_t)Tb[c];
+        fd.T = (int32_t)Tpad[c];
         fd.voff = voff[b];
         dest[b] = fd;
         if (op.flags[b] & 1) {
@@ -342,13 +363,13 @@ void assemble(hodlr_op &op) {
     }
     std::vector<ProjItem> proj;
     std::vector<ProjReduce> red;
-    const int64_t kChunk = 2048;
+    const int64_t kChunk = kProjChunk;
     int64_t slots = 0;
     for (int64_t g = 0; g < nn; g++) {
         if (!Tb[g]) continue;
         const int64_t nr = t.count[g];
-        for (int64_t c0 = 0; c0 < Tb[g]; c0 += 64) {
-            const int w = (int)std::min<int64_t>(64, Tb[g] - c0);
+        for (int64_t c0 = 0; c0 < Tb[g]; c0 += kProjTile) {
+            const int w = (int)std::min<int64_t>(kProjTile, Tb[g] - c0);
             const int64_t nch = (nr + kChunk - 1) / kChunk;
             if (nch > 1) {
                 ProjReduce pr;
@@ -362,9 +383,10 @@ void assemble(hodlr_op &op) {
             for (int64_t ch = 0; ch < nch; ch++) {
                 const int64_t j0 = ch * kChunk, j1 = std::min(nr, j0 + kChunk);
                 ProjItem it;
-                it.vp = vp_base[g] + j0 * Tb[g] + c0;
+                const int64_t tw = std::min<int64_t>(kProjTile, Tpad[g] - c0);  // tile width in memory
+                it.vp = vp_base[g] + c0 * nr + j0 * tw;
                 it.x0 = t.start[g] + j0;
-                it.ld = (int32_t)Tb[g];
+                it.ld = (int32_t)tw;
                 it.rows = (int32_t)(j1 - j0);
                 it.w = w;
                 it.tcol = (int32_t)(vcol_base[g] + c0);
@@ -374,14 +396,21 @@ void assemble(hodlr_op &op) {
             if (nch > 1) slots += nch;
         }
     }
+    std::stable_sort(proj.begin(), proj.end(), [](const ProjItem &a, const ProjItem &b) {
+        return (int64_t)a.rows * a.ld > (int64_t)b.rows * b.ld;
+    });
     if (nvcol > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many V columns for 32-bit tile offsets"};
     op.n_slots = slots;
 
     // ---- 4. allocate and fill ----
     const double t_fill0 = now_s();
+    if (verbose())
+        std::fprintf(stderr, "[hodlr] layout: near %.3g, U %.3g, V %.3g entries (%.2f GB FP64)\n", (double)op.n_np,
+                     (double)op.n_up, (double)op.n_vp, 8e-9 * (double)(op.n_np + op.n_up + op.n_vp));
     op.NP64.alloc(sizeof(double) * op.n_np);
     op.UP64.alloc(sizeof(double) * op.n_up);
     op.VP64.alloc(sizeof(double) * op.n_vp);
+    HIP_CHECK(hipMemsetAsync(op.VP64.p, 0, sizeof(double) * op.n_vp, s));  // padding columns
     {
         DevBuf d_near, d_colsrc, d_dense, d_dest;
         upload(d_near, near);
@@ -414,6 +443,13 @@ void assemble(hodlr_op &op) {
         HIP_CHECK(launch_convert(op.NP64.as<double>(), op.NP32.as<float>(), op.n_np, s));
         HIP_CHECK(launch_convert(op.UP64.as<double>(), op.UP32.as<float>(), op.n_up, s));
         HIP_CHECK(launch_convert(op.VP64.as<double>(), op.VP32.as<float>(), op.n_vp, s));
+        // the diagonal (entry(i, i) = diag for every i) is applied separately
+        // in the working precision of the caller, so FP32 rounding of the
+        // identity part of a second-kind operator does not limit refinement
+        DevBuf d_near;
+        upload(d_near, near);
+        HIP_CHECK(launch_zero_self_diag(d_near.as<NearLeaf>(), (int64_t)near.size(), op.NP32.as<float>(), s));
+        HIP_CHECK(hipStreamSynchronize(s));
     }
     upload(op.proj, proj);
     upload(op.red, red);
@@ -461,19 +497,51 @@ void hodlr_op::ensure_work(int prec, int nrhs) {
     if (prec == HODLR_PREC_F64) {
         if (cap_rhs64 >= nrhs) return;
         t64.alloc(sizeof(double) * (size_t)(n_t > 0 ? n_t : 1) * nrhs);
-        part64.alloc(sizeof(double) * (size_t)(n_slots > 0 ? n_slots : 1) * 64 * nrhs);
+        part64.alloc(sizeof(double) * (size_t)(n_slots > 0 ? n_slots : 1) * kProjTile * nrhs);
         cap_rhs64 = nrhs;
     } else {
         if (cap_rhs32 >= nrhs) return;
         t32.alloc(sizeof(float) * (size_t)(n_t > 0 ? n_t : 1) * nrhs);
-        part32.alloc(sizeof(float) * (size_t)(n_slots > 0 ? n_slots : 1) * 64 * nrhs);
+        part32.alloc(sizeof(float) * (size_t)(n_slots > 0 ? n_slots : 1) * kProjTile * nrhs);
         cap_rhs32 = nrhs;
     }
 }
 
-void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev) {
+hodlr_op::~hodlr_op() {
+    for (auto &e : ev_pool) (void)hipEventDestroy(e);
+}
+
+void hodlr_op::profile_collect() {
+    using namespace hodlr;
+    if (ev_pending.empty()) return;
+    HIP_CHECK(hipEventSynchronize(ev_pool[ev_pending.back().second + 2]));
+    for (auto &pe : ev_pending) {
+        float a = 0, b = 0;
+        HIP_CHECK(hipEventElapsedTime(&a, ev_pool[pe.second], ev_pool[pe.second + 1]));
+        HIP_CHECK(hipEventElapsedTime(&b, ev_pool[pe.second + 1], ev_pool[pe.second + 2]));
+        const int p = pe.first == HODLR_PREC_F64 ? 0 : 1;
+        prof_ms[p][0] += a;
+        prof_ms[p][1] += b;
+        prof_count[p]++;
+    }
+    ev_pending.clear();
+}
+
+void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev,
+                          bool add_diag) {
     using namespace hodlr;
     ensure_work(prec, nrhs);
+    if (profiling && !ev) {
+        const size_t need = 3 * (ev_pending.size() + 1);
+        while (ev_pool.size() < need) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            ev_pool.push_back(e);
+        }
+        const size_t first = 3 * ev_pending.size();
+        ev_pending.push_back({prec, first});
+        ev = &ev_pool[first];
+    }
     if (prec == HODLR_PREC_F64) {
         HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
                                         tcopy.as<TCopy>(), n_tcopy, panel.as<PanelItem>(), n_panel,
@@ -487,5 +555,8 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
                                        segs.as<Segment>(), VP32.as<float>(), NP32.as<float>(), UP32.as<float>(),
                                        static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
                                        tidx.as<int64_t>(), static_cast<float *>(y), ev, s));
+        if (add_diag && kd.diag != 0.0)
+            HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x), (float)kd.diag,
+                                          (int64_t)N * nrhs, s));
     }
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <functional>
+#include <memory>
 
 #include "aca_layout.h"
 #include "device_common.h"
@@ -18,6 +19,9 @@ hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev &K
 hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev &K,
                             double *NP, hipStream_t s);
 hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s);
+hipError_t launch_zero_self_diag(const NearLeaf *l, int64_t nl, float *NP, hipStream_t s);
+hipError_t launch_f32_result(const float *y, const double *x, double diag, double *out, int64_t n, hipStream_t s);
+hipError_t launch_add_diag_f32(float *y, const float *x, float diag, int64_t n, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                          const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
@@ -69,7 +73,23 @@ struct hodlr_op {
 
     hodlr_op_stats stats{};
 
-    void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr);
+    // solver state kept between solves (Krylov basis, mixed-mode vectors)
+    std::shared_ptr<void> solver_ws;
+
+    // live kernel timing: event triplets recorded around the projection and
+    // panel launches of every matvec while profiling is on
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, size_t>> ev_pending;  // (precision, first event index)
+    double prof_ms[2][2] = {{0, 0}, {0, 0}};         // [prec f64/f32][proj/panel]
+    int64_t prof_count[2] = {0, 0};
+    void profile_collect();
+    ~hodlr_op();
+
+    // FP32: with add_diag false the result omits the diagonal (the FP32
+    // copy stores the self blocks with a zero diagonal; callers add diag x)
+    void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr,
+                    bool add_diag = true);
     void ensure_work(int prec, int nrhs);
 };
 

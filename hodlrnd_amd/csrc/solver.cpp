@@ -162,6 +162,18 @@ int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int 
     return max_iter;
 }
 
+struct SolveState {
+    Workspace ws;
+    DevBuf x32, y32, d, rr, w64;
+    SolveState(int64_t n, int restart) : ws(n, restart) {
+        x32.alloc(sizeof(float) * n);
+        y32.alloc(sizeof(float) * n);
+        d.alloc(sizeof(double) * n);
+        rr.alloc(sizeof(double) * n);
+        w64.alloc(sizeof(double) * n);
+    }
+};
+
 void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p, hodlr_gmres_info &info) {
     const double t0 = now_s();
     hipStream_t s = op.ctx->stream;
@@ -169,15 +181,23 @@ void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p
     const int restart = p.restart > 0 ? p.restart : 100;
     if (restart + 1 > 128) throw ArgError{HODLR_E_ARG, "restart must be at most 127"};
     const int max_iter = p.max_iter > 0 ? p.max_iter : (int)std::min<int64_t>(n, INT32_MAX);
-    Workspace ws(n, restart);
+    auto cached = std::static_pointer_cast<SolveState>(op.solver_ws);
+    if (!cached || cached->ws.restart != restart) {
+        op.solver_ws.reset();
+        cached = std::make_shared<SolveState>(n, restart);
+        op.solver_ws = cached;
+    }
+    Workspace &ws = cached->ws;
     std::memset(&info, 0, sizeof(info));
     std::vector<double> hist;
-    if (p.x0_is_b) HIP_CHECK(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (p.x0_is_b == 1) HIP_CHECK(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    const bool x0_zero = p.x0_is_b == 2;
+    if (x0_zero) HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
 
     Apply A64 = [&](const double *in, double *out) { op.matvec_dev(HODLR_PREC_F64, 1, in, out, s); };
     GmresResult gr;
     if (p.mode == HODLR_SOLVE_F64) {
-        gmres_device(A64, ws, b, x, max_iter, p.tol, false, s, gr, &hist);
+        gmres_device(A64, ws, b, x, max_iter, p.tol, x0_zero, s, gr, &hist);
         info.iterations = gr.iterations;
         info.matvecs_f64 = gr.matvecs;
         info.converged = gr.converged;
@@ -192,33 +212,42 @@ void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p
     } else {
         if (!(op.prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
         op.ensure_work(HODLR_PREC_F32, 1);
-        DevBuf x32, y32, d, rr, w64;
-        x32.alloc(sizeof(float) * n);
-        y32.alloc(sizeof(float) * n);
-        d.alloc(sizeof(double) * n);
-        rr.alloc(sizeof(double) * n);
-        w64.alloc(sizeof(double) * n);
+        DevBuf &x32 = cached->x32, &y32 = cached->y32, &d = cached->d, &rr = cached->rr, &w64 = cached->w64;
+        // FP32 off-diagonal operator, diagonal applied in FP64
         Apply A32 = [&](const double *in, double *out) {
             HIP_CHECK(launch_to_f32(in, x32.as<float>(), n, s));
-            op.matvec_dev(HODLR_PREC_F32, 1, x32.as<float>(), y32.as<float>(), s);
-            HIP_CHECK(launch_to_f64(y32.as<float>(), out, n, s));
+            op.matvec_dev(HODLR_PREC_F32, 1, x32.as<float>(), y32.as<float>(), s, nullptr, false);
+            HIP_CHECK(launch_f32_result(y32.as<float>(), in, op.kd.diag, out, n, s));
         };
         const double nb0 = ws.norm2(b, s);
         const double nb = nb0 == 0.0 ? 1.0 : nb0;
-        // r = b - A64 x
-        A64(x, w64.as<double>());
-        info.matvecs_f64++;
-        HIP_CHECK(launch_sub(b, w64.as<double>(), rr.as<double>(), n, s));
+        // The first correction is driven by r0 = b - A32 x0: its error
+        // (~ FP32 rounding of the operator) is removed by the next sweep,
+        // which, like every later one, starts from an FP64 residual.  The
+        // loop always ends on an FP64 residual, so the reported residual is
+        // certified against the FP64 operator.
+        if (x0_zero) {  // r0 = b exactly
+            HIP_CHECK(hipMemcpyAsync(rr.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        } else {
+            A32(x, w64.as<double>());
+            info.matvecs_f32++;
+            HIP_CHECK(launch_sub(b, w64.as<double>(), rr.as<double>(), n, s));
+        }
         double res = ws.norm2(rr.as<double>(), s) / nb;
+        bool res_is_f64 = x0_zero;
         hist.push_back(res);
         const int max_outer = p.max_outer > 0 ? p.max_outer : 20;
-        const double inner_tol = p.inner_tol > 0 ? p.inner_tol : 1e-6;
+        const double inner_floor = p.inner_tol > 0 ? p.inner_tol : 1e-14;
         int total = 0;
-        while (!(res < p.tol) && info.outer < max_outer && total < max_iter) {
+        while (!(res_is_f64 && res < p.tol) && info.outer < max_outer && total < max_iter) {
             GmresResult g;
             g.matvecs = 0;
             HIP_CHECK(hipMemsetAsync(d.p, 0, sizeof(double) * n, s));
-            gmres_device(A32, ws, rr.as<double>(), d.as<double>(), max_iter - total, inner_tol, true, s, g,
+            // aim the sweep at the outer tolerance: relative to ||r||, the inner
+            // solve must reach 0.3 tol ||b|| / ||r|| (never looser than 0.5,
+            // never tighter than the caller's floor inner_tol)
+            const double target = std::min(0.5, std::max(inner_floor, 0.3 * p.tol / std::max(res, 1e-300)));
+            gmres_device(A32, ws, rr.as<double>(), d.as<double>(), max_iter - total, target, true, s, g,
                          nullptr);
             total += g.iterations;
             info.matvecs_f32 += g.matvecs;
@@ -227,9 +256,16 @@ void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p
             info.matvecs_f64++;
             HIP_CHECK(launch_sub(b, w64.as<double>(), rr.as<double>(), n, s));
             res = ws.norm2(rr.as<double>(), s) / nb;
+            res_is_f64 = true;
             hist.push_back(res);
             info.outer++;
             if (g.iterations == 0) break;  // no progress possible at this precision
+        }
+        if (!res_is_f64) {  // converged before any sweep: certify in FP64
+            A64(x, w64.as<double>());
+            info.matvecs_f64++;
+            HIP_CHECK(launch_sub(b, w64.as<double>(), rr.as<double>(), n, s));
+            res = ws.norm2(rr.as<double>(), s) / nb;
         }
         info.iterations = total;
         info.converged = res < p.tol;

@@ -100,8 +100,12 @@ typedef struct {
     int32_t max_iter;      /* reference solve uses N */
     double tol;            /* relative residual (reference solve uses eps_ACA) */
     int32_t mode;          /* HODLR_SOLVE_F64 or HODLR_SOLVE_MIXED */
-    int32_t x0_is_b;       /* 1: initial guess x0 = b (HDD_matrix.hpp:52-53) */
-    double inner_tol;      /* mixed mode: relative tolerance of each FP32 solve */
+    int32_t x0_is_b;       /* 1: initial guess x0 = b (HDD_matrix.hpp:52-53);
+                              2: x0 = 0 (the first residual is b itself);
+                              0: x holds the initial guess */
+    double inner_tol;      /* mixed mode: tightest relative tolerance an FP32 sweep may be
+                              asked for (each sweep targets 0.3 tol ||b|| / ||r||);
+                              0 = no floor */
     int32_t max_outer;     /* mixed mode: refinement sweeps */
     int32_t verify;        /* F64 mode: recompute ||b - A x|| / ||b|| at exit */
     int32_t hist_cap;      /* length of hist (may be 0) */
@@ -180,6 +184,12 @@ int hodlr_gmres_device(hodlr_op *op, const double *b, double *x,
  * kernel; ms[0] = whole matvec, ms[1] = projection kernel (V^T x),
  * ms[2] = panel kernel (near field + U t), averages per launch. */
 int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms);
+/* Live timing: while enabled, every matvec records HIP events around its
+ * projection and panel launches on the stream it runs on.  read returns the
+ * summed milliseconds ms[4] = {f64 proj, f64 panel, f32 proj, f32 panel} and
+ * counts[2] = matvecs timed {f64, f32} since the last reset. */
+int hodlr_op_profile(hodlr_op *op, int enable);
+int hodlr_op_profile_read(hodlr_op *op, double *ms, int64_t *counts, int reset);
 /* algorithmic HBM bytes of one launch of each matvec kernel (same order) */
 int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes);
 

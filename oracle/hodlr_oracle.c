@@ -472,11 +472,15 @@ static int mid_gap(const int *ind, int cnt, int *tmp) {
     return tmp[at] + (int)(mx / 2);
 }
 
-int or_aca(const or_kernel_t *k, const double *pts,
-           const int64_t *rows, int m, const int64_t *cols, int n,
-           double eps, uint64_t seed, int cap, double *U, double *V,
-           int32_t *row_piv, int32_t *col_piv, or_aca_info_t *info) {
+/* Storage of the factors grows on demand (*U, *V realloc'd, *cap updated);
+ * row_piv / col_piv, when given, must hold min(m, n) entries. */
+static int aca_grow(const or_kernel_t *k, const double *pts,
+                    const int64_t *rows, int m, const int64_t *cols, int n,
+                    double eps, uint64_t seed, int *capp, double **Up, double **Vp,
+                    int growable, int32_t *row_piv, int32_t *col_piv, or_aca_info_t *info) {
     const double tol = eps;
+    int cap = *capp;
+    double *U = *Up, *V = *Vp;
     const int max_tries = 10;
     int mn = m < n ? m : n;
     char *rused = (char *)calloc((size_t)m, 1);
@@ -571,7 +575,17 @@ int or_aca(const or_kernel_t *k, const double *pts,
         }
         row_ind[nri++] = pivot; /* :318-319 */
         if (!rused[pivot]) { rused[pivot] = 1; rrem--; }
-        if (r >= cap) break; /* caller's storage is full (never for cap >= min(m,n)) */
+        if (r >= cap) {
+            if (!growable) break; /* caller's storage is full (never for cap >= min(m,n)) */
+            int nc = 2 * cap < mn ? 2 * cap : mn;
+            if (nc < 8) nc = mn < 8 ? mn : 8;
+            U = (double *)realloc(U, sizeof(double) * (size_t)m * (size_t)nc);
+            V = (double *)realloc(V, sizeof(double) * (size_t)n * (size_t)nc);
+            cap = nc;
+            *Up = U;
+            *Vp = V;
+            *capp = cap;
+        }
         double rs = 0.0, cs = 0.0;
         for (int j = 0; j < n; j++) { V[(size_t)r * n + j] = row[j]; rs += row[j] * row[j]; }
         for (int i = 0; i < m; i++) { U[(size_t)r * m + i] = gamma * col[i]; cs += col[i] * col[i]; }
@@ -600,6 +614,13 @@ int or_aca(const or_kernel_t *k, const double *pts,
     }
     free(rused); free(cused); free(row_ind); free(col_ind); free(tmp); free(row); free(col);
     return dense ? mn : r;
+}
+
+int or_aca(const or_kernel_t *k, const double *pts,
+           const int64_t *rows, int m, const int64_t *cols, int n,
+           double eps, uint64_t seed, int cap, double *U, double *V,
+           int32_t *row_piv, int32_t *col_piv, or_aca_info_t *info) {
+    return aca_grow(k, pts, rows, m, cols, n, eps, seed, &cap, &U, &V, 0, row_piv, col_piv, info);
 }
 
 /* ------------------------------------------------------------------ */
@@ -758,13 +779,14 @@ static void build_block(or_op_t *op, or_block_t *b) {
     const or_node_t *cn = &t->lv[b->level][b->col_node];
     int m = (int)rn->n, n = (int)cn->n;
     int mn = m < n ? m : n;
-    double *U = (double *)malloc(sizeof(double) * (size_t)m * (size_t)mn + 8);
-    double *V = (double *)malloc(sizeof(double) * (size_t)n * (size_t)mn + 8);
+    int cap = mn < 32 ? mn : 32;
+    double *U = (double *)malloc(sizeof(double) * (size_t)m * (size_t)cap + 8);
+    double *V = (double *)malloc(sizeof(double) * (size_t)n * (size_t)cap + 8);
     int32_t *rp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(mn + 1));
     int32_t *cp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(mn + 1));
     or_aca_info_t info;
     uint64_t seed = or_block_seed(b->level, b->row_node, b->col_node);
-    or_aca(&op->k, op->pts, rn->idx, m, cn->idx, n, op->eps, seed, mn, U, V, rp, cp, &info);
+    aca_grow(&op->k, op->pts, rn->idx, m, cn->idx, n, op->eps, seed, &cap, &U, &V, 1, rp, cp, &info);
     b->m = m;
     b->n = n;
     b->rank = info.rank;
