@@ -187,3 +187,22 @@ def test_empty_leaves_and_weights():
     y = A * x
     assert relerr(y, ref.matvec(x)) < 1e-12
     assert relerr(y, O.dense_matvec(ok, pts, x)) < 1e-6
+
+
+def test_cpp_api_reference_driver():
+    """hodlrnd_amd/examples/main_4d.cpp: include/main.cpp's flow through the
+    reference-shaped C++ API (kernel_4d_test, numPoints 8, N = 4096) with the
+    driver's own dense self-check."""
+    import os
+    import re
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "hodlrnd_amd", "examples", "build", "main_4d")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    say("case cpp_api_driver")
+    r = subprocess.run([exe, "8"], capture_output=True, text=True, timeout=120, cwd="/tmp")
+    say(r.stdout[-800:])
+    assert r.returncode == 0, r.stderr
+    mv = float(re.search(r"Relative Error in hmatrix matvec \.\.\. (\S+)", r.stdout).group(1))
+    sol = float(re.search(r"Relative Error in GMRES solution \.\.\. (\S+)", r.stdout).group(1))
+    assert mv < 1e-8 and sol < 1e-8

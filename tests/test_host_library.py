@@ -1,0 +1,155 @@
+"""CPU tests of libhodlrnd's host side through the C ABI (no GPU needed).
+
+A host-only context (device -1) builds trees and lists; these must match the
+FP64 oracle bit-exactly (ids, point order, list order).  Everything that
+needs the GPU must fail with HODLR_E_STATE, never fall back to the CPU.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import hodlrnd_amd as H
+from oracle import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "hodlrnd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(hodlr_\w+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 25
+    lib = C.CDLL(H.LIBPATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding knows every one of them
+    assert set(names) <= set(H.SIGNATURES), set(names) - set(H.SIGNATURES)
+
+
+def test_version_string():
+    assert b"gfx950" in H.lib.hodlr_version()
+
+
+@pytest.fixture(scope="module")
+def hctx():
+    return H.Context(-1)
+
+
+CASES = [(3, 10000, 64, 0), (2, 5000, 32, 0), (2, 3000, 16, -1), (4, 10000, 64, 0),
+         (3, 8000, 32, 1), (4, 4096, 16, 3), (3, 8000, 32, 2), (3, 8000, 32, -1), (1, 500, 8, 0)]
+
+
+@pytest.mark.parametrize("nd,n,nmax,dprime", CASES)
+def test_tree_bit_exact_vs_oracle(hctx, nd, n, nmax, dprime):
+    pts = O.splitmix_uniform(nd * 1000 + n, n * nd).reshape(n, nd)
+    lo, hi = -np.ones(nd), np.ones(nd)
+    T = H.Tree(pts, lo, hi, nmax, dprime, hctx)
+    R = O.Tree(pts, lo, hi, nmax, dprime)
+    assert T.nlevels == R.nlevels
+    perm = T.perm()
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    lo_, st, ct = T.nodes()
+    nb, ib = T.lists(0)
+    ni, ii = T.lists(1)
+    for l in range(T.nlevels):
+        for j in range(R.level_size(l)):
+            g = lo_[l] + j
+            seg = perm[st[g]:st[g] + ct[g]]
+            ref = R.points(l, j)
+            assert np.array_equal(np.sort(seg), ref)
+            if l == T.nlevels - 1:
+                assert np.array_equal(seg, ref)  # leaves keep ascending ids
+            assert np.array_equal(ib[nb[g]:nb[g + 1]], R.neighbours(l, j))
+            assert np.array_equal(ii[ni[g]:ni[g + 1]], R.interactions(l, j))
+
+
+def test_tree_grid_ties_and_empty_boxes(hctx):
+    """Tensor grids put points exactly on split planes (x < mid goes low) and
+    Gauss-Legendre grids leave boxes empty: both as the reference."""
+    for pts, nmax, dp in [(O.tensor_grid(3, O.uniform_nodes(-1, 1, 17)), 32, 0),
+                          (O.tensor_grid(4, O.gauss_legendre(8)[0]), 16, 0),
+                          (O.tensor_grid(2, O.uniform_nodes(-1, 1, 40)), 16, -1)]:
+        nd = pts.shape[1]
+        T = H.Tree(pts, -np.ones(nd), np.ones(nd), nmax, dp, hctx)
+        R = O.Tree(pts, -np.ones(nd), np.ones(nd), nmax, dp)
+        lo_, st, ct = T.nodes()
+        L = T.nlevels - 1
+        assert np.array_equal(ct[lo_[L]:lo_[L + 1]], [R.npts(L, j) for j in range(R.level_size(L))])
+        ni, ii = T.lists(1)
+        for j in range(R.level_size(L)):
+            g = lo_[L] + j
+            assert np.array_equal(ii[ni[g]:ni[g + 1]], R.interactions(L, j))
+
+
+def test_depth_zero_tree(hctx):
+    pts = O.splitmix_uniform(1, 90).reshape(30, 3)
+    T = H.Tree(pts, -np.ones(3), np.ones(3), 64, 0, hctx)
+    assert T.nlevels == 1 and T.info.n_admissible == 0
+
+
+def test_host_only_context_refuses_device_work(hctx):
+    pts = O.splitmix_uniform(2, 600).reshape(200, 3)
+    T = H.Tree(pts, -np.ones(3), np.ones(3), 32, 0, hctx)
+    k = H.Kernel(H.KERNEL_INV_R, 3, 1.0 / 200, 1.0)
+    with pytest.raises(H.HodlrError) as e:
+        H.Operator(T, k, 1e-8)
+    assert e.value.code == H.E_STATE
+
+
+def test_argument_errors(hctx):
+    pts = np.zeros((4, 9))
+    with pytest.raises(H.HodlrError) as e:
+        H.Tree(pts, -np.ones(9), np.ones(9), 8, 0, hctx)
+    assert e.value.code == H.E_ARG
+    with pytest.raises(H.HodlrError) as e:
+        H.Tree(np.zeros((4, 2)), -np.ones(2), np.ones(2), 0, 0, hctx)
+    assert e.value.code == H.E_ARG
+    pts = O.splitmix_uniform(3, 300).reshape(100, 3)
+    T = H.Tree(pts, -np.ones(3), np.ones(3), 32, 0, hctx)
+    kd = H.KernelDesc(H.KERNEL_INV_R, 2, 1.0, 1.0, None)  # ndim mismatch
+    h = C.c_void_p()
+    assert H.lib.hodlr_op_assemble(hctx.h, T.h, C.byref(kd), 1e-8, 3, C.byref(h)) == H.E_ARG
+    kd = H.KernelDesc(7, 3, 1.0, 1.0, None)  # unknown family
+    assert H.lib.hodlr_op_assemble(hctx.h, T.h, C.byref(kd), 1e-8, 3, C.byref(h)) == H.E_ARG
+    kd = H.KernelDesc(H.KERNEL_INV_R, 3, 1.0, 1.0, None)
+    assert H.lib.hodlr_op_assemble(hctx.h, T.h, C.byref(kd), 2.0, 3, C.byref(h)) == H.E_ARG
+    assert H.lib.hodlr_op_assemble(None, T.h, C.byref(kd), 1e-8, 3, C.byref(h)) == H.E_ARG
+
+
+def test_tree_order_round_trip(hctx):
+    pts = O.splitmix_uniform(4, 1500).reshape(500, 3)
+    T = H.Tree(pts, -np.ones(3), np.ones(3), 16, 0, hctx)
+    X = np.asfortranarray(np.stack([O.splitmix_uniform(9 + c, 500) for c in range(3)], 1))
+    tree_order = np.empty(500 * 3)
+    back = np.empty_like(X, order="F")
+    assert H.lib.hodlr_to_tree_order(T.h, 3, H._p(X), H._p(tree_order)) == 0
+    assert H.lib.hodlr_from_tree_order(T.h, 3, H._p(tree_order), H._p(back)) == 0
+    assert np.array_equal(back, X)
+    perm = T.perm()
+    assert np.array_equal(tree_order.reshape(500, 3)[:, 1], X[perm, 1])
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_cpp_api_example_compiles(tmp_path):
+    """The reference-shaped C++ API (include/hodlrnd/HODLRnD.hpp) builds with
+    a plain host compiler against the C ABI."""
+    out = tmp_path / "main_4d"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "hodlrnd_amd", "examples", "main_4d.cpp"), "-o", str(out),
+                        "-L" + os.path.join(ROOT, "hodlrnd_amd"), "-lhodlrnd"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
