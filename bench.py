@@ -63,6 +63,22 @@ def splitmix_uniform(seed, n):
     return 2.0 * out - 1.0
 
 
+def measured_traffic(kernel, alg_bytes):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel factors),
+    used only if it was measured on this exact operator (same algorithmic
+    bytes per launch); otherwise None."""
+    path = os.path.join(ROOT, "profiles", "matvec_traffic_grid100_N1e6.json")
+    try:
+        d = json.load(open(path))
+        e = d["kernels"][kernel]
+        if abs(e["algorithmic_bytes_per_launch"] - alg_bytes) <= 1e-3 * alg_bytes:
+            return float(e["hbm_bytes_per_launch"]), "profiles/matvec_traffic_grid100_N1e6.json"
+    except (OSError, KeyError, ValueError):
+        pass
+    return None, None
+
+
 def cpu_baseline(pts, n_mv, nmax, eps, budget_s=20.0):
     """Times the FP64 CPU restatement on a stratified sample of row nodes."""
     from oracle import pyoracle as O
@@ -125,6 +141,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the library's plain copy kernel (1 GiB, 3 launches): the "
+                         "known-size kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one "
@@ -133,6 +152,13 @@ def main():
 
     import hodlrnd_amd as H
 
+    if args.copy_calibration:
+        g = H.stream_copy_gbps(1 << 30, 3)
+        print(json.dumps({"copy_kernel": "hodlr::stream_copy_kernel", "bytes_read_per_launch": 1 << 30,
+                          "bytes_written_per_launch": 1 << 30, "launches": 4, "gbps": g}))
+        return
+
+    copy_gbs = H.stream_copy_gbps(1 << 30, 5)
     n1 = args.n1d
     pts = make_points(n1)
     n = len(pts)
@@ -196,6 +222,7 @@ def main():
     name, tot_ms, c, bytes_ = max(cands, key=lambda t: t[1])
     avg_ms = tot_ms / max(c, 1)
     achieved = bytes_ / (avg_ms * 1e-3) * 1e-9
+    traffic, traffic_src = measured_traffic(name, bytes_)
     mv32 = (ms[2] + ms[3]) / max(cnt[1], 1)
     mv64 = (ms[0] + ms[1]) / max(cnt[0], 1)
 
@@ -252,7 +279,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "measured_copy_gbs": copy_gbs,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": bytes_,
             "avg_launch_ms": avg_ms,
         },

@@ -92,6 +92,7 @@ SIGNATURES = {
     "hodlr_matvec_kernel_bytes": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "hodlr_op_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "hodlr_op_profile_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "hodlr_stream_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p]),
     "hodlr_device_alloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "hodlr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
@@ -144,6 +145,14 @@ class Context:
 
 
 _default_ctx = None
+
+
+def stream_copy_gbps(nbytes=1 << 30, reps=5, ctx=None):
+    """Measured device copy bandwidth (read + write bytes per second, GB/s)."""
+    ctx = ctx or default_context()
+    g = C.c_double()
+    ctx.check(lib.hodlr_stream_copy(ctx.h, int(nbytes), reps, C.byref(g)), "stream_copy")
+    return g.value
 
 
 def default_context():

@@ -89,12 +89,14 @@ struct PanelItem {
     int32_t r0;           // row offset of the tile inside the leaf
 };
 
-// one contiguous ld x len column-major matrix times a slice of x or t
+// one contiguous ld x len column-major matrix times a vector slice:
+//   arena 0 (near-field block): column q multiplies x[vec + q];
+//   arena 1 (U panel slice):    column q multiplies t[vec + q]
 struct Segment {
     int64_t mat;          // element offset in the arena (row 0 of the leaf)
-    int64_t vec;          // < 0: x[-vec-1 ..]; >= 0: t[vec ..]
+    int64_t vec;
     int32_t len;
-    int32_t arena;        // 0 near-field arena, 1 U arena
+    int32_t arena;
 };
 
 }  // namespace hodlr

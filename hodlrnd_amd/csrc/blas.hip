@@ -170,3 +170,26 @@ hipError_t launch_add_diag_f32(float *y, const float *x, float diag, int64_t n, 
     return hipGetLastError();
 }
 }  // namespace hodlr
+
+namespace hodlr {
+// plain streaming copy, 16 bytes per lane per step: the bandwidth ceiling
+// the matvec kernels are compared with, and the known-size kernel used to
+// scale hardware byte counters
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                                          int64_t n4) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {  // four loads in flight per lane
+        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        b[i] = v0;
+        b[i + stride] = v1;
+        b[i + 2 * stride] = v2;
+        b[i + 3 * stride] = v3;
+    }
+    for (; i < n4; i += stride) b[i] = a[i];
+}
+hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s) {
+    stream_copy_kernel<<<dim3(4096), 256, 0, s>>>(static_cast<const float4 *>(a), static_cast<float4 *>(b), bytes / 16);
+    return hipGetLastError();
+}
+}  // namespace hodlr

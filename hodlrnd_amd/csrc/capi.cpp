@@ -370,6 +370,32 @@ int hodlr_op_profile_read(hodlr_op *op, double *ms, int64_t *counts, int reset) 
     });
 }
 
+int hodlr_stream_copy(hodlr_ctx *ctx, int64_t bytes, int reps, double *gbps) {
+    if (!ctx || bytes < 16 || reps < 1 || !gbps) return HODLR_E_ARG;
+    return guard(ctx, [&] {
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        bytes = bytes / 16 * 16;
+        DevBuf a, b;
+        a.alloc((size_t)bytes);
+        b.alloc((size_t)bytes);
+        HIP_CHECK(hipMemsetAsync(a.p, 1, (size_t)bytes, s));
+        HIP_CHECK(launch_stream_copy(a.p, b.p, bytes, s));  // warm-up
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) HIP_CHECK(launch_stream_copy(a.p, b.p, bytes, s));
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) * 1e-9;
+    });
+}
+
 int hodlr_device_alloc(hodlr_ctx *ctx, int64_t bytes, void **ptr) {
     if (!ctx || !ptr || bytes < 0) return HODLR_E_ARG;
     return guard(ctx, [&] {

@@ -15,10 +15,10 @@
 //   tcopy:  dense-fallback blocks take t = x_b (their U panel holds K).
 //   panel:  y = sum over segments of M x_seg, one workgroup per leaf row
 //           tile.  A leaf's segments are its near-field blocks (self and
-//           neighbours) and one slice of the U panel of each ancestor,
-//           all stored column-major with leading dimension = leaf size, so
-//           G = 64/h column groups of h lanes read G*h contiguous elements
-//           per step.  Each leaf row is written once (no atomics).
+//           neighbours) and one slice of the U panel of each ancestor, all
+//           column-major with leading dimension = leaf size, so G = 64/h
+//           column groups of h lanes read G*h contiguous elements per step.
+//           Each leaf row is written once (no atomics).
 #include "aca_layout.h"
 #include "device_common.h"
 
@@ -165,8 +165,9 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelItem *__restrict_
     for (int k = 0; k < K; k++) acc[k] = T(0);
     for (int s = 0; s < P.nseg; s++) {
         const Segment S = segs[P.seg0 + s];
-        const T *M = (S.arena ? UP : NP) + S.mat + P.r0 + o;
-        const T *v = S.vec < 0 ? x + (-S.vec - 1) * K : t + S.vec * K;
+        const bool near = S.arena == 0;
+        const T *M = (near ? NP : UP) + S.mat + P.r0 + o;
+        const T *v = (near ? x : t) + S.vec * K;
         if (!active) continue;
         const int len = S.len;
         // kU column steps per iteration, predicated, so short segments

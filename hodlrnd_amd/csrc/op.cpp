@@ -280,12 +280,14 @@ void assemble(hodlr_op &op) {
         nl.nc = (int32_t)nc;
         const int64_t seg0 = (int64_t)segs.size();
         int64_t ncols = 0;
+        // one segment per non-empty neighbour block (self first), each
+        // multiplying a contiguous slice of x
         auto add_near = [&](int64_t gb) {
             const int64_t c = t.count[gb];
             if (!c) return;
             Segment sg;
             sg.mat = op.n_np + ncols * nc;
-            sg.vec = -t.start[gb] - 1;
+            sg.vec = t.start[gb];
             sg.len = (int32_t)c;
             sg.arena = 0;
             segs.push_back(sg);

@@ -22,6 +22,7 @@ hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s);
 hipError_t launch_zero_self_diag(const NearLeaf *l, int64_t nl, float *NP, hipStream_t s);
 hipError_t launch_f32_result(const float *y, const double *x, double diag, double *out, int64_t n, hipStream_t s);
 hipError_t launch_add_diag_f32(float *y, const float *x, float diag, int64_t n, hipStream_t s);
+hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                          const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,

@@ -193,6 +193,13 @@ int hodlr_op_profile_read(hodlr_op *op, double *ms, int64_t *counts, int reset);
 /* algorithmic HBM bytes of one launch of each matvec kernel (same order) */
 int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes);
 
+/* Measurement helper with no counterpart in the reference: copies `bytes`
+ * between two device buffers `reps` times with a plain streaming kernel and
+ * returns the achieved (read + write) GB/s -- the measured ceiling that the
+ * matvec's HBM figures are compared with, and the known-size kernel used to
+ * scale the FETCH_SIZE / WRITE_SIZE hardware counters. */
+int hodlr_stream_copy(hodlr_ctx *ctx, int64_t bytes, int reps, double *gbps);
+
 /* device memory helpers for callers that keep vectors resident */
 int hodlr_device_alloc(hodlr_ctx *ctx, int64_t bytes, void **ptr);
 int hodlr_device_free(hodlr_ctx *ctx, void *ptr);
