@@ -80,6 +80,12 @@ SIGNATURES = {
     "hodlr_tree_destroy": (None, [C.c_void_p]),
     "hodlr_op_assemble": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                     C.c_void_p]),
+    "hodlr_op_assemble_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                                          C.c_int, C.c_void_p]),
+    "hodlr_op_problem_offsets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "hodlr_gmres_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hodlr_gmres_batch_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
     "hodlr_op_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_op_block_ranks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "hodlr_op_destroy": (None, [C.c_void_p]),
@@ -262,16 +268,60 @@ def kernel_4d_test(num_points, kii):
 
 
 class Operator:
-    """Assembled HODLR operator (HDD_matrix.hpp:39-47, GMRES.hpp:160-303)."""
+    """Assembled HODLR operator (HDD_matrix.hpp:39-47, GMRES.hpp:160-303).
+    With lists of trees and kernels: the block-diagonal operator of a batch
+    of independent problems (hodlr_op_assemble_batch)."""
 
     def __init__(self, tree, kernel, eps_aca, prec=PREC_F64 | PREC_F32):
-        self.tree, self.kernel, self.ctx = tree, kernel, tree.ctx
         self.h = C.c_void_p()
-        kd = kernel.desc()
-        self.ctx.check(lib.hodlr_op_assemble(self.ctx.h, tree.h, C.byref(kd), eps_aca, prec,
-                                             C.byref(self.h)), "op_assemble")
-        self.n = tree.n
+        if isinstance(tree, (list, tuple)):
+            self.trees, self.kernels = list(tree), list(kernel)
+            self.ctx = self.trees[0].ctx
+            arr = (C.c_void_p * len(self.trees))(*[t.h.value for t in self.trees])
+            kds = (KernelDesc * len(self.kernels))(*[k.desc() for k in self.kernels])
+            self.ctx.check(lib.hodlr_op_assemble_batch(self.ctx.h, len(self.trees), arr, kds, eps_aca, prec,
+                                                       C.byref(self.h)), "op_assemble_batch")
+            self.tree, self.kernel = self.trees[0], self.kernels[0]
+        else:
+            self.trees, self.kernels = [tree], [kernel]
+            self.tree, self.kernel, self.ctx = tree, kernel, tree.ctx
+            kd = kernel.desc()
+            self.ctx.check(lib.hodlr_op_assemble(self.ctx.h, tree.h, C.byref(kd), eps_aca, prec,
+                                                 C.byref(self.h)), "op_assemble")
+        self.n = sum(t.n for t in self.trees)
         self.prec = prec
+
+    def offsets(self):
+        out = np.zeros(len(self.trees) + 1, dtype=np.int64)
+        lib.hodlr_op_problem_offsets(self.h, _p(out), len(out))
+        return out
+
+    def gmres_batch(self, B, restart=100, max_iter=None, tol=1e-8, mode=SOLVE_F64, x0="b",
+                    inner_tol=0.0, max_outer=20, X0=None):
+        """Every (problem, column) of B (N x nrhs, problems concatenated in
+        point order) solved independently; returns X and per-instance infos."""
+        B = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(self.n, -1))
+        nrhs = B.shape[1]
+        x0code = {"b": 1, "zero": 2, "given": 0}[x0]
+        X = np.asfortranarray(B.copy() if X0 is None else np.asarray(X0, dtype=np.float64).reshape(self.n, -1))
+        p = GmresParams(restart, 0 if max_iter is None else max_iter, tol, mode, x0code, inner_tol,
+                        max_outer, 0, 0, None)
+        infos = (GmresInfo * (len(self.trees) * nrhs))()
+        rc = lib.hodlr_gmres_batch(self.h, nrhs, _p(B), _p(X), C.byref(p), infos)
+        if rc not in (OK, E_NOCONV):
+            self.ctx.check(rc, "gmres_batch")
+        return X, [{f: getattr(i, f) for f, _ in GmresInfo._fields_} for i in infos]
+
+    def gmres_batch_device(self, b_dev, x_dev, nrhs, restart=100, max_iter=None, tol=1e-8,
+                           mode=SOLVE_F64, x0="b", inner_tol=0.0, max_outer=20):
+        x0code = {"b": 1, "zero": 2, "given": 0}[x0]
+        p = GmresParams(restart, 0 if max_iter is None else max_iter, tol, mode, x0code, inner_tol,
+                        max_outer, 0, 0, None)
+        infos = (GmresInfo * (len(self.trees) * nrhs))()
+        rc = lib.hodlr_gmres_batch_device(self.h, nrhs, b_dev, x_dev, C.byref(p), infos)
+        if rc not in (OK, E_NOCONV):
+            self.ctx.check(rc, "gmres_batch_device")
+        return [{f: getattr(i, f) for f, _ in GmresInfo._fields_} for i in infos]
 
     def __del__(self):
         if getattr(self, "h", None):

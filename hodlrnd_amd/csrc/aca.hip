@@ -189,7 +189,8 @@ __device__ __forceinline__ uint64_t splitmix64_next(uint64_t &s) {
 }  // namespace
 
 template <int BS>
-__global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ blocks, KernelDev K,
+__global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ blocks,
+                                                 const KernelDev *__restrict__ Ks,
                                                  double tol, double *__restrict__ pool,
                                                  int32_t *__restrict__ out_rank,
                                                  int32_t *__restrict__ out_flags, int fill,
@@ -199,6 +200,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
                                                  double *__restrict__ UP, double *__restrict__ VP) {
     __shared__ Shared<BS> sh;
     const AcaBlock B = blocks[blockIdx.x];
+    const KernelDev K = Ks[B.prob];
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     const int m = B.m, n = B.n, cap = B.cap;
@@ -226,11 +228,11 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     // residual row ri of the block (LowRank_matrix.hpp:126-133); argmax over
     // the columns not yet used as pivots
     auto eval_row = [&](int ri) {
-        const int64_t pi = B.ord[B.row0 + ri];
+        const int64_t pi = B.ord[B.row0l + ri];
         double bv = 0.0, ba = -1.0;
         int bi = INT_MAX;
         for (int j = tid; j < n; j += BS) {
-            double v = entry(K, pi, B.ord[B.col0 + j]);
+            double v = entry(K, pi, B.ord[B.col0l + j]);
             for (int q = 0; q < r; q++) v = fma(-U[(size_t)q * m + ri], V[(size_t)q * n + j], v);
             row[j] = v;
             if (!cused[j]) {
@@ -249,11 +251,11 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     };
     // residual column cj (LowRank_matrix.hpp:237-243)
     auto eval_col = [&](int cj) {
-        const int64_t pj = B.ord[B.col0 + cj];
+        const int64_t pj = B.ord[B.col0l + cj];
         double bv = 0.0, ba = -1.0;
         int bi = INT_MAX;
         for (int i = tid; i < m; i += BS) {
-            double v = entry(K, B.ord[B.row0 + i], pj);
+            double v = entry(K, B.ord[B.row0l + i], pj);
             for (int q = 0; q < r; q++) v = fma(-V[(size_t)q * n + cj], U[(size_t)q * m + i], v);
             col[i] = v;
             if (!rused[i]) {
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     const FillDest D = dest[B.id];
     // U columns into the row node's leaf-blocked panel
     for (int i = tid; i < m; i += BS) {
-        const int64_t p = B.ord[B.row0 + i];
+        const int64_t p = B.ord[B.row0l + i];
         const int64_t ls = leaf_start[p];
         const int64_t nc = leaf_cnt[p];
         const int64_t base = D.up_base + (int64_t)D.R * (ls - B.row0) + (p - ls);
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     // V columns into the column node's tile-major panel: tile i holds panel
     // columns [256 i, 256 i + tw) as an n x tw row-major block
     for (int j = tid; j < n; j += BS) {
-        const int64_t row = B.ord[B.col0 + j] - B.col0;
+        const int64_t row = B.ord[B.col0l + j] - B.col0;
         for (int q = 0; q < r; q++) {
             const int col = D.voff + q;
             const int tile = col / kProjTile;
@@ -426,11 +428,12 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
 
 // dense-fallback blocks written into the row node's U panel
 __global__ __launch_bounds__(256) void dense_fill_kernel(const DenseBlock *__restrict__ blocks,
-                                                         KernelDev K,
+                                                         const KernelDev *__restrict__ Ks,
                                                          const int64_t *__restrict__ leaf_start,
                                                          const int32_t *__restrict__ leaf_cnt,
                                                          double *__restrict__ UP) {
     const DenseBlock B = blocks[blockIdx.x];
+    const KernelDev K = Ks[B.prob];
     const int64_t tot = (int64_t)B.m * B.n;
     for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
         const int64_t i = e % B.m, q = e / B.m;
@@ -444,8 +447,10 @@ __global__ __launch_bounds__(256) void dense_fill_kernel(const DenseBlock *__res
 // near-field panel of one leaf: self block then neighbours (HDD_node.hpp:136-144)
 __global__ __launch_bounds__(256) void near_fill_kernel(const NearLeaf *__restrict__ leaves,
                                                         const int64_t *__restrict__ colsrc,
-                                                        KernelDev K, double *__restrict__ NP) {
+                                                        const KernelDev *__restrict__ Ks,
+                                                        double *__restrict__ NP) {
     const NearLeaf L = leaves[blockIdx.x];
+    const KernelDev K = Ks[L.prob];
     const int64_t tot = (int64_t)L.nc * L.ncols;
     for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
         const int64_t i = e % L.nc, q = e / L.nc;
@@ -476,7 +481,7 @@ __global__ void convert_f64_f32_kernel(const double *__restrict__ a, float *__re
 
 // ---- host launchers ----
 
-hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev &K, double tol,
+hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,
                       double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
                       const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
                       hipStream_t s) {
@@ -496,14 +501,14 @@ hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const Ker
     return hipGetLastError();
 }
 
-hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev &K, const int64_t *leaf_start,
+hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev *K, const int64_t *leaf_start,
                              const int32_t *leaf_cnt, double *UP, hipStream_t s) {
     if (nb == 0) return hipSuccess;
     dense_fill_kernel<<<dim3((unsigned)nb), 256, 0, s>>>(b, K, leaf_start, leaf_cnt, UP);
     return hipGetLastError();
 }
 
-hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev &K,
+hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev *K,
                             double *NP, hipStream_t s) {
     if (nl == 0) return hipSuccess;
     near_fill_kernel<<<dim3((unsigned)nl), 256, 0, s>>>(l, colsrc, K, NP);

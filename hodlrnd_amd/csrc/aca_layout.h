@@ -11,16 +11,19 @@ constexpr int kProjChunk = 2048;  // rows of one projection work item
 
 // one admissible block for the ACA kernel
 // Row i of the block is the i-th smallest point id of the row node (the
-// order of index_of_points, HDD_clusters.hpp:112-124), found at tree
-// position ord[row0 + i] of the block's level; columns likewise.
+// order of index_of_points, HDD_clusters.hpp:112-124), found at global tree
+// position ord[row0l + i], where ord is the block's (problem, level) order
+// map indexed by problem-local position; columns likewise.
 struct AcaBlock {
-    int64_t row0, col0;   // first tree position of the row / column node
+    int64_t row0, col0;   // first global tree position of the row / column node
+    int64_t row0l, col0l; // the same, local to the block's problem
     int64_t scratch;      // offset (doubles) of this block's scratch in the pool
-    const int32_t *ord;   // ascending-id order map of the block's level
+    const int32_t *ord;   // ascending-id order map of the block's problem and level
     int32_t m, n;
     int32_t cap;          // rank capacity of the scratch
     int32_t id;           // block id
     uint64_t seed;        // retry stream (replaces rand(), LowRank_matrix.hpp:175,274)
+    int32_t prob, pad;    // problem of the block (kernel descriptor index)
 };
 
 // where a block's factors go in the operator (fill pass)
@@ -35,14 +38,16 @@ struct FillDest {
 struct DenseBlock {
     int64_t row0, col0, up_base;
     int32_t m, n, R, coff;
+    int32_t prob, pad;
 };
 
 // one leaf's near-field panel (HDD_node.hpp:136-144)
 struct NearLeaf {
     int64_t np_base;      // panel base (elements), nc x ncols column-major
-    int64_t row0;         // first tree position of the leaf
+    int64_t row0;         // first global tree position of the leaf
     int64_t colsrc;       // offset into the column-source table
     int32_t nc, ncols;
+    int32_t prob, pad;
 };
 
 __host__ __device__ inline int64_t aca_icap(int cap) { return (int64_t)cap + 8; }
@@ -97,6 +102,13 @@ struct Segment {
     int64_t vec;
     int32_t len;
     int32_t arena;
+};
+
+// rows of a batched vector that belong to one problem (never crosses a
+// problem boundary)
+struct Chunk {
+    int64_t row0;
+    int32_t rows, prob;
 };
 
 }  // namespace hodlr

@@ -5,6 +5,7 @@
 // instead of the reference's i+1 dependent dot/axpy pairs (modified
 // Gram-Schmidt, GMRES.hpp:236-240).  Dot products reduce in a fixed order:
 // per-chunk workgroup sums, then one pass over the chunks.
+#include "aca_layout.h"
 #include "device_common.h"
 
 namespace hodlr {
@@ -87,17 +88,20 @@ __global__ void to_f64_kernel(const float *__restrict__ a, double *__restrict__ 
 }
 
 // out = (double) y + diag * x: the FP32 operator's result plus its diagonal
-// applied in FP64
-__global__ void f32_result_kernel(const float *__restrict__ y, const double *__restrict__ x, double diag,
-                                  double *__restrict__ out, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (double)y[i] + diag * x[i];
+// applied in FP64 (n rows of k interleaved columns)
+__global__ void f32_result_kernel(const float *__restrict__ y, const double *__restrict__ x,
+                                  const double *__restrict__ diag, double *__restrict__ out, int64_t n, int k) {
+    const int64_t tot = n * k;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (double)y[i] + diag[i / k] * x[i];
 }
 
 // y += diag * x in FP32 (the public FP32 matvec)
-__global__ void add_diag_f32_kernel(float *__restrict__ y, const float *__restrict__ x, float diag, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        y[i] = fmaf(diag, x[i], y[i]);
+__global__ void add_diag_f32_kernel(float *__restrict__ y, const float *__restrict__ x,
+                                    const double *__restrict__ diag, int64_t n, int k) {
+    const int64_t tot = n * k;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = fmaf((float)diag[i / k], x[i], y[i]);
 }
 
 // ---- launchers ----
@@ -161,12 +165,13 @@ hipError_t launch_to_f64(const float *a, double *b, int64_t n, hipStream_t s) {
 }  // namespace hodlr
 
 namespace hodlr {
-hipError_t launch_f32_result(const float *y, const double *x, double diag, double *out, int64_t n, hipStream_t s) {
-    f32_result_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(y, x, diag, out, n);
+hipError_t launch_f32_result(const float *y, const double *x, const double *diag, double *out, int64_t n, int k,
+                             hipStream_t s) {
+    f32_result_kernel<<<dim3(grid_for(n * k)), 256, 0, s>>>(y, x, diag, out, n, k);
     return hipGetLastError();
 }
-hipError_t launch_add_diag_f32(float *y, const float *x, float diag, int64_t n, hipStream_t s) {
-    add_diag_f32_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(y, x, diag, n);
+hipError_t launch_add_diag_f32(float *y, const float *x, const double *diag, int64_t n, int k, hipStream_t s) {
+    add_diag_f32_kernel<<<dim3(grid_for(n * k)), 256, 0, s>>>(y, x, diag, n, k);
     return hipGetLastError();
 }
 }  // namespace hodlr
@@ -192,4 +197,143 @@ hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t
     stream_copy_kernel<<<dim3(4096), 256, 0, s>>>(static_cast<const float4 *>(a), static_cast<float4 *>(b), bytes / 16);
     return hipGetLastError();
 }
+}  // namespace hodlr
+
+// ---- segmented (per problem, per column) kernels for batched solves ----
+// Vectors hold n rows of K interleaved columns; rows are split into chunks
+// that never cross a problem boundary.  Every reduction sums in a fixed
+// order that depends only on the problem's own rows, so an instance gives
+// the same bits alone or inside any batch.
+namespace hodlr {
+
+// partial[(chunk * nv + v) * K + c] = sum over the chunk of V_v[row, c] * w[row, c]
+template <int K>
+__global__ __launch_bounds__(256) void bmdot_kernel(const double *const *__restrict__ V, int nv,
+                                                    const double *__restrict__ w, const Chunk *__restrict__ chunks,
+                                                    double *__restrict__ partial) {
+    __shared__ double red[4][K];
+    const Chunk C = chunks[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int v = 0; v < nv; v++) {
+        const double *vv = V[v];
+        double acc[K];
+#pragma unroll
+        for (int c = 0; c < K; c++) acc[c] = 0.0;
+        for (int r = tid; r < C.rows; r += 256) {
+            const int64_t base = (C.row0 + r) * K;
+#pragma unroll
+            for (int c = 0; c < K; c++) acc[c] = fma(vv[base + c], w[base + c], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            const double sum = wave_sum(acc[c]);
+            if (lane == 0) red[wid][c] = sum;
+        }
+        __syncthreads();
+        if (tid < K) partial[((int64_t)blockIdx.x * nv + v) * K + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+        __syncthreads();
+    }
+}
+
+// out[(p * nv + v) * K + c] = sum of partials over problem p's chunks
+__global__ __launch_bounds__(256) void bfinish_kernel(const double *__restrict__ partial,
+                                                      const int32_t *__restrict__ pchunk, int nv, int K,
+                                                      double *__restrict__ out) {
+    const int p = blockIdx.x;
+    const int c0 = pchunk[p], c1 = pchunk[p + 1];
+    for (int e = threadIdx.x; e < nv * K; e += blockDim.x) {
+        double s = 0.0;
+        for (int ch = c0; ch < c1; ch++) s += partial[(int64_t)ch * nv * K + e];
+        out[(int64_t)p * nv * K + e] = s;
+    }
+}
+
+// w[row, c] += sign * sum_v coef[(prob * nv + v) * K + c] * V_v[row, c]
+template <int K>
+__global__ __launch_bounds__(256) void bgemv_kernel(const double *const *__restrict__ V, int nv,
+                                                    const double *__restrict__ coef, double sign,
+                                                    double *__restrict__ w, const Chunk *__restrict__ chunks) {
+    __shared__ double cs[128 * K];
+    const Chunk C = chunks[blockIdx.x];
+    for (int e = threadIdx.x; e < nv * K; e += blockDim.x) cs[e] = sign * coef[(int64_t)C.prob * nv * K + e];
+    __syncthreads();
+    for (int r = threadIdx.x; r < C.rows; r += 256) {
+        const int64_t base = (C.row0 + r) * K;
+        double acc[K];
+#pragma unroll
+        for (int c = 0; c < K; c++) acc[c] = w[base + c];
+        for (int v = 0; v < nv; v++) {
+            const double *vv = V[v];
+#pragma unroll
+            for (int c = 0; c < K; c++) acc[c] = fma(cs[v * K + c], vv[base + c], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < K; c++) w[base + c] = acc[c];
+    }
+}
+
+// dst[row, c] = src[row, c] * coef[prob * K + c]
+template <int K>
+__global__ __launch_bounds__(256) void bscale_kernel(const double *__restrict__ src, const double *__restrict__ coef,
+                                                     double *__restrict__ dst, const Chunk *__restrict__ chunks) {
+    const Chunk C = chunks[blockIdx.x];
+    double cf[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) cf[c] = coef[(int64_t)C.prob * K + c];
+    for (int r = threadIdx.x; r < C.rows; r += 256) {
+        const int64_t base = (C.row0 + r) * K;
+#pragma unroll
+        for (int c = 0; c < K; c++) dst[base + c] = src[base + c] * cf[c];
+    }
+}
+
+#define HODLR_BATCH_K(MACRO) \
+    MACRO(1)                 \
+    MACRO(2)                 \
+    MACRO(4)                 \
+    MACRO(8)                 \
+    MACRO(16)
+
+hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const Chunk *chunks, int nchunk,
+                        double *partial, const int32_t *pchunk, int nprob, int K, double *out, hipStream_t s) {
+    if (nv > 128) return hipErrorInvalidValue;
+    switch (K) {
+#define CASE(KK) \
+    case KK: bmdot_kernel<KK><<<dim3((unsigned)nchunk), 256, 0, s>>>(V, nv, w, chunks, partial); break;
+        HODLR_BATCH_K(CASE)
+#undef CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    bfinish_kernel<<<dim3((unsigned)nprob), 256, 0, s>>>(partial, pchunk, nv, K, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_bgemv(const double *const *V, int nv, const double *coef, double sign, double *w,
+                        const Chunk *chunks, int nchunk, int K, hipStream_t s) {
+    if (nv > 128 || nv * K > 128 * K) return hipErrorInvalidValue;
+    switch (K) {
+#define CASE(KK) \
+    case KK: bgemv_kernel<KK><<<dim3((unsigned)nchunk), 256, 0, s>>>(V, nv, coef, sign, w, chunks); break;
+        HODLR_BATCH_K(CASE)
+#undef CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bscale(const double *src, const double *coef, double *dst, const Chunk *chunks, int nchunk, int K,
+                         hipStream_t s) {
+    switch (K) {
+#define CASE(KK) \
+    case KK: bscale_kernel<KK><<<dim3((unsigned)nchunk), 256, 0, s>>>(src, coef, dst, chunks); break;
+        HODLR_BATCH_K(CASE)
+#undef CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace hodlr

@@ -37,6 +37,29 @@ int padded_rhs(int nrhs) {
     return -1;
 }
 
+// host vectors: problems concatenated, each in its point order, column-major
+// N x k; device: tree order, K interleaved columns (zero-padded past k)
+void to_tree(const hodlr_op *op, int k, int K, const double *src, std::vector<double> &dst) {
+    const int64_t N = op->N;
+    dst.assign((size_t)N * K, 0.0);
+    for (size_t p = 0; p < op->trees.size(); p++) {
+        const int64_t pb = op->pos_base[p];
+        const auto &perm = op->trees[p]->perm;
+        for (int64_t q = 0; q < (int64_t)perm.size(); q++)
+            for (int c = 0; c < k; c++) dst[(pb + q) * K + c] = src[(int64_t)c * N + pb + perm[q]];
+    }
+}
+
+void from_tree(const hodlr_op *op, int k, int K, const double *src, double *dst) {
+    const int64_t N = op->N;
+    for (size_t p = 0; p < op->trees.size(); p++) {
+        const int64_t pb = op->pos_base[p];
+        const auto &perm = op->trees[p]->perm;
+        for (int64_t q = 0; q < (int64_t)perm.size(); q++)
+            for (int c = 0; c < k; c++) dst[(int64_t)c * N + pb + perm[q]] = src[(pb + q) * K + c];
+    }
+}
+
 void set_device(hodlr_ctx *ctx) {
     if (ctx->device < 0) throw ArgError{HODLR_E_STATE, "host-only context (device -1) cannot run device work"};
     HIP_CHECK(hipSetDevice(ctx->device));
@@ -168,8 +191,8 @@ int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
         set_device(ctx);
         auto *op = new hodlr_op;
         op->ctx = ctx;
-        op->tree = t;
-        op->kd = *k;
+        op->trees = {t};
+        op->kds = {*k};
         op->eps = eps_aca;
         op->prec_flags = prec_flags | HODLR_PREC_F64;
         try {
@@ -178,9 +201,52 @@ int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
             delete op;
             throw;
         }
-        op->kd.weight = nullptr;  // the caller's array is not referenced after assembly
+        for (auto &kd : op->kds) kd.weight = nullptr;  // caller arrays are not referenced after assembly
+        op->kd.weight = nullptr;
         *out = op;
     });
+}
+
+int hodlr_op_assemble_batch(hodlr_ctx *ctx, int nprob, hodlr_tree *const *trees, const hodlr_kernel_desc *k,
+                            double eps_aca, int prec_flags, hodlr_op **out) {
+    if (!ctx || nprob < 1 || !trees || !k || !out) return HODLR_E_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        int64_t total = 0;
+        for (int p = 0; p < nprob; p++) {
+            if (!trees[p]) throw ArgError{HODLR_E_ARG, "null tree in batch"};
+            if (k[p].ndim != trees[p]->ndim) throw ArgError{HODLR_E_ARG, "kernel ndim differs from the tree's"};
+            if (k[p].family < 0 || k[p].family > 2) throw ArgError{HODLR_E_ARG, "unknown kernel family"};
+            total += trees[p]->n;
+        }
+        if (total > INT32_MAX) throw ArgError{HODLR_E_ARG, "a batch holds at most 2^31-1 points"};
+        if (!(eps_aca > 0.0 && eps_aca < 1.0)) throw ArgError{HODLR_E_ARG, "eps_aca must be in (0, 1)"};
+        set_device(ctx);
+        auto *op = new hodlr_op;
+        op->ctx = ctx;
+        op->trees.assign(trees, trees + nprob);
+        op->kds.assign(k, k + nprob);
+        op->eps = eps_aca;
+        op->prec_flags = prec_flags | HODLR_PREC_F64;
+        try {
+            assemble(*op);
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        for (auto &kd : op->kds) kd.weight = nullptr;
+        op->kd.weight = nullptr;
+        *out = op;
+    });
+}
+
+int hodlr_op_problem_offsets(hodlr_op *op, int64_t *offsets, int cap) {
+    if (!op) return HODLR_E_ARG;
+    const int P = (int)op->trees.size();
+    if (!offsets) return P;
+    if (cap < P + 1) return HODLR_E_ARG;
+    for (int p = 0; p <= P; p++) offsets[p] = op->pos_base[p];
+    return P;
 }
 
 int hodlr_op_get_stats(hodlr_op *op, hodlr_op_stats *s) {
@@ -206,7 +272,7 @@ void hodlr_op_destroy(hodlr_op *op) {
     delete op;
 }
 
-int hodlr_to_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst) {
+int hodlr_to_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst) {  // one problem
     if (!t || !src || !dst || nrhs < 1) return HODLR_E_ARG;
     for (int64_t p = 0; p < t->n; p++)
         for (int c = 0; c < nrhs; c++) dst[p * nrhs + c] = src[(int64_t)c * t->n + t->perm[p]];
@@ -229,10 +295,8 @@ int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y) {
         set_device(op->ctx);
         const int64_t n = op->N;
         hipStream_t s = op->ctx->stream;
-        std::vector<double> xt((size_t)n * K, 0.0), yt((size_t)n * K);
-        const auto &perm = op->tree->perm;
-        for (int64_t p = 0; p < n; p++)
-            for (int c = 0; c < nrhs; c++) xt[p * K + c] = x[(int64_t)c * n + perm[p]];
+        std::vector<double> xt, yt((size_t)n * K);
+        to_tree(op, nrhs, K, x, xt);
         DevBuf dx, dy;
         if (prec == HODLR_PREC_F64) {
             upload(dx, xt);
@@ -249,8 +313,7 @@ int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y) {
             HIP_CHECK(hipMemcpy(yf.data(), dy.p, sizeof(float) * n * K, hipMemcpyDeviceToHost));
             for (size_t i = 0; i < yf.size(); i++) yt[i] = yf[i];
         }
-        for (int64_t p = 0; p < n; p++)
-            for (int c = 0; c < nrhs; c++) y[(int64_t)c * n + perm[p]] = yt[p * K + c];
+        from_tree(op, nrhs, K, yt.data(), y);
     });
 }
 
@@ -282,21 +345,61 @@ int hodlr_gmres(hodlr_op *op, const double *b, double *x, const hodlr_gmres_para
     int rc = guard(op->ctx, [&] {
         set_device(op->ctx);
         const int64_t n = op->N;
-        std::vector<double> bt(n), xt(n);
-        const auto &perm = op->tree->perm;
-        for (int64_t p = 0; p < n; p++) {
-            bt[p] = b[perm[p]];
-            xt[p] = x[perm[p]];
-        }
+        std::vector<double> bt, xt;
+        to_tree(op, 1, 1, b, bt);
+        to_tree(op, 1, 1, x, xt);
         DevBuf db, dx;
         upload(db, bt);
         upload(dx, xt);
         solve(*op, db.as<double>(), dx.as<double>(), *params, *info);
         HIP_CHECK(hipMemcpy(xt.data(), dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-        for (int64_t p = 0; p < n; p++) x[perm[p]] = xt[p];
+        from_tree(op, 1, 1, xt.data(), x);
     });
     if (rc == HODLR_OK && !info->converged) return HODLR_E_NOCONV;
     return rc;
+}
+
+int hodlr_gmres_batch_device(hodlr_op *op, int nrhs, const double *b, double *x, const hodlr_gmres_params *params,
+                             hodlr_gmres_info *infos) {
+    if (!op || !b || !x || !params || !infos) return HODLR_E_ARG;
+    int rc = guard(op->ctx, [&] {
+        if (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8 && nrhs != 16)
+            throw ArgError{HODLR_E_ARG, "nrhs must be 1, 2, 4, 8 or 16"};
+        set_device(op->ctx);
+        solve_batch(*op, nrhs, b, x, *params, infos);
+    });
+    if (rc != HODLR_OK) return rc;
+    for (size_t a = 0; a < op->trees.size() * (size_t)nrhs; a++)
+        if (!infos[a].converged) return HODLR_E_NOCONV;
+    return HODLR_OK;
+}
+
+int hodlr_gmres_batch(hodlr_op *op, int nrhs, const double *b, double *x, const hodlr_gmres_params *params,
+                      hodlr_gmres_info *infos) {
+    if (!op || !b || !x || !params || !infos || nrhs < 1) return HODLR_E_ARG;
+    int rc = guard(op->ctx, [&] {
+        const int K = padded_rhs(nrhs);
+        if (K < 0 || K > 16) throw ArgError{HODLR_E_ARG, "at most 16 right-hand sides per problem"};
+        set_device(op->ctx);
+        const int64_t n = op->N;
+        std::vector<double> bt, xt;
+        to_tree(op, nrhs, K, b, bt);
+        to_tree(op, nrhs, K, x, xt);
+        DevBuf db, dx;
+        upload(db, bt);
+        upload(dx, xt);
+        const int P = (int)op->trees.size();
+        std::vector<hodlr_gmres_info> all((size_t)P * K);
+        solve_batch(*op, K, db.as<double>(), dx.as<double>(), *params, all.data());
+        HIP_CHECK(hipMemcpy(xt.data(), dx.p, sizeof(double) * n * K, hipMemcpyDeviceToHost));
+        from_tree(op, nrhs, K, xt.data(), x);
+        for (int p = 0; p < P; p++)
+            for (int c = 0; c < nrhs; c++) infos[(size_t)p * nrhs + c] = all[(size_t)p * K + c];
+    });
+    if (rc != HODLR_OK) return rc;
+    for (size_t a = 0; a < op->trees.size() * (size_t)nrhs; a++)
+        if (!infos[a].converged) return HODLR_E_NOCONV;
+    return HODLR_OK;
 }
 
 int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes) {

@@ -13,6 +13,9 @@ namespace hodlr {
 
 constexpr int kWave = 64;
 
+// One problem's kernel.  Positions are global over a batch of problems;
+// this problem's points occupy [pos0, pos0 + n) and X, W hold them in tree
+// order starting at pos0.
 struct KernelDev {
     int family;   // HODLR_KERNEL_*
     int ndim;
@@ -20,14 +23,15 @@ struct KernelDev {
     double diag;
     const double *X;   // tree order, n x ndim
     const double *W;   // tree order weights or nullptr
+    int64_t pos0;
 };
 
-// A_ij for tree positions i, j (user_kernel.hpp:50-58,105-113,171-177 and
-// points_dt.hpp:31-39): squared distance by separate multiply and add.
+// A_ij for global tree positions i, j (user_kernel.hpp:50-58,105-113,171-177
+// and points_dt.hpp:31-39): squared distance by separate multiply and add.
 __device__ __forceinline__ double entry(const KernelDev &k, int64_t i, int64_t j) {
     if (i == j) return k.diag;
-    const double *a = k.X + i * k.ndim;
-    const double *b = k.X + j * k.ndim;
+    const double *a = k.X + (i - k.pos0) * k.ndim;
+    const double *b = k.X + (j - k.pos0) * k.ndim;
     double s = 0.0;
     for (int d = 0; d < k.ndim; d++) {
         double t = a[d] - b[d];
@@ -40,29 +44,7 @@ __device__ __forceinline__ double entry(const KernelDev &k, int64_t i, int64_t j
     else if (k.family == 0) f = log(r);
     else f = exp(-r);
     double v = k.scale * f;
-    if (k.W) v = v * k.W[j];
-    return v;
-}
-
-// same with the row point's coordinates already in registers
-template <int ND>
-__device__ __forceinline__ double entry_fixed(const KernelDev &k, const double *xi, int64_t i, int64_t j) {
-    if (i == j) return k.diag;
-    const double *b = k.X + j * ND;
-    double s = 0.0;
-#pragma unroll
-    for (int d = 0; d < ND; d++) {
-        double t = xi[d] - b[d];
-        double t2 = t * t;
-        s = s + t2;
-    }
-    double r = sqrt(s);
-    double f;
-    if (k.family == 1) f = 1.0 / r;
-    else if (k.family == 0) f = log(r);
-    else f = exp(-r);
-    double v = k.scale * f;
-    if (k.W) v = v * k.W[j];
+    if (k.W) v = v * k.W[j - k.pos0];
     return v;
 }
 

@@ -1,6 +1,9 @@
 // op.cpp -- operator assembly and the matvec driver (host side).
 //
-// Assembly (HDD_tree.hpp:76-83, HDD_node.hpp:118-147) runs in four steps:
+// Assembly (HDD_tree.hpp:76-83, HDD_node.hpp:118-147) runs in four steps,
+// over one problem or over a batch of independent problems at once (their
+// block-diagonal operator; positions and nodes numbered globally, problem
+// after problem):
 //   1. enumerate the admissible blocks in the reference's order (level,
 //      node, interaction-list index), skipping empty nodes;
 //   2. ACA rank pass over every block (batched, chunked scratch);
@@ -54,17 +57,11 @@ uint64_t block_seed(int level, int64_t row_node, int64_t col_node) {
     return s;
 }
 
-struct AcaRun {
-    std::vector<AcaBlock> blocks;   // one size class
-    int bs = 64;
-};
-
 // Runs one ACA pass over `ids` (block ids) with per-block caps, chunked so
 // that the scratch pool stays within `budget` doubles.
 void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vector<int32_t> &cap, int fill,
-                  const KernelDev &K, DevBuf &pool, int64_t budget, DevBuf &d_rank, DevBuf &d_flags,
-                  const DevBuf *d_dest, hipStream_t s) {
-    const hodlr_tree &t = *op.tree;
+                  DevBuf &pool, int64_t budget, DevBuf &d_rank, DevBuf &d_flags, const DevBuf *d_dest,
+                  hipStream_t s) {
     // group by workgroup size, large blocks first inside a class
     std::vector<int64_t> order(ids);
     std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
@@ -84,25 +81,29 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
             if (aca_block_size(op.b_m[b], op.b_n[b]) != bs) break;
             const int64_t need = aca_scratch_doubles(op.b_m[b], op.b_n[b], cap[b]);
             if (!chunk.empty() && used + need > budget) break;
+            const int p = op.b_prob[b];
             AcaBlock a;
-            a.row0 = t.start[op.b_row[b]];
-            a.col0 = t.start[op.b_col[b]];
+            a.row0 = op.node_start[op.b_row[b]];
+            a.col0 = op.node_start[op.b_col[b]];
+            a.row0l = a.row0 - op.pos_base[p];
+            a.col0l = a.col0 - op.pos_base[p];
             a.scratch = used;
-            a.ord = op.ord.as<int32_t>() + (int64_t)op.b_level[b] * t.n;
+            a.ord = op.ord.as<int32_t>() + op.ord_off[p] + (int64_t)op.b_level[b] * op.trees[p]->n;
             a.m = op.b_m[b];
             a.n = op.b_n[b];
             a.cap = cap[b];
             a.id = (int32_t)b;
-            a.seed = block_seed(op.b_level[b], op.b_row[b] - t.level_off[op.b_level[b]],
-                                op.b_col[b] - t.level_off[op.b_level[b]]);
+            a.seed = block_seed(op.b_level[b], op.b_rowj[b], op.b_colj[b]);
+            a.prob = p;
+            a.pad = 0;
             chunk.push_back(a);
             used += need;
             i++;
         }
         if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double));
         upload(d_blocks, chunk);
-        HIP_CHECK(launch_aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), K, op.eps, pool.as<double>(),
-                             d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
+        HIP_CHECK(launch_aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), op.kdev.as<KernelDev>(), op.eps,
+                             pool.as<double>(), d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
                              d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
                              op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
@@ -116,72 +117,113 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
 
 void assemble(hodlr_op &op) {
     const double t_start = now_s();
-    hodlr_tree &t = *op.tree;
     hipStream_t s = op.ctx->stream;
-    const int d = t.ndim;
-    const int L = t.nlevels - 1;  // leaf level
-    const int64_t N = t.n;
-    op.N = N;
+    const int P = (int)op.trees.size();
+    op.tree = op.trees[0];
+    op.kd = op.kds[0];
 
-    // ---- tree-order inputs ----
-    {
-        std::vector<double> X((size_t)N * d), W;
-        for (int64_t p = 0; p < N; p++)
-            for (int k = 0; k < d; k++) X[p * d + k] = t.pts[t.perm[p] * d + k];
-        upload(op.X, X);
-        if (op.kd.weight) {
-            W.resize(N);
-            for (int64_t p = 0; p < N; p++) W[p] = op.kd.weight[t.perm[p]];
-            upload(op.W, W);
+    // ---- global numbering of positions and nodes ----
+    std::vector<int64_t> node_base(P + 1, 0);
+    op.pos_base.assign(P + 1, 0);
+    for (int p = 0; p < P; p++) {
+        op.pos_base[p + 1] = op.pos_base[p] + op.trees[p]->n;
+        node_base[p + 1] = node_base[p] + op.trees[p]->level_off.back();
+    }
+    const int64_t N = op.pos_base[P];
+    const int64_t nn = node_base[P];
+    op.N = N;
+    op.node_start.assign(nn, 0);
+    std::vector<int64_t> count(nn, 0);
+    for (int p = 0; p < P; p++) {
+        const hodlr_tree &t = *op.trees[p];
+        for (int64_t g = 0; g < t.level_off.back(); g++) {
+            op.node_start[node_base[p] + g] = op.pos_base[p] + t.start[g];
+            count[node_base[p] + g] = t.count[g];
         }
+    }
+
+    // ---- tree-order inputs: coordinates, weights, leaf of every position,
+    //      per (problem, level) ascending-id order maps ----
+    {
+        std::vector<double> X, W;
+        std::vector<int64_t> xoff(P), woff(P, -1);
+        for (int p = 0; p < P; p++) {
+            const hodlr_tree &t = *op.trees[p];
+            xoff[p] = (int64_t)X.size();
+            for (int64_t q = 0; q < t.n; q++)
+                for (int k = 0; k < t.ndim; k++) X.push_back(t.pts[t.perm[q] * t.ndim + k]);
+            if (op.kds[p].weight) {
+                woff[p] = (int64_t)W.size();
+                for (int64_t q = 0; q < t.n; q++) W.push_back(op.kds[p].weight[t.perm[q]]);
+            }
+        }
+        upload(op.X, X);
+        upload(op.W, W);
+        std::vector<KernelDev> kd(P);
+        for (int p = 0; p < P; p++) {
+            kd[p].family = op.kds[p].family;
+            kd[p].ndim = op.trees[p]->ndim;
+            kd[p].scale = op.kds[p].scale;
+            kd[p].diag = op.kds[p].diag;
+            kd[p].X = op.X.as<double>() + xoff[p];
+            kd[p].W = woff[p] >= 0 ? op.W.as<double>() + woff[p] : nullptr;
+            kd[p].pos0 = op.pos_base[p];
+        }
+        upload(op.kdev, kd);
         std::vector<int64_t> ls(N > 0 ? N : 1, 0);
         std::vector<int32_t> lc(N > 0 ? N : 1, 0);
-        for (int64_t j = 0; j < t.level_size(L); j++) {
-            const int64_t g = t.node(L, j);
-            for (int64_t p = t.start[g]; p < t.start[g] + t.count[g]; p++) {
-                ls[p] = t.start[g];
-                lc[p] = (int32_t)t.count[g];
+        std::vector<int32_t> ord;
+        op.ord_off.assign(P, 0);
+        for (int p = 0; p < P; p++) {
+            const hodlr_tree &t = *op.trees[p];
+            const int L = t.nlevels - 1;
+            const int64_t pb = op.pos_base[p];
+            for (int64_t j = 0; j < t.level_size(L); j++) {
+                const int64_t g = t.node(L, j);
+                for (int64_t q = t.start[g]; q < t.start[g] + t.count[g]; q++) {
+                    ls[pb + q] = pb + t.start[g];
+                    lc[pb + q] = (int32_t)t.count[g];
+                }
+            }
+            op.ord_off[p] = (int64_t)ord.size();
+            ord.resize(ord.size() + (size_t)t.nlevels * t.n);
+            for (int l = 0; l <= L; l++) {
+                int32_t *o = ord.data() + op.ord_off[p] + (size_t)l * t.n;
+                for (int64_t q = 0; q < t.n; q++) o[q] = (int32_t)(pb + q);
+                if (l == L) continue;  // leaves are already ascending
+                for (int64_t j = 0; j < t.level_size(l); j++) {
+                    const int64_t g = t.node(l, j);
+                    std::sort(o + t.start[g], o + t.start[g] + t.count[g],
+                              [&](int32_t a, int32_t b) { return t.perm[a - pb] < t.perm[b - pb]; });
+                }
             }
         }
         upload(op.leaf_start, ls);
         upload(op.leaf_cnt, lc);
-        // per level: each node's tree positions sorted by point id
-        std::vector<int32_t> ord((size_t)t.nlevels * (N > 0 ? N : 1));
-        for (int l = 0; l <= L; l++) {
-            int32_t *o = ord.data() + (size_t)l * N;
-            for (int64_t p = 0; p < N; p++) o[p] = (int32_t)p;
-            if (l == L) continue;  // leaves are already ascending
-            for (int64_t j = 0; j < t.level_size(l); j++) {
-                const int64_t g = t.node(l, j);
-                std::sort(o + t.start[g], o + t.start[g] + t.count[g],
-                          [&](int32_t a, int32_t b) { return t.perm[a] < t.perm[b]; });
-            }
-        }
         upload(op.ord, ord);
     }
-    KernelDev K;
-    K.family = op.kd.family;
-    K.ndim = d;
-    K.scale = op.kd.scale;
-    K.diag = op.kd.diag;
-    K.X = op.X.as<double>();
-    K.W = op.kd.weight ? op.W.as<double>() : nullptr;
 
-    // ---- 1. blocks in reference order ----
-    for (int l = 0; l <= L; l++)
-        for (int64_t j = 0; j < t.level_size(l); j++) {
-            const int64_t g = t.node(l, j);
-            if (!t.count[g]) continue;
-            for (int64_t a = t.il_ptr[g]; a < t.il_ptr[g + 1]; a++) {
-                const int64_t gb = t.node(l, t.il[a]);
-                if (!t.count[gb]) continue;
-                op.b_level.push_back(l);
-                op.b_row.push_back(g);
-                op.b_col.push_back(gb);
-                op.b_m.push_back((int32_t)t.count[g]);
-                op.b_n.push_back((int32_t)t.count[gb]);
+    // ---- 1. blocks in reference order, problem by problem ----
+    for (int p = 0; p < P; p++) {
+        const hodlr_tree &t = *op.trees[p];
+        for (int l = 0; l < t.nlevels; l++)
+            for (int64_t j = 0; j < t.level_size(l); j++) {
+                const int64_t g = t.node(l, j);
+                if (!t.count[g]) continue;
+                for (int64_t a = t.il_ptr[g]; a < t.il_ptr[g + 1]; a++) {
+                    const int64_t gb = t.node(l, t.il[a]);
+                    if (!t.count[gb]) continue;
+                    op.b_level.push_back(l);
+                    op.b_prob.push_back(p);
+                    op.b_row.push_back(node_base[p] + g);
+                    op.b_col.push_back(node_base[p] + gb);
+                    op.b_rowj.push_back(j);
+                    op.b_colj.push_back(t.il[a]);
+                    op.b_m.push_back((int32_t)t.count[g]);
+                    op.b_n.push_back((int32_t)t.count[gb]);
+                }
             }
-        }
+    }
     op.nblk = (int64_t)op.b_level.size();
     const int64_t nb = op.nblk;
 
@@ -202,8 +244,8 @@ void assemble(hodlr_op &op) {
     }
     op.rank.assign(nb, 0);
     op.flags.assign(nb, 0);
-    for (int round = 0; !todo.empty(); round++) {
-        run_aca_pass(op, todo, cap, 0, K, pool, budget, d_rank, d_flags, nullptr, s);
+    while (!todo.empty()) {
+        run_aca_pass(op, todo, cap, 0, pool, budget, d_rank, d_flags, nullptr, s);
         std::vector<int32_t> rk(nb), fl(nb);
         if (nb) {
             HIP_CHECK(hipMemcpy(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
@@ -223,13 +265,11 @@ void assemble(hodlr_op &op) {
         todo.swap(again);
     }
     const double t_rank1 = now_s();
-
     if (verbose()) std::fprintf(stderr, "[hodlr] rank pass %.3f s, %lld blocks\n", t_rank1 - t_rank0, (long long)nb);
+
     // ---- 3. layout ----
-    const int64_t nn = t.level_off.back();
     std::vector<int64_t> R(nn, 0), Tb(nn, 0), up_base(nn, 0), t_base(nn, 0), vp_base(nn, 0), vcol_base(nn, 0);
     std::vector<int32_t> width(nb), coff(nb), voff(nb);
-    std::vector<int64_t> first_blk(nn + 1, -1);
     for (int64_t b = 0; b < nb; b++) {
         const bool dense = op.flags[b] & 1;
         width[b] = dense ? op.b_n[b] : op.rank[b];
@@ -241,90 +281,93 @@ void assemble(hodlr_op &op) {
             Tb[op.b_col[b]] += op.rank[b];
         }
     }
-    op.n_up = op.n_vp = op.n_t = 0;
-    int64_t nvcol = 0;
-    op.e_u = op.e_v = 0;
     // V panels are padded to a multiple of 4 columns so that every row of a
     // projection tile starts on a 4-element boundary
     std::vector<int64_t> Tpad(nn, 0);
+    op.n_up = op.n_vp = op.n_t = 0;
+    int64_t nvcol = 0;
+    op.e_v = 0;
     for (int64_t g = 0; g < nn; g++) {
         Tpad[g] = (Tb[g] + 3) / 4 * 4;
         up_base[g] = op.n_up;
-        op.n_up += t.count[g] * R[g];
+        op.n_up += count[g] * R[g];
         t_base[g] = op.n_t;
         op.n_t += R[g];
         vp_base[g] = op.n_vp;
-        op.n_vp += t.count[g] * Tpad[g];
+        op.n_vp += count[g] * Tpad[g];
         vcol_base[g] = nvcol;
         nvcol += Tpad[g];
+        op.e_v += (double)count[g] * Tb[g];
     }
     op.e_u = (double)op.n_up;
-    op.e_v = 0;
-    for (int64_t g = 0; g < nn; g++) op.e_v += (double)t.count[g] * Tb[g];
 
-    // near field per non-empty leaf
+    // near field and panel work per non-empty leaf
     std::vector<NearLeaf> near;
     std::vector<int64_t> colsrc;
     std::vector<Segment> segs;
     std::vector<PanelItem> panel;
     op.n_np = 0;
-    op.e_near = 0;
-    for (int64_t j = 0; j < t.level_size(L); j++) {
-        const int64_t g = t.node(L, j);
-        const int64_t nc = t.count[g];
-        if (!nc) continue;
-        NearLeaf nl;
-        nl.np_base = op.n_np;
-        nl.row0 = t.start[g];
-        nl.colsrc = (int64_t)colsrc.size();
-        nl.nc = (int32_t)nc;
-        const int64_t seg0 = (int64_t)segs.size();
-        int64_t ncols = 0;
-        // one segment per non-empty neighbour block (self first), each
-        // multiplying a contiguous slice of x
-        auto add_near = [&](int64_t gb) {
-            const int64_t c = t.count[gb];
-            if (!c) return;
-            Segment sg;
-            sg.mat = op.n_np + ncols * nc;
-            sg.vec = t.start[gb];
-            sg.len = (int32_t)c;
-            sg.arena = 0;
-            segs.push_back(sg);
-            for (int64_t p = t.start[gb]; p < t.start[gb] + c; p++) colsrc.push_back(p);
-            ncols += c;
-        };
-        add_near(g);
-        for (int64_t a = t.nbr_ptr[g]; a < t.nbr_ptr[g + 1]; a++) add_near(t.node(L, t.nbr[a]));
-        nl.ncols = (int32_t)ncols;
-        near.push_back(nl);
-        op.n_np += nc * ncols;
-        // far field: one U-panel slice per ancestor (and the leaf itself)
-        for (int l = 1; l <= L; l++) {
-            const int64_t ja = j >> (d * (L - l));
-            const int64_t ga = t.node(l, ja);
-            if (!R[ga]) continue;
-            Segment sg;
-            sg.mat = up_base[ga] + R[ga] * (t.start[g] - t.start[ga]);
-            sg.vec = t_base[ga];
-            sg.len = (int32_t)R[ga];
-            sg.arena = 1;
-            segs.push_back(sg);
-        }
-        const int64_t nseg = (int64_t)segs.size() - seg0;
-        for (int64_t r0 = 0; r0 < nc; r0 += 64) {
-            PanelItem pi;
-            pi.y0 = t.start[g] + r0;
-            pi.seg0 = seg0;
-            pi.nseg = (int32_t)nseg;
-            pi.h = (int32_t)std::min<int64_t>(64, nc - r0);
-            pi.ld = (int32_t)nc;
-            pi.r0 = (int32_t)r0;
-            panel.push_back(pi);
+    for (int p = 0; p < P; p++) {
+        const hodlr_tree &t = *op.trees[p];
+        const int L = t.nlevels - 1, d = t.ndim;
+        for (int64_t j = 0; j < t.level_size(L); j++) {
+            const int64_t g = node_base[p] + t.node(L, j);
+            const int64_t nc = count[g];
+            if (!nc) continue;
+            NearLeaf nl;
+            nl.np_base = op.n_np;
+            nl.row0 = op.node_start[g];
+            nl.colsrc = (int64_t)colsrc.size();
+            nl.nc = (int32_t)nc;
+            nl.prob = p;
+            nl.pad = 0;
+            const int64_t seg0 = (int64_t)segs.size();
+            int64_t ncols = 0;
+            // one segment per non-empty neighbour block (self first), each
+            // multiplying a contiguous slice of x
+            auto add_near = [&](int64_t gb) {
+                const int64_t c = count[gb];
+                if (!c) return;
+                Segment sg;
+                sg.mat = op.n_np + ncols * nc;
+                sg.vec = op.node_start[gb];
+                sg.len = (int32_t)c;
+                sg.arena = 0;
+                segs.push_back(sg);
+                for (int64_t q = op.node_start[gb]; q < op.node_start[gb] + c; q++) colsrc.push_back(q);
+                ncols += c;
+            };
+            add_near(g);
+            const int64_t gl = t.node(L, j);
+            for (int64_t a = t.nbr_ptr[gl]; a < t.nbr_ptr[gl + 1]; a++) add_near(node_base[p] + t.node(L, t.nbr[a]));
+            nl.ncols = (int32_t)ncols;
+            near.push_back(nl);
+            op.n_np += nc * ncols;
+            // far field: one U-panel slice per ancestor (and the leaf itself)
+            for (int l = 1; l <= L; l++) {
+                const int64_t ga = node_base[p] + t.node(l, j >> (d * (L - l)));
+                if (!R[ga]) continue;
+                Segment sg;
+                sg.mat = up_base[ga] + R[ga] * (op.node_start[g] - op.node_start[ga]);
+                sg.vec = t_base[ga];
+                sg.len = (int32_t)R[ga];
+                sg.arena = 1;
+                segs.push_back(sg);
+            }
+            const int64_t nseg = (int64_t)segs.size() - seg0;
+            for (int64_t r0 = 0; r0 < nc; r0 += 64) {
+                PanelItem pi;
+                pi.y0 = op.node_start[g] + r0;
+                pi.seg0 = seg0;
+                pi.nseg = (int32_t)nseg;
+                pi.h = (int32_t)std::min<int64_t>(64, nc - r0);
+                pi.ld = (int32_t)nc;
+                pi.r0 = (int32_t)r0;
+                panel.push_back(pi);
+            }
         }
     }
     op.e_near = (double)op.n_np;
-    // root as a leaf (depth 0): the whole matrix is its near field, handled above
 
     // projection tiles over the V panels; t index of every V column
     std::vector<int64_t> tidx(nvcol > 0 ? nvcol : 1, 0);
@@ -345,18 +388,20 @@ void assemble(hodlr_op &op) {
         if (op.flags[b] & 1) {
             TCopy tc;
             tc.toff = t_base[a] + coff[b];
-            tc.x0 = t.start[c];
+            tc.x0 = op.node_start[c];
             tc.n = op.b_n[b];
             tc.pad = 0;
             tcopy.push_back(tc);
             DenseBlock db;
-            db.row0 = t.start[a];
-            db.col0 = t.start[c];
+            db.row0 = op.node_start[a];
+            db.col0 = op.node_start[c];
             db.up_base = up_base[a];
             db.m = op.b_m[b];
             db.n = op.b_n[b];
             db.R = (int32_t)R[a];
             db.coff = coff[b];
+            db.prob = op.b_prob[b];
+            db.pad = 0;
             dense.push_back(db);
         } else {
             for (int k = 0; k < op.rank[b]; k++) tidx[vcol_base[c] + voff[b] + k] = t_base[a] + coff[b] + k;
@@ -365,14 +410,13 @@ void assemble(hodlr_op &op) {
     }
     std::vector<ProjItem> proj;
     std::vector<ProjReduce> red;
-    const int64_t kChunk = kProjChunk;
     int64_t slots = 0;
     for (int64_t g = 0; g < nn; g++) {
         if (!Tb[g]) continue;
-        const int64_t nr = t.count[g];
+        const int64_t nr = count[g];
         for (int64_t c0 = 0; c0 < Tb[g]; c0 += kProjTile) {
             const int w = (int)std::min<int64_t>(kProjTile, Tb[g] - c0);
-            const int64_t nch = (nr + kChunk - 1) / kChunk;
+            const int64_t nch = (nr + kProjChunk - 1) / kProjChunk;
             if (nch > 1) {
                 ProjReduce pr;
                 pr.slot0 = slots;
@@ -383,11 +427,11 @@ void assemble(hodlr_op &op) {
                 red.push_back(pr);
             }
             for (int64_t ch = 0; ch < nch; ch++) {
-                const int64_t j0 = ch * kChunk, j1 = std::min(nr, j0 + kChunk);
-                ProjItem it;
+                const int64_t j0 = ch * kProjChunk, j1 = std::min(nr, j0 + kProjChunk);
                 const int64_t tw = std::min<int64_t>(kProjTile, Tpad[g] - c0);  // tile width in memory
+                ProjItem it;
                 it.vp = vp_base[g] + c0 * nr + j0 * tw;
-                it.x0 = t.start[g] + j0;
+                it.x0 = op.node_start[g] + j0;
                 it.ld = (int32_t)tw;
                 it.rows = (int32_t)(j1 - j0);
                 it.w = w;
@@ -398,6 +442,8 @@ void assemble(hodlr_op &op) {
             if (nch > 1) slots += nch;
         }
     }
+    // largest items first: the hardware's round-robin over XCDs then
+    // balances bytes per XCD
     std::stable_sort(proj.begin(), proj.end(), [](const ProjItem &a, const ProjItem &b) {
         return (int64_t)a.rows * a.ld > (int64_t)b.rows * b.ld;
     });
@@ -413,23 +459,24 @@ void assemble(hodlr_op &op) {
     op.UP64.alloc(sizeof(double) * op.n_up);
     op.VP64.alloc(sizeof(double) * op.n_vp);
     HIP_CHECK(hipMemsetAsync(op.VP64.p, 0, sizeof(double) * op.n_vp, s));  // padding columns
+    DevBuf d_near;
+    upload(d_near, near);
     {
-        DevBuf d_near, d_colsrc, d_dense, d_dest;
-        upload(d_near, near);
+        DevBuf d_colsrc, d_dense, d_dest;
         upload(d_colsrc, colsrc);
         upload(d_dense, dense);
         upload(d_dest, dest);
-        HIP_CHECK(launch_near_fill(d_near.as<NearLeaf>(), (int64_t)near.size(), d_colsrc.as<int64_t>(), K,
-                                   op.NP64.as<double>(), s));
-        HIP_CHECK(launch_dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), K, op.leaf_start.as<int64_t>(),
-                                    op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), s));
+        HIP_CHECK(launch_near_fill(d_near.as<NearLeaf>(), (int64_t)near.size(), d_colsrc.as<int64_t>(),
+                                   op.kdev.as<KernelDev>(), op.NP64.as<double>(), s));
+        HIP_CHECK(launch_dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), op.kdev.as<KernelDev>(),
+                                    op.leaf_start.as<int64_t>(), op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
         std::vector<int64_t> lowrank;
         for (int64_t b = 0; b < nb; b++)
             if (!(op.flags[b] & 1)) lowrank.push_back(b);
         // the fill pass repeats the rank pass with the same caps
         HIP_CHECK(hipMemset(d_rank.p, 0xff, sizeof(int32_t) * (nb + 1)));
-        run_aca_pass(op, lowrank, cap, 1, K, pool, budget, d_rank, d_flags, &d_dest, s);
+        run_aca_pass(op, lowrank, cap, 1, pool, budget, d_rank, d_flags, &d_dest, s);
         std::vector<int32_t> rk(nb > 0 ? nb : 1);
         if (nb) HIP_CHECK(hipMemcpy(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
         for (int64_t b : lowrank)
@@ -448,8 +495,6 @@ void assemble(hodlr_op &op) {
         // the diagonal (entry(i, i) = diag for every i) is applied separately
         // in the working precision of the caller, so FP32 rounding of the
         // identity part of a second-kind operator does not limit refinement
-        DevBuf d_near;
-        upload(d_near, near);
         HIP_CHECK(launch_zero_self_diag(d_near.as<NearLeaf>(), (int64_t)near.size(), op.NP32.as<float>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -464,6 +509,14 @@ void assemble(hodlr_op &op) {
     op.n_tcopy = (int64_t)tcopy.size();
     op.n_panel = (int64_t)panel.size();
     op.n_seg = (int64_t)segs.size();
+    {  // per-position diagonal (a batch may mix diagonals) for the FP32 path
+        std::vector<double> dg(N > 0 ? N : 1, 0.0);
+        for (int p = 0; p < P; p++)
+            for (int64_t q = op.pos_base[p]; q < op.pos_base[p + 1]; q++) dg[q] = op.kds[p].diag;
+        upload(op.diag, dg);
+        op.uniform_diag = true;
+        for (int p = 1; p < P; p++) op.uniform_diag = op.uniform_diag && op.kds[p].diag == op.kds[0].diag;
+    }
     HIP_CHECK(hipStreamSynchronize(s));
     const double t_end = now_s();
 
@@ -557,8 +610,8 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
                                        segs.as<Segment>(), VP32.as<float>(), NP32.as<float>(), UP32.as<float>(),
                                        static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
                                        tidx.as<int64_t>(), static_cast<float *>(y), ev, s));
-        if (add_diag && kd.diag != 0.0)
-            HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x), (float)kd.diag,
-                                          (int64_t)N * nrhs, s));
+        if (add_diag)
+            HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x),
+                                          diag.as<double>(), N, nrhs, s));
     }
 }

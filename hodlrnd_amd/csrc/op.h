@@ -10,24 +10,32 @@
 
 namespace hodlr {
 
-hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev &K, double tol,
+hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,
                       double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
                       const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
                       hipStream_t s);
-hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev &K,
+hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev *K,
                              const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, hipStream_t s);
-hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev &K,
+hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev *K,
                             double *NP, hipStream_t s);
 hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s);
 hipError_t launch_zero_self_diag(const NearLeaf *l, int64_t nl, float *NP, hipStream_t s);
-hipError_t launch_f32_result(const float *y, const double *x, double diag, double *out, int64_t n, hipStream_t s);
-hipError_t launch_add_diag_f32(float *y, const float *x, float diag, int64_t n, hipStream_t s);
+hipError_t launch_f32_result(const float *y, const double *x, const double *diag, double *out, int64_t n, int k,
+                             hipStream_t s);
+hipError_t launch_add_diag_f32(float *y, const float *x, const double *diag, int64_t n, int k, hipStream_t s);
 hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                          const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
                          const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
                          T *t, T *part, const int64_t *tidx, T *y, hipEvent_t *ev, hipStream_t s);
+
+hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const Chunk *chunks, int nchunk,
+                        double *partial, const int32_t *pchunk, int nprob, int K, double *out, hipStream_t s);
+hipError_t launch_bgemv(const double *const *V, int nv, const double *coef, double sign, double *w,
+                        const Chunk *chunks, int nchunk, int K, hipStream_t s);
+hipError_t launch_bscale(const double *src, const double *coef, double *dst, const Chunk *chunks, int nchunk, int K,
+                         hipStream_t s);
 
 int64_t mdot_chunks(int64_t n);
 hipError_t launch_mdot(const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *partial,
@@ -42,9 +50,15 @@ hipError_t launch_to_f64(const float *a, double *b, int64_t n, hipStream_t s);
 
 }  // namespace hodlr
 
+// The operator of one problem, or the block-diagonal operator of a batch of
+// independent problems (their vectors concatenated in problem order, each
+// in its own tree order).
 struct hodlr_op {
     hodlr_ctx *ctx = nullptr;
-    hodlr_tree *tree = nullptr;
+    hodlr_tree *tree = nullptr;           // first problem's tree
+    std::vector<hodlr_tree *> trees;      // every problem's tree
+    std::vector<hodlr_kernel_desc> kds;   // every problem's kernel (weights not kept)
+    std::vector<int64_t> pos_base;        // nprob + 1: first global position of each problem
     hodlr_kernel_desc kd{};
     double eps = 0.0;
     int prec_flags = 0;
@@ -52,12 +66,18 @@ struct hodlr_op {
 
     // tree-order coordinates, weights, per-position leaf start / size
     hodlr::DevBuf X, W, leaf_start, leaf_cnt;
-    hodlr::DevBuf ord;   // nlevels x N: per node, tree positions in ascending point id
+    hodlr::DevBuf ord;   // per problem and level: global positions in ascending point id per node
+    hodlr::DevBuf kdev;  // per problem KernelDev
+    hodlr::DevBuf diag;  // per global position: A_ii (the FP32 copy stores a zero diagonal)
+    bool uniform_diag = true;
 
     // blocks in reference order (level, node, interaction index)
     int64_t nblk = 0;
-    std::vector<int32_t> b_level, b_m, b_n, rank, flags;
-    std::vector<int64_t> b_row, b_col;   // level-major node ids
+    std::vector<int32_t> b_level, b_m, b_n, b_prob, rank, flags;
+    std::vector<int64_t> b_row, b_col;   // global node ids (problem base + level-major id)
+    std::vector<int64_t> b_rowj, b_colj; // node index within its level (seeds)
+    std::vector<int64_t> node_start;     // global node id -> first global position
+    std::vector<int64_t> ord_off;        // (problem, level) -> offset of its order map
 
     // arenas: near field, U panels, V panels
     int64_t n_np = 0, n_up = 0, n_vp = 0, n_t = 0, n_slots = 0;
@@ -76,6 +96,7 @@ struct hodlr_op {
 
     // solver state kept between solves (Krylov basis, mixed-mode vectors)
     std::shared_ptr<void> solver_ws;
+    std::shared_ptr<void> batch_ws;
 
     // live kernel timing: event triplets recorded around the projection and
     // panel launches of every matvec while profiling is on
@@ -95,5 +116,6 @@ struct hodlr_op {
 };
 
 namespace hodlr {
+// op.trees and op.kds must be set; weights in kds are read during assembly only
 void assemble(hodlr_op &op);
 }

@@ -111,7 +111,8 @@ int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int 
         HIP_CHECK(launch_scale(r, 1.0 / beta, V, n, s));
         std::fill(sv.begin(), sv.end(), 0.0);
         sv[0] = beta;
-        for (int i = 0; i < m && j <= max_iter; i++, j++) {
+        int i = 0;
+        for (; i < m && j <= max_iter; i++, j++) {
             if (hist) hist->push_back(resid);
             double *vi = V + (size_t)i * n;
             A(vi, w);
@@ -144,7 +145,9 @@ int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int 
                 return j;
             }
         }
-        update(m - 1);
+        // GMRES.hpp:271 updates with m - 1 columns; when the iteration budget
+        // ends the cycle early only i columns exist
+        update(i - 1);
         A(x, w);
         res.matvecs++;
         HIP_CHECK(launch_sub(b, w, r, n, s));
@@ -217,7 +220,7 @@ void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p
         Apply A32 = [&](const double *in, double *out) {
             HIP_CHECK(launch_to_f32(in, x32.as<float>(), n, s));
             op.matvec_dev(HODLR_PREC_F32, 1, x32.as<float>(), y32.as<float>(), s, nullptr, false);
-            HIP_CHECK(launch_f32_result(y32.as<float>(), in, op.kd.diag, out, n, s));
+            HIP_CHECK(launch_f32_result(y32.as<float>(), in, op.diag.as<double>(), out, n, 1, s));
         };
         const double nb0 = ws.norm2(b, s);
         const double nb = nb0 == 0.0 ? 1.0 : nb0;

@@ -34,4 +34,9 @@ int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int 
 void solve(struct ::hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p,
            hodlr_gmres_info &info);
 
+// every (problem, column) of a batch operator solved independently;
+// b, x: n x K interleaved device arrays; infos: nprob * K entries
+void solve_batch(struct ::hodlr_op &op, int K, const double *b, double *x, const hodlr_gmres_params &p,
+                 hodlr_gmres_info *infos);
+
 }  // namespace hodlr

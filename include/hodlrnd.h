@@ -154,6 +154,18 @@ void hodlr_tree_destroy(hodlr_tree *t);
  * the dense near field.  The tree must outlive the operator. */
 int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
                       double eps_aca, int prec_flags, hodlr_op **out);
+/* A batch of independent problems assembled together (no reference
+ * counterpart; the reference assembles one operator at a time through
+ * HDD_matrix.hpp:39-41).  The result is their block-diagonal operator: one
+ * ACA launch per pass and one launch per matvec kernel serve every problem.
+ * Device vectors concatenate the problems in order, each in its own tree
+ * order; hodlr_op_problem_offsets gives the nprob + 1 boundaries (and
+ * returns nprob).  Each problem's blocks and ranks are the same as when it
+ * is assembled alone. */
+int hodlr_op_assemble_batch(hodlr_ctx *ctx, int nprob, hodlr_tree *const *trees,
+                            const hodlr_kernel_desc *k, double eps_aca, int prec_flags,
+                            hodlr_op **out);
+int hodlr_op_problem_offsets(hodlr_op *op, int64_t *offsets, int cap);
 /* HDD_matrix.hpp:69-85 */
 int hodlr_op_get_stats(hodlr_op *op, hodlr_op_stats *s);
 /* ranks and dense flags in block order (level, node, interaction index) */
@@ -178,6 +190,20 @@ int hodlr_gmres(hodlr_op *op, const double *b, double *x,
 /* Same with b and x on the device (tree order, double). */
 int hodlr_gmres_device(hodlr_op *op, const double *b, double *x,
                        const hodlr_gmres_params *params, hodlr_gmres_info *info);
+
+/* Every (problem, right-hand side) of a batch operator solved as its own
+ * GMRES instance with the semantics of GMRES.hpp:160-303 (no reference
+ * counterpart for the batching itself).  The instances advance in lockstep
+ * so each Arnoldi step is one nrhs-column matvec over the whole batch; an
+ * instance's arithmetic is the same alone or in any batch.  Device form:
+ * b, x are N x nrhs interleaved (element (i, c) at i * nrhs + c), tree order,
+ * nrhs in {1, 2, 4, 8, 16}.  Host form: problems concatenated, each in point
+ * order, column-major N x nrhs (nrhs <= 16).  infos holds nprob * nrhs
+ * entries, instance p * nrhs + c. */
+int hodlr_gmres_batch_device(hodlr_op *op, int nrhs, const double *b, double *x,
+                             const hodlr_gmres_params *params, hodlr_gmres_info *infos);
+int hodlr_gmres_batch(hodlr_op *op, int nrhs, const double *b, double *x,
+                      const hodlr_gmres_params *params, hodlr_gmres_info *infos);
 
 /* ---- measurement ---- */
 /* Times `reps` back-to-back matvecs on the op's stream with HIP events per
