@@ -39,28 +39,16 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_points(n1d, a=-1.0, b=1.0):
-    """uniform_nodes(a, b, n) (myHeaders.hpp:111-120) on every axis, tensor
+def make_points(n1d):
+    """uniform_nodes(-1, 1, n1d) (myHeaders.hpp:111-120) on every axis, tensor
     grid with dimension 0 outermost (user_kernel.hpp:79-95)."""
-    nodes = a + (b - a) / float(n1d - 1) * np.arange(n1d, dtype=np.float64)
-    g = np.meshgrid(nodes, nodes, nodes, indexing="ij")
-    return np.ascontiguousarray(np.stack([c.ravel() for c in g], axis=1))
+    from hodlrnd_amd import workloads as Wl
+    return Wl.grid_points(n1d)
 
 
 def splitmix_uniform(seed, n):
-    """SplitMix64 -> u = (z >> 11) * 2^-53 -> 2u - 1 (same stream as the tests)."""
-    out = np.empty(n, dtype=np.float64)
-    s = np.uint64(seed)
-    gold = np.uint64(0x9E3779B97F4A7C15)
-    m1, m2 = np.uint64(0xBF58476D1CE4E5B9), np.uint64(0x94D049BB133111EB)
-    idx = np.arange(1, n + 1, dtype=np.uint64)
-    with np.errstate(over="ignore"):
-        z = s + idx * gold
-        z = (z ^ (z >> np.uint64(30))) * m1
-        z = (z ^ (z >> np.uint64(27))) * m2
-        z = z ^ (z >> np.uint64(31))
-    out[:] = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
-    return 2.0 * out - 1.0
+    from hodlrnd_amd import workloads as Wl
+    return Wl.splitmix_uniform(seed, n)
 
 
 def measured_traffic(kernel, alg_bytes):
@@ -130,6 +118,211 @@ def cpu_baseline(pts, n_mv, nmax, eps, budget_s=20.0):
     }
 
 
+def to_tree_interleaved(trees, off, M):
+    """Host N x K (problems concatenated, point order) -> tree order with the
+    K columns interleaved, the device layout of the batch entry points."""
+    out = np.empty_like(M)
+    for p, t in enumerate(trees):
+        out[off[p]:off[p + 1]] = M[off[p]:off[p + 1]][t.perm()]
+    return np.ascontiguousarray(out)
+
+
+def from_tree_interleaved(trees, off, M):
+    out = np.empty_like(M)
+    for p, t in enumerate(trees):
+        out[off[p]:off[p + 1]][t.perm()] = M[off[p]:off[p + 1]]
+    return out
+
+
+def cpu_baseline_batch(specs, nmax, eps, tol, budget_s=25.0):
+    """The FP64 CPU restatement solving problems of the batch one at a time
+    (one thread, reference execution model), over problems spread across the
+    batch until the budget is used; solves/s = problems solved / time."""
+    from oracle import pyoracle as O
+    from hodlrnd_amd import workloads as Wl
+    t0 = time.time()
+    done, t_solve = [], 0.0
+    order = list(range(0, len(specs), 16)) + list(range(8, len(specs), 16))
+    for p in order:
+        n, nd = specs[p]
+        X = Wl.problem_points(p, n, nd)
+        T = O.Tree(X, -np.ones(nd), np.ones(nd), nmax, 0)
+        k = O.KernelDesc(O.K_EXP, nd, 1.0 / n, 1.0 + 1.0 / n)
+        op = O.Operator(T, k, eps, sys_size=2 * nmax, nthreads=1)
+        b = Wl.splitmix_uniform(11_000 + p, n)
+        t1 = time.time()
+        op.gmres(b, tol=tol, nthreads=1)
+        t_solve += time.time() - t1
+        done.append(p)
+        del op
+        if time.time() - t0 > budget_s:
+            break
+    return {
+        "value": len(done) / t_solve,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("oracle FP64 restatement, 1 thread, reference execution model (skeleton blocks for "
+                   "|cols| > 2*Nmax), GMRES(100) from x0=b to %g (assembly untimed) for 1 RHS of each of %d "
+                   "problems spread over the batch (%s); solves/s = problems / time" %
+                   (tol, len(done), ",".join(str(q) for q in done))),
+        "wall_s": time.time() - t0,
+    }
+
+
+def bench_batch(args, H):
+    """Batched uneven problems: 256 independent systems, N log-uniform in
+    [2000, 64000], NDIM from {2, 3}, points uniform in [-1,1]^d,
+    A = I + (1/N) exp(-r) (A_ii = 1 + 1/N), Nmax 64, eps_ACA 1e-8, d' = 0,
+    8 uniform right-hand sides per system, all solved to 1e-8 as one batch."""
+    from hodlrnd_amd import workloads as Wl
+    K = 8
+    specs = Wl.batch_specs(args.nprob)
+    trees, kerns = [], []
+    t0 = time.time()
+    for p, (n, nd) in enumerate(specs):
+        X = Wl.problem_points(p, n, nd)
+        trees.append(H.Tree(X, -np.ones(nd), np.ones(nd), args.nmax, 0))
+        kerns.append(H.Kernel(H.KERNEL_EXP, nd, 1.0 / n, 1.0 + 1.0 / n))
+    t_tree = time.time() - t0
+    t0 = time.time()
+    op = H.Operator(trees, kerns, args.eps, H.PREC_F64 | H.PREC_F32)
+    t_asm = time.time() - t0
+    st = op.stats()
+    N = op.n
+    off = op.offsets()
+    NI = len(specs) * K
+    log("batch: %d problems, N total %d, trees %.2fs, assembled in %.2fs: %.1f GB FP64 + %.1f GB FP32"
+        % (len(specs), N, t_tree, t_asm, st["bytes_f64"] * 1e-9, st["bytes_f32"] * 1e-9))
+    t0 = time.time()
+    op.tune(reps=2)
+    t_tune = time.time() - t0
+    variants = {("f64" if pr == H.PREC_F64 else "f32") + "_k%d" % k: op.get_variant(pr, k)
+                for pr in (H.PREC_F64, H.PREC_F32) for k in (4, 8, 16)}
+    log("kernel forms tuned in %.1fs: %s" % (t_tune, variants))
+    ctx = op.ctx
+    B = np.empty((N, K))
+    for p in range(len(specs)):
+        for c in range(K):
+            B[off[p]:off[p + 1], c] = Wl.splitmix_uniform(11_000 + 1000 * c + p, off[p + 1] - off[p])
+    db = H.DeviceArray(ctx, 8 * N * K)
+    dx = H.DeviceArray(ctx, 8 * N * K)
+    db.upload(to_tree_interleaved(trees, off, B))
+
+    def run(mode, k):
+        x0 = "b" if mode == H.SOLVE_F64 else "zero"
+        lat, infos = [], None
+        for _ in range(k):
+            t1 = time.perf_counter()
+            infos = op.gmres_batch_device(db.ptr, dx.ptr, K, restart=100, tol=args.tol, mode=mode, x0=x0)
+            lat.append(time.perf_counter() - t1)
+        return np.array(lat), infos
+
+    run(H.SOLVE_MIXED, args.warmup)
+    op.profile(True)
+    op.profile_read(reset=True)
+    latm, infm = run(H.SOLVE_MIXED, args.steps)
+    ms, cnt = op.profile_read(reset=True)
+    op.profile(False)
+    log("batched mixed: %.1f ms per batch of %d solves" % (latm.mean() * 1e3, NI))
+    Xt = dx.download(np.empty((N, K)))
+    X = from_tree_interleaved(trees, off, Xt)
+    R = B - op.matvec(X)
+    res = [np.linalg.norm(R[off[p]:off[p + 1], c]) / np.linalg.norm(B[off[p]:off[p + 1], c])
+           for p in range(len(specs)) for c in range(K)]
+    run(H.SOLVE_F64, 1)
+    lat64b, inf64b = run(H.SOLVE_F64, max(1, args.steps // 2))
+    log("batched fp64: %.1f ms per batch" % (lat64b.mean() * 1e3))
+
+    # the plain FP64 GPU path: every (problem, RHS) solved alone with the
+    # single-problem solver on an FP64 operator of that problem (x0 = b)
+    del db, dx
+    t_plain, n_plain, it_plain = 0.0, 0, []
+    nsample = len(specs) if args.plain_all else min(len(specs), 64)
+    picks = np.linspace(0, len(specs) - 1, nsample).round().astype(int)
+    for p in picks:
+        n = specs[p][0]
+        o1 = H.Operator(trees[p], kerns[p], args.eps, H.PREC_F64)
+        perm = trees[p].perm()
+        d1b = H.DeviceArray(ctx, 8 * n)
+        d1x = H.DeviceArray(ctx, 8 * n)
+        for c in range(K):
+            d1b.upload(np.ascontiguousarray(B[off[p]:off[p + 1], c][perm]))
+            t1 = time.perf_counter()
+            inf = o1.gmres_device(d1b.ptr, d1x.ptr, restart=100, max_iter=n, tol=args.tol, mode=H.SOLVE_F64,
+                                  x0="b")
+            t_plain += time.perf_counter() - t1
+            n_plain += 1
+            it_plain.append(inf["iterations"])
+        del o1, d1b, d1x
+    plain_sps = n_plain / t_plain
+    log("plain fp64 single solves: %.3f ms each (%d problems x %d RHS)" % (t_plain / n_plain * 1e3, nsample, K))
+
+    kb32 = op.kernel_bytes(H.PREC_F32, K)
+    kb64 = op.kernel_bytes(H.PREC_F64, K)
+    cands = [("matvec_proj_f64", ms[0], cnt[0], kb64[1]), ("matvec_panel_f64", ms[1], cnt[0], kb64[2]),
+             ("matvec_proj_f32", ms[2], cnt[1], kb32[1]), ("matvec_panel_f32", ms[3], cnt[1], kb32[2])]
+    name, tot_ms, c, bytes_ = max(cands, key=lambda t: t[1])
+    avg_ms = tot_ms / max(c, 1)
+    achieved = bytes_ / (avg_ms * 1e-3) * 1e-9
+    mv32 = (ms[2] + ms[3]) / max(cnt[1], 1)
+    copy_gbs = H.stream_copy_gbps(1 << 30, 5)
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle, problems one at a time)...")
+        cpu = cpu_baseline_batch(specs, args.nmax, args.eps, args.tol)
+        log("cpu baseline: %.4g solves/s" % cpu["value"])
+    val = NI / latm.mean()
+    iters = [i["iterations"] for i in infm]
+    out = {
+        "metric": METRIC,
+        "value": val,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": latm.mean() * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 matvec + fp64 refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d independent problems, N log-uniform [2000, 64000] (total %d), NDIM in {2,3}, "
+                        "points uniform [-1,1]^d, A=I+(1/N)exp(-r) (A_ii=1+1/N), Nmax=%d, eps_ACA=%g, "
+                        "d'=0, %d RHS each, GMRES(100) to rel. residual %g; a step solves the whole batch"
+                        % (len(specs), N, args.nmax, args.eps, K, args.tol),
+            "problems": len(specs), "rhs_per_problem": K, "n_total": int(N),
+        },
+        "p50_ms": float(np.percentile(latm, 50) * 1e3),
+        "p99_ms": float(np.percentile(latm, 99) * 1e3),
+        "instances_per_step": NI,
+        "max_true_resid": float(max(res)),
+        "all_converged": bool(all(i["converged"] for i in infm)),
+        "inner_iterations": {"max": int(max(iters)), "mean": float(np.mean(iters))},
+        "refinement_sweeps": int(infm[0]["outer"]),
+        "batched_fp64_solves_per_s": NI / lat64b.mean(),
+        "plain_fp64_solves_per_s": plain_sps,
+        "plain_fp64_sample": "%d problems x %d RHS, single-problem FP64 operator and solver, x0=b" % (nsample, K),
+        "plain_fp64_iterations_mean": float(np.mean(it_plain)),
+        "speedup_vs_plain_fp64": val / plain_sps,
+        "speedup_vs_batched_fp64": val * lat64b.mean() / NI,
+        "matvec_ms_f32_k8": mv32,
+        "matvec_hbm_gbs_f32_k8": kb32[0] / (mv32 * 1e-3) * 1e-9 if mv32 else None,
+        "assemble_s": t_asm,
+        "tune_s": t_tune,
+        "kernel_forms": variants,
+        "operator_gb": {"f64": st["bytes_f64"] * 1e-9, "f32": st["bytes_f32"] * 1e-9},
+        "roofline": {
+            "bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "measured_copy_gbs": copy_gbs, "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_, "avg_launch_ms": avg_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +334,12 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["grid100", "batch256"], default="grid100",
+                    help="grid100: one N=1e6 solve per step; batch256: 256 uneven problems x 8 RHS "
+                         "solved as one batch per step")
+    ap.add_argument("--nprob", type=int, default=256)
+    ap.add_argument("--plain-all", action="store_true",
+                    help="batch256: time the plain FP64 path on every problem, not 64 spread ones")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's plain copy kernel (1 GiB, 3 launches): the "
                          "known-size kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -156,6 +355,12 @@ def main():
         g = H.stream_copy_gbps(1 << 30, 3)
         print(json.dumps({"copy_kernel": "hodlr::stream_copy_kernel", "bytes_read_per_launch": 1 << 30,
                           "bytes_written_per_launch": 1 << 30, "launches": 4, "gbps": g}))
+        return
+
+    if args.workload == "batch256":
+        if args.nmax == 128:
+            args.nmax = 64
+        bench_batch(args, H)
         return
 
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)

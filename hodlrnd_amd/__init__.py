@@ -33,6 +33,9 @@ class HodlrError(RuntimeError):
         self.code = code
 
 
+MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT = 1, 2, 4
+
+
 class KernelDesc(C.Structure):
     _fields_ = [("family", C.c_int32), ("ndim", C.c_int32), ("scale", C.c_double),
                 ("diag", C.c_double), ("weight", C.c_void_p)]
@@ -86,6 +89,9 @@ SIGNATURES = {
     "hodlr_gmres_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hodlr_gmres_batch_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p]),
+    "hodlr_op_tune_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "hodlr_op_set_matvec_variant": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "hodlr_op_get_matvec_variant": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "hodlr_op_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_op_block_ranks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "hodlr_op_destroy": (None, [C.c_void_p]),
@@ -376,6 +382,18 @@ class Operator:
         if rc not in (OK, E_NOCONV):
             self.ctx.check(rc, "gmres_device")
         return {f: getattr(info, f) for f, _ in GmresInfo._fields_}
+
+    def tune(self, prec_mask=PREC_F64 | PREC_F32, reps=3):
+        """Pick the fastest kernel form per matvec pass for 1..32 RHS."""
+        self.ctx.check(lib.hodlr_op_tune_matvec(self.h, prec_mask, reps), "tune_matvec")
+
+    def set_variant(self, prec, nrhs, variant):
+        self.ctx.check(lib.hodlr_op_set_matvec_variant(self.h, prec, nrhs, variant), "set_variant")
+
+    def get_variant(self, prec, nrhs):
+        v = C.c_int()
+        self.ctx.check(lib.hodlr_op_get_matvec_variant(self.h, prec, nrhs, C.byref(v)), "get_variant")
+        return v.value
 
     def matvec_timing(self, prec=PREC_F64, nrhs=1, reps=10):
         ms = np.zeros(3)

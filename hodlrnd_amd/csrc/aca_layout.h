@@ -8,6 +8,13 @@ namespace hodlr {
 constexpr int kMaxTries = 10;     // LowRank_matrix.hpp:117
 constexpr int kProjTile = 256;    // columns of one V-panel tile (projection work item)
 constexpr int kProjChunk = 2048;  // rows of one projection work item
+// matvec kernel forms (bit set): matrix-core projection / panel product
+// (only used with 4 or more right-hand sides); vector-ALU panel product
+// walking the leaf's segments as one flattened column list (otherwise one
+// segment at a time)
+constexpr int kMatvecProjMfma = 1;
+constexpr int kMatvecPanelMfma = 2;
+constexpr int kMatvecPanelFlat = 4;
 
 // one admissible block for the ACA kernel
 // Row i of the block is the i-th smallest point id of the row node (the
@@ -60,7 +67,6 @@ __host__ __device__ inline int64_t aca_scratch_doubles(int m, int n, int cap) {
 
 // ---- matvec work ----
 
-
 // V-panel projection tile: t[cols] (+)= VP[rows j0..j1, c0..c0+w)^T x[b0+j0..]
 struct ProjItem {
     int64_t vp;           // element offset of (row j0, column c0) in the V arena
@@ -78,12 +84,6 @@ struct ProjReduce {
     int32_t nchunks, w, tcol, pad;
 };
 
-// dense-fallback block: t[toff..toff+n) = x[x0..x0+n)
-struct TCopy {
-    int64_t toff, x0;
-    int32_t n, pad;
-};
-
 // one row tile of one leaf for the panel kernel
 struct PanelItem {
     int64_t y0;           // first tree position of the tile's rows
@@ -92,11 +92,14 @@ struct PanelItem {
     int32_t h;            // rows in this tile (<= 64)
     int32_t ld;           // leaf size (column stride of every segment)
     int32_t r0;           // row offset of the tile inside the leaf
+    int32_t nq4;          // sum over segments of ceil(len / 4)
+    int32_t nqg;          // sum over segments of ceil(len / G), G = 64 / h
 };
 
 // one contiguous ld x len column-major matrix times a vector slice:
-//   arena 0 (near-field block): column q multiplies x[vec + q];
-//   arena 1 (U panel slice):    column q multiplies t[vec + q]
+//   arena 0 (near-field block):           column q multiplies x[vec + q];
+//   arena 1 (U panel, low-rank blocks):   column q multiplies t[vec + q];
+//   arena 2 (U panel, dense-fallback K):  column q multiplies x[vec + q]
 struct Segment {
     int64_t mat;          // element offset in the arena (row 0 of the leaf)
     int64_t vec;

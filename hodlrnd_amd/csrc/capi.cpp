@@ -411,6 +411,40 @@ int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes) {
     return HODLR_OK;
 }
 
+int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps) {
+    if (!op || reps < 1) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        set_device(op->ctx);
+        op->tune_matvec(prec_mask, reps);
+    });
+}
+
+namespace {
+int variant_slot(int prec, int nrhs) {
+    if (prec != HODLR_PREC_F64 && prec != HODLR_PREC_F32) return -1;
+    if (padded_rhs(nrhs) != nrhs) return -1;
+    int lg = 0;
+    while ((1 << lg) < nrhs) lg++;
+    return lg;
+}
+}  // namespace
+
+int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant) {
+    if (!op || variant < 0 || variant > 7) return HODLR_E_ARG;
+    const int lg = variant_slot(prec, nrhs);
+    if (lg < 0) return HODLR_E_ARG;
+    op->mv_variant[prec == HODLR_PREC_F64 ? 0 : 1][lg] = variant;
+    return HODLR_OK;
+}
+
+int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant) {
+    if (!op || !variant) return HODLR_E_ARG;
+    const int lg = variant_slot(prec, nrhs);
+    if (lg < 0) return HODLR_E_ARG;
+    *variant = op->mv_variant[prec == HODLR_PREC_F64 ? 0 : 1][lg];
+    return HODLR_OK;
+}
+
 int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms) {
     if (!op || !ms || reps < 1) return HODLR_E_ARG;
     return guard(op->ctx, [&] {

@@ -2,8 +2,8 @@
 //
 // The reference applies every node's blocks one GEMV at a time
 // (HDD_tree.hpp:86-107, HDD_node.hpp:223-254, LowRank_matrix.hpp:40-63).
-// Here the operator is stored in three streaming arenas and applied by
-// three launches, each reading its arena exactly once:
+// Here the operator is stored in three streaming arenas and applied by two
+// passes, each reading its arena exactly once:
 //   proj:   t = V^T x for every compressed block.  The V factors of all
 //           blocks that share a column node b form one panel (n_b x T_b,
 //           T_b padded to a multiple of 4) stored as 256-column tiles, each
@@ -12,13 +12,16 @@
 //           x_b[j] is a broadcast.  Tall tiles are cut
 //           into row chunks whose partial sums are added by a second small
 //           pass (deterministic order).
-//   tcopy:  dense-fallback blocks take t = x_b (their U panel holds K).
-//   panel:  y = sum over segments of M x_seg, one workgroup per leaf row
+//   panel:  y = sum over segments of M v_seg, one workgroup per leaf row
 //           tile.  A leaf's segments are its near-field blocks (self and
-//           neighbours) and one slice of the U panel of each ancestor, all
+//           neighbours, times x) and its slice of the U panel of each
+//           ancestor, cut into runs of low-rank blocks (times t) and runs of
+//           dense-fallback blocks, whose U columns hold K (times x).  All are
 //           column-major with leading dimension = leaf size, so G = 64/h
 //           column groups of h lanes read G*h contiguous elements per step.
 //           Each leaf row is written once (no atomics).
+// With 4 or more right-hand sides both passes also have matrix-core forms
+// (further down); the operator picks one per pass.
 #include "aca_layout.h"
 #include "device_common.h"
 
@@ -60,7 +63,8 @@ __global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ 
                                                    const int64_t *__restrict__ tidx) {
     constexpr int U = 4;  // rows in flight per wave per step
     __shared__ T red[kProjTile][K];
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    // uniform wave index: the row loop is scalar, x_b[j] a scalar load
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     // items are sorted by size, largest first: the hardware's round-robin
     // dispatch over XCDs then balances bytes per XCD
     const ProjItem P = items[blockIdx.x];
@@ -133,24 +137,18 @@ __global__ __launch_bounds__(256) void proj_reduce_kernel(const ProjReduce *__re
     for (int k = 0; k < K; k++) t[ti * K + k] = acc[k];
 }
 
+// Vector-ALU panel product, one segment at a time.  Lane (o, g) of a wave
+// owns row o of the tile and column group g (G = 64 / h groups of h lanes);
+// the 4 waves interleave column groups inside each segment, kU predicated
+// column steps per iteration so short segments still keep kU loads in
+// flight.  Best when segments are long (wide U slices, large leaves).
 template <typename T, int K>
-__global__ __launch_bounds__(256) void tcopy_kernel(const TCopy *__restrict__ items, int64_t nitems,
-                                                    const T *__restrict__ x, T *__restrict__ t) {
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const int64_t it = (int64_t)blockIdx.x * 4 + wid;
-    if (it >= nitems) return;
-    const TCopy C = items[it];
-    const int64_t tot = (int64_t)C.n * K;
-    for (int64_t e = lane; e < tot; e += kWave) t[C.toff * K + e] = x[C.x0 * K + e];
-}
-
-template <typename T, int K>
-__global__ __launch_bounds__(256) void panel_kernel(const PanelItem *__restrict__ items,
-                                                    const Segment *__restrict__ segs,
-                                                    const T *__restrict__ NP, const T *__restrict__ UP,
-                                                    const T *__restrict__ x, const T *__restrict__ t,
-                                                    T *__restrict__ y) {
-    constexpr int NW = 4;
+__global__ __launch_bounds__(256) void panel_seg_kernel(const PanelItem *__restrict__ items,
+                                                        const Segment *__restrict__ segs,
+                                                        const T *__restrict__ NP, const T *__restrict__ UP,
+                                                        const T *__restrict__ x, const T *__restrict__ t,
+                                                        T *__restrict__ y) {
+    constexpr int NW = 4, kU = 8;
     __shared__ T red[NW][kWave][K];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
@@ -165,14 +163,10 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelItem *__restrict_
     for (int k = 0; k < K; k++) acc[k] = T(0);
     for (int s = 0; s < P.nseg; s++) {
         const Segment S = segs[P.seg0 + s];
-        const bool near = S.arena == 0;
-        const T *M = (near ? NP : UP) + S.mat + P.r0 + o;
-        const T *v = (near ? x : t) + S.vec * K;
+        const T *M = (S.arena == 0 ? NP : UP) + S.mat + P.r0 + o;
+        const T *v = (S.arena == 1 ? t : x) + S.vec * K;
         if (!active) continue;
         const int len = S.len;
-        // kU column steps per iteration, predicated, so short segments
-        // (near-field blocks of ~leaf-size width) still keep kU loads in flight
-        constexpr int kU = 8;
         for (int q = gg; q < len; q += kU * GG) {
             T a[kU], vv[kU][K];
 #pragma unroll
@@ -201,31 +195,281 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelItem *__restrict_
     }
 }
 
+// Vector-ALU panel product, flattened walk.  Lane (o, g) of a wave owns row o of the tile
+// and column group g (G = 64 / h groups of h lanes).  The leaf's segments
+// are walked as one list of G-column steps (ceil(len / G) per segment); the
+// 4 waves take consecutive groups of kU steps in turn, so kU loads stay in
+// flight across segment boundaries.  The walk state is wave-uniform (scalar
+// loads).  Per-lane partial sums are added through LDS in a fixed order.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void panel_kernel(const PanelItem *__restrict__ items,
+                                                    const Segment *__restrict__ segs,
+                                                    const T *__restrict__ NP, const T *__restrict__ UP,
+                                                    const T *__restrict__ x, const T *__restrict__ t,
+                                                    T *__restrict__ y) {
+    constexpr int NW = 4, kU = 8;
+    __shared__ T red[NW][kWave][K];
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const int h = P.h;
+    const int G = kWave / h;
+    const int o = lane % h, g = lane / h;
+    const bool active = g < G;
+    const int64_t ld = P.ld, nq = P.nqg;
+    T acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = T(0);
+    int s = -1, len = 0;
+    int64_t qbeg = 0, qend = 0;
+    const T *Mb = nullptr, *vb = nullptr;
+    for (int64_t g0 = (int64_t)wid * kU; g0 < nq; g0 += NW * kU) {
+        T a[kU], vv[kU][K];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int64_t step = g0 + u;
+            while (step >= qend && s + 1 < P.nseg) {
+                s++;
+                const Segment S = segs[P.seg0 + s];
+                qbeg = qend;
+                qend += (S.len + G - 1) / G;
+                len = S.len;
+                Mb = (S.arena == 0 ? NP : UP) + S.mat + P.r0;
+                vb = (S.arena == 1 ? t : x) + S.vec * K;
+            }
+            const int col = (int)(step - qbeg) * G + g;
+            const bool ok = active && step < nq && col < len;
+            a[u] = ok ? load_stream(Mb + (int64_t)col * ld + o) : T(0);
+#pragma unroll
+            for (int k = 0; k < K; k++) vv[u][k] = ok ? vb[(int64_t)col * K + k] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++) acc[k] = fma(a[u], vv[u][k], acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) red[wid][lane][k] = active ? acc[k] : T(0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < h * K; e += blockDim.x) {
+        const int row = e / K, k = e % K;
+        T sum = T(0);
+        for (int w = 0; w < NW; w++)
+            for (int gi = 0; gi < G; gi++) sum += red[w][gi * h + row][k];
+        y[(P.y0 + row) * K + k] = sum;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core forms for many right-hand sides.  With K >= 4 columns the
+// products are small GEMMs: a 16x16x4 MFMA takes a 16-row x 4-column piece
+// of the operator (one element per lane) and 4 rows x 16 RHS of the vector
+// block, so one A load, one B load and one MFMA replace 4K vector FMAs and
+// K broadcast loads per element.  Lane layouts of the 16x16x4 builtins:
+// A[i][k] and B[k][j] come from lane i + 16k (resp. j + 16k); accumulator
+// register r of lane l holds D[row(l, r)][l % 16].
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+template <>
+struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+
+// Panel product on matrix cores.  The leaf tile's rows are cut into row
+// tiles of 16 (NT <= 4 for h <= 64); the 4 waves are dealt over the row
+// tiles (4 / NT waves each) and every wave walks its tile's segments as one
+// list of 4-column quads, taking groups of kU consecutive quads in turn with
+// the other waves of its tile, so loads stay in flight across the short
+// near-field blocks.  Per quad a lane holds one operator element and one
+// vector fragment per 16 RHS (vector fragments of a quad are re-read per
+// row tile from cache).  Partial tiles are added through LDS in a fixed
+// order.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void panel_mfma_kernel(const PanelItem *__restrict__ items,
+                                                         const Segment *__restrict__ segs,
+                                                         const T *__restrict__ NP, const T *__restrict__ UP,
+                                                         const T *__restrict__ x, const T *__restrict__ t,
+                                                         T *__restrict__ y) {
+    using MF = Mfma<T>;
+    constexpr int NW = 4, NK = (K + 15) / 16, kU = sizeof(T) == 4 ? 16 : 8;
+    __shared__ T red[NW][16][K];
+    // the wave index is uniform: the segment walk stays in scalar registers
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int i16 = lane & 15, kq = lane >> 4;
+    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const int h = P.h, NT = (h + 15) >> 4;
+    const int wpt = NW / NT;                      // waves per row tile
+    const int tt = wid / wpt, wsub = wid % wpt;   // this wave's tile and rank in it
+    const int64_t ld = P.ld;
+    if (tt < NT) {
+        const int64_t qtot = P.nq4;
+        typename MF::acc_t acc[NK];
+#pragma unroll
+        for (int n = 0; n < NK; n++) acc[n] = typename MF::acc_t{0, 0, 0, 0};
+        const int r = tt * 16 + i16;
+        const bool rok = r < h;
+        // the wave's position in the segment list
+        int s = -1, len = 0;
+        int64_t qbeg = 0, qend = 0;
+        const T *Mb = nullptr, *vb = nullptr;
+        for (int64_t g0 = (int64_t)wsub * kU; g0 < qtot; g0 += (int64_t)wpt * kU) {
+            T a[kU], b[kU][NK];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int64_t quad = g0 + u;
+                while (quad >= qend && s + 1 < P.nseg) {
+                    s++;
+                    const Segment S = segs[P.seg0 + s];
+                    qbeg = qend;
+                    qend += (S.len + 3) >> 2;
+                    len = S.len;
+                    Mb = (S.arena == 0 ? NP : UP) + S.mat + P.r0;
+                    vb = (S.arena == 1 ? t : x) + S.vec * K;
+                }
+                const int col = (int)(quad - qbeg) * 4 + kq;
+                const bool ok = quad < qtot && col < len;
+                a[u] = (ok && rok) ? Mb[(int64_t)col * ld + r] : T(0);
+#pragma unroll
+                for (int n = 0; n < NK; n++) {
+                    const int j = n * 16 + i16;
+                    b[u][n] = (ok && j < K) ? vb[(int64_t)col * K + j] : T(0);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+#pragma unroll
+                for (int n = 0; n < NK; n++) acc[n] = MF::mma(a[u], b[u][n], acc[n]);
+        }
+#pragma unroll
+        for (int n = 0; n < NK; n++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = n * 16 + i16;
+                if (j < K) red[wid][MF::row(lane, q)][j] = acc[n][q];
+            }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < h * K; e += blockDim.x) {
+        const int row = e / K, k = e % K;
+        const int w0 = (row >> 4) * wpt;
+        T sum = red[w0][row & 15][k];
+        for (int w = 1; w < wpt; w++) sum += red[w0 + w][row & 15][k];
+        y[(P.y0 + row) * K + k] = sum;
+    }
+}
+
+// V^T x on matrix cores.  A (tile, row chunk) item is split over 4
+// workgroups of 64 columns; wave w of quarter q owns the 16 columns
+// 64q + 16w .. + 15 over all rows of the chunk, so no cross-wave reduction
+// is needed and short items still put 4 x 4 waves on the tile.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void proj_mfma_kernel(const ProjItem *__restrict__ items, int64_t nitems,
+                                                        const T *__restrict__ VP, const T *__restrict__ x,
+                                                        T *__restrict__ t, T *__restrict__ part,
+                                                        const int64_t *__restrict__ tidx) {
+    using MF = Mfma<T>;
+    constexpr int NK = (K + 15) / 16, kU = 8;
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int i16 = lane & 15, kq = lane >> 4;
+    const ProjItem P = items[blockIdx.x >> 2];
+    const int c0 = (blockIdx.x & 3) * 64 + wid * 16;
+    if (c0 >= P.w) return;
+    const T *base = VP + P.vp;
+    const T *xv = x + P.x0 * K;
+    const int64_t ld = P.ld;
+    const int col = c0 + i16;
+    const bool cok = col < P.w;
+    typename MF::acc_t acc[NK];
+#pragma unroll
+    for (int n = 0; n < NK; n++) acc[n] = typename MF::acc_t{0, 0, 0, 0};
+    for (int j0 = 0; j0 < P.rows; j0 += 4 * kU) {
+        T a[kU], b[kU][NK];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int row = j0 + 4 * u + kq;
+            const bool ok = row < P.rows;
+            a[u] = (ok && cok) ? base[(int64_t)row * ld + col] : T(0);
+#pragma unroll
+            for (int n = 0; n < NK; n++) {
+                const int j = n * 16 + i16;
+                b[u][n] = (ok && j < K) ? xv[(int64_t)row * K + j] : T(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+#pragma unroll
+            for (int n = 0; n < NK; n++) acc[n] = MF::mma(a[u], b[u][n], acc[n]);
+    }
+#pragma unroll
+    for (int n = 0; n < NK; n++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int oc = c0 + MF::row(lane, q), j = n * 16 + i16;
+            if (oc < P.w && j < K) {
+                if (P.out >= 0) part[(P.out * kProjTile + oc) * K + j] = acc[n][q];
+                else t[tidx[P.tcol + oc] * K + j] = acc[n][q];
+            }
+        }
+}
+
 template <typename T, int K>
 hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
-                           const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
+                           const PanelItem *pa, int64_t npa,
                            const Segment *segs, const T *VP, const T *NP, const T *UP,
                            const T *x, T *t, T *part, const int64_t *tidx, T *y,
-                           hipEvent_t *ev, hipStream_t s) {
+                           int variant, hipEvent_t *ev, hipStream_t s) {
+    constexpr bool kMfma = K >= 4;
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (npi) proj_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+    if (npi) {
+        bool done = false;
+        if constexpr (kMfma)
+            if (variant & kMatvecProjMfma) {
+                proj_mfma_kernel<T, K><<<dim3((unsigned)(4 * npi)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+                done = true;
+            }
+        if (!done) proj_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+    }
     if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 256, 0, s>>>(pr, part, t, tidx);
-    if (ntc) tcopy_kernel<T, K><<<dim3((unsigned)((ntc + 3) / 4)), 256, 0, s>>>(tc, ntc, x, t);
     if (ev) (void)hipEventRecord(ev[1], s);
-    if (npa) panel_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+    if (npa) {
+        bool done = false;
+        if constexpr (kMfma)
+            if (variant & kMatvecPanelMfma) {
+                panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                done = true;
+            }
+        if (!done) {
+            if (variant & kMatvecPanelFlat)
+                panel_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+            else
+                panel_seg_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+        }
+    }
     if (ev) (void)hipEventRecord(ev[2], s);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
-                         const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
+                         const PanelItem *pa, int64_t npa,
                          const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
-                         T *t, T *part, const int64_t *tidx, T *y, hipEvent_t *ev, hipStream_t s) {
+                         T *t, T *part, const int64_t *tidx, T *y, int variant, hipEvent_t *ev,
+                         hipStream_t s) {
 #define HODLR_MV(KK)                                                                                 \
     case KK:                                                                                         \
-        return launch_matvec_k<T, KK>(pi, npi, pr, npr, tc, ntc, pa, npa, segs, VP, NP, UP, x, t, \
-                                      part, tidx, y, ev, s);
+        return launch_matvec_k<T, KK>(pi, npi, pr, npr, pa, npa, segs, VP, NP, UP, x, t,         \
+                                      part, tidx, y, variant, ev, s);
     switch (nrhs) {
         HODLR_MV(1)
         HODLR_MV(2)
@@ -234,8 +478,8 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
         HODLR_MV(16)
     case 32:  // the panel kernel's reduction buffer is 4*64*K*sizeof(T) bytes of LDS
         if constexpr (sizeof(T) == 4)
-            return launch_matvec_k<T, 32>(pi, npi, pr, npr, tc, ntc, pa, npa, segs, VP, NP, UP, x, t,
-                                          part, tidx, y, ev, s);
+            return launch_matvec_k<T, 32>(pi, npi, pr, npr, pa, npa, segs, VP, NP, UP, x, t,
+                                          part, tidx, y, variant, ev, s);
         return hipErrorInvalidValue;
     default:
         return hipErrorInvalidValue;
@@ -244,14 +488,14 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
 }
 
 template hipError_t launch_matvec<double>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
-                                          const TCopy *, int64_t, const PanelItem *, int64_t,
+                                          const PanelItem *, int64_t,
                                           const Segment *, const double *, const double *, const double *,
                                           const double *, double *, double *, const int64_t *, double *,
-                                          hipEvent_t *, hipStream_t);
+                                          int, hipEvent_t *, hipStream_t);
 template hipError_t launch_matvec<float>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
-                                         const TCopy *, int64_t, const PanelItem *, int64_t,
+                                         const PanelItem *, int64_t,
                                          const Segment *, const float *, const float *, const float *,
                                          const float *, float *, float *, const int64_t *, float *,
-                                         hipEvent_t *, hipStream_t);
+                                         int, hipEvent_t *, hipStream_t);
 
 }  // namespace hodlr

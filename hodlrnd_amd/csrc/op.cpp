@@ -268,18 +268,32 @@ void assemble(hodlr_op &op) {
     if (verbose()) std::fprintf(stderr, "[hodlr] rank pass %.3f s, %lld blocks\n", t_rank1 - t_rank0, (long long)nb);
 
     // ---- 3. layout ----
-    std::vector<int64_t> R(nn, 0), Tb(nn, 0), up_base(nn, 0), t_base(nn, 0), vp_base(nn, 0), vcol_base(nn, 0);
-    std::vector<int32_t> width(nb), coff(nb), voff(nb);
+    // U panel of row node a: R[a] columns, its blocks in block order (a
+    // dense-fallback block contributes its n columns of K); t of row node a:
+    // TR[a] entries, the low-rank blocks only.  Dense blocks multiply x
+    // directly in the panel product.
+    std::vector<int64_t> R(nn, 0), TR(nn, 0), Tb(nn, 0), up_base(nn, 0), t_base(nn, 0), vp_base(nn, 0),
+        vcol_base(nn, 0);
+    std::vector<int32_t> width(nb), coff(nb), voff(nb), tcoff(nb, 0);
+    std::vector<int64_t> rb_ptr(nn + 1, 0), rb_list(nb > 0 ? nb : 1);  // blocks per row node, block order
     for (int64_t b = 0; b < nb; b++) {
         const bool dense = op.flags[b] & 1;
         width[b] = dense ? op.b_n[b] : op.rank[b];
         const int64_t a = op.b_row[b];
         coff[b] = (int32_t)R[a];
         R[a] += width[b];
+        rb_ptr[a + 1]++;
         if (!dense) {
+            tcoff[b] = (int32_t)TR[a];
+            TR[a] += op.rank[b];
             voff[b] = (int32_t)Tb[op.b_col[b]];
             Tb[op.b_col[b]] += op.rank[b];
         }
+    }
+    for (int64_t g = 0; g < nn; g++) rb_ptr[g + 1] += rb_ptr[g];
+    {
+        std::vector<int64_t> fillp(rb_ptr.begin(), rb_ptr.end() - 1);
+        for (int64_t b = 0; b < nb; b++) rb_list[fillp[op.b_row[b]]++] = b;
     }
     // V panels are padded to a multiple of 4 columns so that every row of a
     // projection tile starts on a 4-element boundary
@@ -292,7 +306,7 @@ void assemble(hodlr_op &op) {
         up_base[g] = op.n_up;
         op.n_up += count[g] * R[g];
         t_base[g] = op.n_t;
-        op.n_t += R[g];
+        op.n_t += TR[g];
         vp_base[g] = op.n_vp;
         op.n_vp += count[g] * Tpad[g];
         vcol_base[g] = nvcol;
@@ -328,12 +342,17 @@ void assemble(hodlr_op &op) {
             auto add_near = [&](int64_t gb) {
                 const int64_t c = count[gb];
                 if (!c) return;
-                Segment sg;
-                sg.mat = op.n_np + ncols * nc;
-                sg.vec = op.node_start[gb];
-                sg.len = (int32_t)c;
-                sg.arena = 0;
-                segs.push_back(sg);
+                if ((int64_t)segs.size() > seg0 && segs.back().arena == 0 &&
+                    segs.back().vec + segs.back().len == op.node_start[gb]) {
+                    segs.back().len += (int32_t)c;  // neighbour adjacent in x: same segment
+                } else {
+                    Segment sg;
+                    sg.mat = op.n_np + ncols * nc;
+                    sg.vec = op.node_start[gb];
+                    sg.len = (int32_t)c;
+                    sg.arena = 0;
+                    segs.push_back(sg);
+                }
                 for (int64_t q = op.node_start[gb]; q < op.node_start[gb] + c; q++) colsrc.push_back(q);
                 ncols += c;
             };
@@ -343,16 +362,32 @@ void assemble(hodlr_op &op) {
             nl.ncols = (int32_t)ncols;
             near.push_back(nl);
             op.n_np += nc * ncols;
-            // far field: one U-panel slice per ancestor (and the leaf itself)
+            // far field: the leaf's slice of the U panel of every ancestor
+            // (and of the leaf itself), cut into segments: a run of low-rank
+            // blocks multiplies their contiguous t entries, a run of dense
+            // blocks whose column nodes are adjacent multiplies x directly
             for (int l = 1; l <= L; l++) {
                 const int64_t ga = node_base[p] + t.node(l, j >> (d * (L - l)));
                 if (!R[ga]) continue;
-                Segment sg;
-                sg.mat = up_base[ga] + R[ga] * (op.node_start[g] - op.node_start[ga]);
-                sg.vec = t_base[ga];
-                sg.len = (int32_t)R[ga];
-                sg.arena = 1;
-                segs.push_back(sg);
+                const int64_t slice = up_base[ga] + R[ga] * (op.node_start[g] - op.node_start[ga]);
+                Segment cur{0, 0, 0, -1};
+                for (int64_t e = rb_ptr[ga]; e < rb_ptr[ga + 1]; e++) {
+                    const int64_t b = rb_list[e];
+                    if (!width[b]) continue;
+                    const bool dn = op.flags[b] & 1;
+                    const int arena = dn ? 2 : 1;
+                    const int64_t vec = dn ? op.node_start[op.b_col[b]] : t_base[ga] + tcoff[b];
+                    if (cur.arena == arena && cur.vec + cur.len == vec) {
+                        cur.len += width[b];
+                        continue;
+                    }
+                    if (cur.arena >= 0) segs.push_back(cur);
+                    cur.mat = slice + (int64_t)coff[b] * nc;
+                    cur.vec = vec;
+                    cur.len = width[b];
+                    cur.arena = arena;
+                }
+                if (cur.arena >= 0) segs.push_back(cur);
             }
             const int64_t nseg = (int64_t)segs.size() - seg0;
             for (int64_t r0 = 0; r0 < nc; r0 += 64) {
@@ -363,6 +398,15 @@ void assemble(hodlr_op &op) {
                 pi.h = (int32_t)std::min<int64_t>(64, nc - r0);
                 pi.ld = (int32_t)nc;
                 pi.r0 = (int32_t)r0;
+                const int G = 64 / pi.h;
+                int64_t n4 = 0, ng = 0;
+                for (int64_t q = seg0; q < seg0 + nseg; q++) {
+                    n4 += (segs[q].len + 3) / 4;
+                    ng += (segs[q].len + G - 1) / G;
+                }
+                if (n4 > INT32_MAX || ng > INT32_MAX) throw ArgError{HODLR_E_ARG, "leaf panel too wide"};
+                pi.nq4 = (int32_t)n4;
+                pi.nqg = (int32_t)ng;
                 panel.push_back(pi);
             }
         }
@@ -371,7 +415,6 @@ void assemble(hodlr_op &op) {
 
     // projection tiles over the V panels; t index of every V column
     std::vector<int64_t> tidx(nvcol > 0 ? nvcol : 1, 0);
-    std::vector<TCopy> tcopy;
     std::vector<DenseBlock> dense;
     std::vector<FillDest> dest(nb > 0 ? nb : 1);
     op.t_lowrank = 0;
@@ -386,12 +429,6 @@ void assemble(hodlr_op &op) {
         fd.voff = voff[b];
         dest[b] = fd;
         if (op.flags[b] & 1) {
-            TCopy tc;
-            tc.toff = t_base[a] + coff[b];
-            tc.x0 = op.node_start[c];
-            tc.n = op.b_n[b];
-            tc.pad = 0;
-            tcopy.push_back(tc);
             DenseBlock db;
             db.row0 = op.node_start[a];
             db.col0 = op.node_start[c];
@@ -404,7 +441,7 @@ void assemble(hodlr_op &op) {
             db.pad = 0;
             dense.push_back(db);
         } else {
-            for (int k = 0; k < op.rank[b]; k++) tidx[vcol_base[c] + voff[b] + k] = t_base[a] + coff[b] + k;
+            for (int k = 0; k < op.rank[b]; k++) tidx[vcol_base[c] + voff[b] + k] = t_base[a] + tcoff[b] + k;
             op.t_lowrank += op.rank[b];
         }
     }
@@ -500,13 +537,11 @@ void assemble(hodlr_op &op) {
     }
     upload(op.proj, proj);
     upload(op.red, red);
-    upload(op.tcopy, tcopy);
     upload(op.panel, panel);
     upload(op.segs, segs);
     upload(op.tidx, tidx);
     op.n_proj = (int64_t)proj.size();
     op.n_red = (int64_t)red.size();
-    op.n_tcopy = (int64_t)tcopy.size();
     op.n_panel = (int64_t)panel.size();
     op.n_seg = (int64_t)segs.size();
     {  // per-position diagonal (a batch may mix diagonals) for the FP32 path
@@ -599,19 +634,77 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
     }
     if (prec == HODLR_PREC_F64) {
         HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
-                                        tcopy.as<TCopy>(), n_tcopy, panel.as<PanelItem>(), n_panel,
+                                        panel.as<PanelItem>(), n_panel,
                                         segs.as<Segment>(), VP64.as<double>(), NP64.as<double>(),
                                         UP64.as<double>(), static_cast<const double *>(x), t64.as<double>(),
-                                        part64.as<double>(), tidx.as<int64_t>(), static_cast<double *>(y), ev, s));
+                                        part64.as<double>(), tidx.as<int64_t>(), static_cast<double *>(y),
+                                        variant_for(prec, nrhs), ev, s));
     } else {
         if (!(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
         HIP_CHECK(launch_matvec<float>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
-                                       tcopy.as<TCopy>(), n_tcopy, panel.as<PanelItem>(), n_panel,
+                                       panel.as<PanelItem>(), n_panel,
                                        segs.as<Segment>(), VP32.as<float>(), NP32.as<float>(), UP32.as<float>(),
                                        static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
-                                       tidx.as<int64_t>(), static_cast<float *>(y), ev, s));
+                                       tidx.as<int64_t>(), static_cast<float *>(y), variant_for(prec, nrhs), ev,
+                                       s));
         if (add_diag)
             HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x),
                                           diag.as<double>(), N, nrhs, s));
+    }
+}
+
+int hodlr_op::variant_for(int prec, int nrhs) const {
+    int lg = 0;
+    while ((1 << lg) < nrhs && lg < 5) lg++;
+    return mv_variant[prec == HODLR_PREC_F64 ? 0 : 1][lg];
+}
+
+void hodlr_op::tune_matvec(int prec_mask, int reps) {
+    using namespace hodlr;
+    hipStream_t s = ctx->stream;
+    for (int pi = 0; pi < 2; pi++) {
+        const int prec = pi == 0 ? HODLR_PREC_F64 : HODLR_PREC_F32;
+        if (!(prec_mask & prec) || !(prec_flags & prec)) continue;
+        const size_t es = prec == HODLR_PREC_F32 ? sizeof(float) : sizeof(double);
+        for (int lg = 0; lg < 6; lg++) {
+            const int k = 1 << lg;
+            if (prec == HODLR_PREC_F64 && k > 16) continue;
+            DevBuf dx, dy;
+            dx.alloc(es * N * k);
+            dy.alloc(es * N * k);
+            HIP_CHECK(hipMemsetAsync(dx.p, 0, es * N * k, s));
+            double best[2] = {1e300, 1e300};
+            int pick[2] = {0, 0};
+            std::vector<int> cands = {0, kMatvecPanelFlat};
+            if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
+            for (int v : cands) {
+                mv_variant[pi][lg] = v;
+                hipEvent_t ev[3];
+                for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+                matvec_dev(prec, k, dx.p, dy.p, s);  // warm-up
+                double ph[2] = {0, 0};
+                for (int r = 0; r < reps; r++) {
+                    matvec_dev(prec, k, dx.p, dy.p, s, ev);
+                    HIP_CHECK(hipEventSynchronize(ev[2]));
+                    float a = 0, b = 0;
+                    HIP_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+                    HIP_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
+                    ph[0] += a;
+                    ph[1] += b;
+                }
+                for (auto &e : ev) (void)hipEventDestroy(e);
+                const int bits[2] = {kMatvecProjMfma, kMatvecPanelMfma | kMatvecPanelFlat};
+                for (int q = 0; q < 2; q++)
+                    if (ph[q] < best[q]) {
+                        best[q] = ph[q];
+                        pick[q] = v & bits[q];
+                    }
+            }
+            mv_variant[pi][lg] = pick[0] | pick[1];
+            if (verbose())
+                std::fprintf(stderr, "[hodlr] tune %s K=%d: proj %s %.3f ms, panel %s %.3f ms\n",
+                             pi == 0 ? "f64" : "f32", k, pick[0] ? "mfma" : "valu", best[0] / reps,
+                             pick[1] & kMatvecPanelMfma ? "mfma" : (pick[1] ? "flat" : "seg"), best[1] / reps);
+        }
     }
 }

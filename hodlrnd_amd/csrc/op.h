@@ -26,9 +26,10 @@ hipError_t launch_add_diag_f32(float *y, const float *x, const double *diag, int
 hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
-                         const TCopy *tc, int64_t ntc, const PanelItem *pa, int64_t npa,
+                         const PanelItem *pa, int64_t npa,
                          const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
-                         T *t, T *part, const int64_t *tidx, T *y, hipEvent_t *ev, hipStream_t s);
+                         T *t, T *part, const int64_t *tidx, T *y, int variant, hipEvent_t *ev,
+                         hipStream_t s);
 
 hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const Chunk *chunks, int nchunk,
                         double *partial, const int32_t *pchunk, int nprob, int K, double *out, hipStream_t s);
@@ -84,8 +85,8 @@ struct hodlr_op {
     hodlr::DevBuf NP64, UP64, VP64, NP32, UP32, VP32;
 
     // matvec work lists
-    int64_t n_proj = 0, n_red = 0, n_tcopy = 0, n_panel = 0, n_seg = 0;
-    hodlr::DevBuf proj, red, tcopy, panel, segs, tidx;
+    int64_t n_proj = 0, n_red = 0, n_panel = 0, n_seg = 0;
+    hodlr::DevBuf proj, red, panel, segs, tidx;
     double e_near = 0, e_far = 0, e_v = 0, e_u = 0, t_lowrank = 0;
 
     // per-precision work buffers (grown on demand)
@@ -113,6 +114,13 @@ struct hodlr_op {
     void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr,
                     bool add_diag = true);
     void ensure_work(int prec, int nrhs);
+
+    // kernel form per (precision, RHS count): kMatvec* bits, indexed
+    // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller
+    int mv_variant[2][6] = {{0, 0, 2, 2, 2, 3}, {0, 0, 2, 2, 2, 3}};
+    int variant_for(int prec, int nrhs) const;
+    // time every kernel form per phase on this operator and keep the fastest
+    void tune_matvec(int prec_mask, int reps);
 };
 
 namespace hodlr {

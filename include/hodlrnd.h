@@ -205,6 +205,22 @@ int hodlr_gmres_batch_device(hodlr_op *op, int nrhs, const double *b, double *x,
 int hodlr_gmres_batch(hodlr_op *op, int nrhs, const double *b, double *x,
                       const hodlr_gmres_params *params, hodlr_gmres_info *infos);
 
+/* ---- kernel selection (no reference counterpart) ----
+ * Each matvec pass has several kernel forms.  variant bits: 1 = matrix-core
+ * (MFMA) projection V^T x, 2 = MFMA panel product (both only with 4 or more
+ * RHS), 4 = vector-ALU panel product walking each leaf's segments as one
+ * flattened column list (else one segment at a time; suits many short
+ * segments).  tune times the forms of each pass on this operator for
+ * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
+ * results agree up to rounding order.  get/set read or force the choice for
+ * one (prec, nrhs). */
+#define HODLR_MV_PROJ_MFMA 1
+#define HODLR_MV_PANEL_MFMA 2
+#define HODLR_MV_PANEL_FLAT 4
+int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps);
+int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant);
+int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);
+
 /* ---- measurement ---- */
 /* Times `reps` back-to-back matvecs on the op's stream with HIP events per
  * kernel; ms[0] = whole matvec, ms[1] = projection kernel (V^T x),

@@ -206,3 +206,39 @@ def test_cpp_api_reference_driver():
     mv = float(re.search(r"Relative Error in hmatrix matvec \.\.\. (\S+)", r.stdout).group(1))
     sol = float(re.search(r"Relative Error in GMRES solution \.\.\. (\S+)", r.stdout).group(1))
     assert mv < 1e-8 and sol < 1e-8
+
+
+@pytest.mark.parametrize("nd,n,nmax", [(3, 6000, 64), (2, 5000, 32)])
+def test_matvec_kernel_forms_agree(nd, n, nmax):
+    """Vector-ALU and matrix-core (MFMA) forms of both matvec phases give the
+    same product for 4..32 right-hand sides: FP64 to 1e-14, FP32 to 1e-6
+    relative to each other, and every column equals the 1-RHS product."""
+    say("case kernel_forms", nd, n)
+    pts, fam, sc, dg, w = problem(nd, n, 300 + n)
+    A, ref, ok = build_pair(pts, fam, sc, dg, w, nmax, 0, 1e-8)
+    op = A.op
+    X = np.stack([O.splitmix_uniform(50 + c, n) for c in range(32)], axis=1)
+    y1 = np.stack([op.matvec(X[:, c]) for c in range(4)], axis=1)
+    for prec, ks, tol in ((H.PREC_F64, (4, 8, 16), 1e-14), (H.PREC_F32, (4, 8, 16, 32), 1e-6)):
+        for k in ks:
+            out = {}
+            for v in (0, H.MV_PROJ_MFMA, H.MV_PANEL_MFMA, H.MV_PROJ_MFMA | H.MV_PANEL_MFMA, H.MV_PANEL_FLAT):
+                op.set_variant(prec, k, v)
+                out[v] = op.matvec(X[:, :k], prec=prec)
+            for v in (1, 2, 3, 4):
+                e = relerr(out[v], out[0])
+                assert e < tol, (prec, k, v, e)
+            e1 = relerr(out[3][:, :4], y1)
+            assert e1 < (1e-14 if prec == H.PREC_F64 else 1e-5), (prec, k, e1)
+            say(" prec", prec, "K", k, "mfma vs valu", relerr(out[3], out[0]))
+    for prec in (H.PREC_F64, H.PREC_F32):  # flattened walk at 1 and 2 RHS
+        for k in (1, 2):
+            op.set_variant(prec, k, 0)
+            a = op.matvec(X[:, :k], prec=prec)
+            op.set_variant(prec, k, H.MV_PANEL_FLAT)
+            b = op.matvec(X[:, :k], prec=prec)
+            assert relerr(b, a) < (1e-14 if prec == H.PREC_F64 else 1e-6)
+    op.tune(reps=1)
+    for prec, ks in ((H.PREC_F64, (1, 2, 4, 8, 16)), (H.PREC_F32, (1, 2, 4, 8, 16, 32))):
+        for k in ks:
+            assert op.get_variant(prec, k) in range(8)
