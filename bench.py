@@ -323,6 +323,108 @@ def bench_batch(args, H):
     print(json.dumps(out))
 
 
+def bench_serve(args, H):
+    """Serving stream: Poisson arrivals of solve requests, NDIM = 3, N
+    log-uniform in [1000, 200000], points uniform in [-1,1]^3, A = I + (1/N)/r
+    (A_ii = 1), eps_ACA 1e-8, Nmax 64, d' 0 (Nmax and d' are not fixed by the
+    workload; these are the library defaults).  30% of requests bring new
+    points (build + solve), 70% reuse an operator built earlier in the stream
+    (uniformly), each with 1..32 right-hand sides.  Every column is solved to
+    a FP64-certified relative residual of 1e-8."""
+    from hodlrnd_amd import workloads as Wl
+    nreq, rate = args.requests, args.rate
+    arr = Wl.poisson_arrivals(rate, nreq, seed=5) if rate > 0 else np.zeros(nreq)
+    u = (Wl.splitmix_uniform(55, 4 * nreq).reshape(nreq, 4) + 1.0) / 2.0
+    plan, nbuild = [], 0
+    for q in range(nreq):
+        build = q == 0 or u[q, 0] < 0.3
+        n = Wl.log_uniform_int(2 * u[q, 1] - 1, 1000, 200000) if build else None
+        nrhs = 1 + min(31, int(u[q, 2] * 32))
+        plan.append((build, n, nrhs, u[q, 3]))
+        nbuild += build
+    srv = H.Server(slots=args.slots, tol=args.tol, eps_aca=args.eps, nmax=args.nmax, dprime=0,
+                   hbm_budget=args.hbm_budget * 1e9 if args.hbm_budget > 0 else 0.0)
+    # warm the kernels once (first launches load code objects)
+    w = srv.submit_build(Wl.problem_points(999, 2000, 3), H.Kernel(H.KERNEL_INV_R, 3, 1.0 / 2000, 1.0))
+    srv.wait(srv.submit_solve(w, Wl.splitmix_uniform(998, 2000)), timeout=120)
+    log("serve: %d requests at %s req/s, %d builds" % (nreq, rate if rate > 0 else "inf (all at t=0)", nbuild))
+    ops, nof, rids = [], {}, []
+    t0 = time.perf_counter()
+    for q, (build, n, nrhs, pick) in enumerate(plan):
+        dt = arr[q] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        if build:
+            pts = Wl.problem_points(5000 + q, n, 3)
+            oid = srv.submit_build(pts, H.Kernel(H.KERNEL_INV_R, 3, 1.0 / n, 1.0))
+            ops.append(oid)
+            nof[oid] = n
+        else:
+            oid = ops[min(len(ops) - 1, int(pick * len(ops)))]
+            n = nof[oid]
+        B = np.stack([Wl.splitmix_uniform(100000 + 64 * q + c, n) for c in range(nrhs)], axis=1)
+        rids.append((srv.submit_solve(oid, B), build, nrhs, n))
+        if (q + 1) % 50 == 0:
+            log("submitted %d/%d" % (q + 1, nreq))
+    t_sub = time.perf_counter() - t0
+    lat, lat_b, lat_r, cols, worst, okc = [], [], [], 0, 0.0, True
+    t_first, t_last = None, 0.0
+    for k, (rid, build, nrhs, n) in enumerate(rids):
+        _, inf = srv.wait(rid, timeout=1200, want_x=False)
+        if inf["rc"] != 0:
+            raise RuntimeError("request %d failed: %s" % (rid, inf))
+        L = inf["t_done"] - inf["t_submit"]
+        lat.append(L)
+        (lat_b if build else lat_r).append(L)
+        cols += nrhs
+        worst = max(worst, inf["max_resid"])
+        okc = okc and inf["converged"] == 1
+        t_first = inf["t_submit"] if t_first is None else min(t_first, inf["t_submit"])
+        t_last = max(t_last, inf["t_done"])
+        if (k + 1) % 50 == 0:
+            log("done %d/%d" % (k + 1, nreq))
+    st = srv.stats()
+    wall = t_last - t_first
+    srv.close()
+    lat = np.array(lat)
+    out = {
+        "metric": METRIC,
+        "value": cols / wall,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": nreq,
+        "warmup": 1,
+        "ms_per_step": float(np.mean(lat) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 matvec + fp64 refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": "serving stream: %d requests, Poisson %s req/s, 3D, N log-uniform [1000, 200000], "
+                        "A=I+(1/N)/r, eps_ACA=%g, Nmax=%d, d'=0, 30%% new operators / 70%% reuse, 1-32 RHS, "
+                        "tol %g; a step is one request" % (nreq, rate if rate > 0 else "inf", args.eps, args.nmax,
+                                                            args.tol),
+            "requests": nreq, "rate": rate, "slots": args.slots,
+        },
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "p50_ms_reuse": float(np.percentile(lat_r, 50) * 1e3) if lat_r else None,
+        "p99_ms_reuse": float(np.percentile(lat_r, 99) * 1e3) if lat_r else None,
+        "p50_ms_build": float(np.percentile(lat_b, 50) * 1e3) if lat_b else None,
+        "p99_ms_build": float(np.percentile(lat_b, 99) * 1e3) if lat_b else None,
+        "requests_per_s": nreq / wall,
+        "columns": cols,
+        "wall_s": wall,
+        "submit_s": t_sub,
+        "all_converged": bool(okc),
+        "max_resid": worst,
+        "server": {k: st[k] for k in ("builds", "evictions", "ticks", "peak_bytes", "budget_bytes",
+                                      "build_seconds")},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,9 +436,13 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["grid100", "batch256"], default="grid100",
+    ap.add_argument("--workload", choices=["grid100", "batch256", "serve"], default="grid100",
                     help="grid100: one N=1e6 solve per step; batch256: 256 uneven problems x 8 RHS "
-                         "solved as one batch per step")
+                         "solved as one batch per step; serve: a Poisson stream of solve requests")
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second (0: all at once)")
+    ap.add_argument("--slots", type=int, default=16, help="serve: solve columns per operator engine")
+    ap.add_argument("--hbm-budget", type=float, default=0.0, help="serve: GB for operators (0: 90%% of free)")
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--plain-all", action="store_true",
                     help="batch256: time the plain FP64 path on every problem, not 64 spread ones")
@@ -357,10 +463,10 @@ def main():
                           "bytes_written_per_launch": 1 << 30, "launches": 4, "gbps": g}))
         return
 
-    if args.workload == "batch256":
+    if args.workload in ("batch256", "serve"):
         if args.nmax == 128:
             args.nmax = 64
-        bench_batch(args, H)
+        (bench_batch if args.workload == "batch256" else bench_serve)(args, H)
         return
 
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)

@@ -92,6 +92,14 @@ SIGNATURES = {
     "hodlr_op_tune_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "hodlr_op_set_matvec_variant": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hodlr_op_get_matvec_variant": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hodlr_server_create": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p]),
+    "hodlr_server_destroy": (None, [C.c_void_p]),
+    "hodlr_server_submit_build": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]),
+    "hodlr_server_submit_solve": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p]),
+    "hodlr_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_void_p, C.c_void_p]),
+    "hodlr_server_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hodlr_server_last_error": (C.c_char_p, [C.c_void_p]),
     "hodlr_op_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_op_block_ranks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "hodlr_op_destroy": (None, [C.c_void_p]),
@@ -134,6 +142,26 @@ def _p(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_double), ("slots", C.c_int32), ("restart", C.c_int32),
+                ("max_outer", C.c_int32), ("tol", C.c_double), ("inner_floor", C.c_double),
+                ("eps_aca", C.c_double), ("nmax", C.c_int32), ("dprime", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("op_id", C.c_int64), ("nrhs", C.c_int32), ("status", C.c_int32), ("converged", C.c_int32),
+                ("iterations", C.c_int32), ("outer", C.c_int32), ("built", C.c_int32),
+                ("max_resid", C.c_double), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("builds", C.c_int64), ("evictions", C.c_int64), ("requests_done", C.c_int64),
+                ("columns_done", C.c_int64), ("ticks", C.c_int64), ("resident_ops", C.c_int64),
+                ("used_bytes", C.c_double), ("peak_bytes", C.c_double), ("budget_bytes", C.c_double),
+                ("build_seconds", C.c_double)]
 
 
 class Context:
@@ -452,3 +480,69 @@ class HODLRdD_matrix:
         print("Maximum rank across the Tree : %d" % s["max_rank"])
         print("Time to Initialize (in s) : %g" % s["assemble_seconds"])
         return s
+
+
+class Server:
+    """Continuous-batching solve server (hodlr_server_*): operators built by a
+    builder thread with HBM-footprint admission, solve columns batched per
+    operator as FP32-inner / FP64-refined GMRES instances."""
+
+    def __init__(self, device=0, hbm_budget=0.0, slots=8, restart=100, max_outer=20, tol=1e-8,
+                 inner_floor=0.0, eps_aca=1e-8, nmax=64, dprime=0):
+        self.cfg = ServerConfig(hbm_budget, slots, restart, max_outer, tol, inner_floor, eps_aca, nmax, dprime)
+        self.h = C.c_void_p()
+        rc = lib.hodlr_server_create(device, C.byref(self.cfg), C.byref(self.h))
+        if rc != OK:
+            raise HodlrError(rc, "server creation failed")
+        self.n_of = {}
+
+    def _check(self, rc, what):
+        if rc != OK:
+            raise HodlrError(rc, "%s: %s" % (what, lib.hodlr_server_last_error(self.h).decode()))
+
+    def submit_build(self, points, kernel, lo=None, hi=None):
+        pts = _f64(points)
+        n, nd = pts.shape
+        lo = -np.ones(nd) if lo is None else _f64(lo)
+        hi = np.ones(nd) if hi is None else _f64(hi)
+        kd = kernel.desc()
+        oid = C.c_int64()
+        self._check(lib.hodlr_server_submit_build(self.h, nd, n, _p(pts), _p(lo), _p(hi), C.byref(kd),
+                                                  C.byref(oid)), "submit_build")
+        self.n_of[oid.value] = n
+        return oid.value
+
+    def submit_solve(self, op_id, B):
+        if op_id not in self.n_of:
+            raise HodlrError(E_ARG, "submit_solve: unknown operator id %r" % (op_id,))
+        n = self.n_of[op_id]
+        B = np.asfortranarray(np.asarray(B, dtype=np.float64).reshape(n, -1))
+        rid = C.c_int64()
+        self._check(lib.hodlr_server_submit_solve(self.h, op_id, B.shape[1], _p(B), C.byref(rid)),
+                    "submit_solve")
+        self._pending = getattr(self, "_pending", {})
+        self._pending[rid.value] = (n, B.shape[1])
+        return rid.value
+
+    def wait(self, req_id, timeout=-1.0, want_x=True):
+        n, k = self._pending[req_id]
+        X = np.empty((n, k), order="F") if want_x else None
+        info = RequestInfo()
+        rc = lib.hodlr_server_wait(self.h, req_id, timeout, _p(X) if want_x else None, C.byref(info))
+        out = {f: getattr(info, f) for f, _ in RequestInfo._fields_}
+        out["rc"] = rc
+        return X, out
+
+    def stats(self):
+        st = ServerStats()
+        self._check(lib.hodlr_server_get_stats(self.h, C.byref(st)), "stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib.hodlr_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+

@@ -336,4 +336,73 @@ hipError_t launch_bscale(const double *src, const double *coef, double *dst, con
     return hipGetLastError();
 }
 
+
+// ---- per-column kernels of the serving engine ----
+// Every column of an n x K interleaved block is an independent solve at its
+// own Krylov step; these move single columns between blocks.
+
+// out[r, c] = src[c][r, c] (zero where src[c] is null)
+__global__ __launch_bounds__(256) void gather_cols_kernel(const double *const *__restrict__ src,
+                                                          double *__restrict__ out, int64_t n, int K) {
+    const int64_t tot = n * K;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(e % K);
+        const double *p = src[c];
+        out[e] = p ? p[e] : 0.0;
+    }
+}
+
+// dst[c][r, c] = w[r, c] * scale[c] where dst[c] is not null
+__global__ __launch_bounds__(256) void scatter_cols_kernel(const double *__restrict__ w,
+                                                           const double *__restrict__ scale,
+                                                           double *const *__restrict__ dst, int64_t n, int K) {
+    const int64_t tot = n * K;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(e % K);
+        double *p = dst[c];
+        if (p) p[e] = w[e] * scale[c];
+    }
+}
+
+// cf[v * K + c] = 0 for v >= lim[c]
+__global__ __launch_bounds__(256) void mask_coef_kernel(double *__restrict__ cf, int nv, int K,
+                                                        const int32_t *__restrict__ lim) {
+    for (int e = threadIdx.x; e < nv * K; e += blockDim.x)
+        if (e / K >= lim[e % K]) cf[e] = 0.0;
+}
+
+// column c of an interleaved block to / from a contiguous vector
+__global__ __launch_bounds__(256) void put_col_kernel(double *__restrict__ dst, const double *__restrict__ src,
+                                                      int64_t n, int K, int c) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        dst[r * K + c] = src ? src[r] : 0.0;
+}
+__global__ __launch_bounds__(256) void get_col_kernel(const double *__restrict__ src, double *__restrict__ dst,
+                                                      int64_t n, int K, int c) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        dst[r] = src[r * K + c];
+}
+
+hipError_t launch_gather_cols(const double *const *src, double *out, int64_t n, int K, hipStream_t s) {
+    gather_cols_kernel<<<dim3(grid_for(n * K)), 256, 0, s>>>(src, out, n, K);
+    return hipGetLastError();
+}
+hipError_t launch_scatter_cols(const double *w, const double *scale, double *const *dst, int64_t n, int K,
+                               hipStream_t s) {
+    scatter_cols_kernel<<<dim3(grid_for(n * K)), 256, 0, s>>>(w, scale, dst, n, K);
+    return hipGetLastError();
+}
+hipError_t launch_mask_coef(double *cf, int nv, int K, const int32_t *lim, hipStream_t s) {
+    mask_coef_kernel<<<dim3(1), 256, 0, s>>>(cf, nv, K, lim);
+    return hipGetLastError();
+}
+hipError_t launch_put_col(double *dst, const double *src, int64_t n, int K, int c, hipStream_t s) {
+    put_col_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(dst, src, n, K, c);
+    return hipGetLastError();
+}
+hipError_t launch_get_col(const double *src, double *dst, int64_t n, int K, int c, hipStream_t s) {
+    get_col_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(src, dst, n, K, c);
+    return hipGetLastError();
+}
+
 }  // namespace hodlr

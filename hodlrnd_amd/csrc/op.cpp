@@ -117,6 +117,12 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
 
 void assemble(hodlr_op &op) {
     const double t_start = now_s();
+    double t_mark = t_start;
+    auto mark = [&](const char *phase) {  // HODLR_VERBOSE phase timing
+        const double t = now_s();
+        if (verbose()) std::fprintf(stderr, "[hodlr] %-12s %8.3f s\n", phase, t - t_mark);
+        t_mark = t;
+    };
     hipStream_t s = op.ctx->stream;
     const int P = (int)op.trees.size();
     op.tree = op.trees[0];
@@ -203,6 +209,7 @@ void assemble(hodlr_op &op) {
         upload(op.ord, ord);
     }
 
+    mark("inputs");
     // ---- 1. blocks in reference order, problem by problem ----
     for (int p = 0; p < P; p++) {
         const hodlr_tree &t = *op.trees[p];
@@ -227,6 +234,7 @@ void assemble(hodlr_op &op) {
     op.nblk = (int64_t)op.b_level.size();
     const int64_t nb = op.nblk;
 
+    mark("block list");
     // ---- 2. rank pass ----
     const double t_rank0 = now_s();
     DevBuf d_rank, d_flags, pool;
@@ -267,6 +275,7 @@ void assemble(hodlr_op &op) {
     const double t_rank1 = now_s();
     if (verbose()) std::fprintf(stderr, "[hodlr] rank pass %.3f s, %lld blocks\n", t_rank1 - t_rank0, (long long)nb);
 
+    mark("rank pass");
     // ---- 3. layout ----
     // U panel of row node a: R[a] columns, its blocks in block order (a
     // dense-fallback block contributes its n columns of K); t of row node a:
@@ -487,6 +496,13 @@ void assemble(hodlr_op &op) {
     if (nvcol > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many V columns for 32-bit tile offsets"};
     op.n_slots = slots;
 
+    mark("layout");
+    if (op.admit) {
+        double bytes = 8.0 * (double)(op.n_np + op.n_up + op.n_vp);
+        if (op.prec_flags & HODLR_PREC_F32) bytes *= 1.5;
+        op.admit(bytes);
+        mark("admission");
+    }
     // ---- 4. allocate and fill ----
     const double t_fill0 = now_s();
     if (verbose())
@@ -553,6 +569,7 @@ void assemble(hodlr_op &op) {
         for (int p = 1; p < P; p++) op.uniform_diag = op.uniform_diag && op.kds[p].diag == op.kds[0].diag;
     }
     HIP_CHECK(hipStreamSynchronize(s));
+    mark("fill");
     const double t_end = now_s();
 
     // ---- statistics (HDD_node.hpp:183-220, HDD_matrix.hpp:69-85) ----

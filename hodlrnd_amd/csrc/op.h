@@ -38,6 +38,13 @@ hipError_t launch_bgemv(const double *const *V, int nv, const double *coef, doub
 hipError_t launch_bscale(const double *src, const double *coef, double *dst, const Chunk *chunks, int nchunk, int K,
                          hipStream_t s);
 
+hipError_t launch_gather_cols(const double *const *src, double *out, int64_t n, int K, hipStream_t s);
+hipError_t launch_scatter_cols(const double *w, const double *scale, double *const *dst, int64_t n, int K,
+                               hipStream_t s);
+hipError_t launch_mask_coef(double *cf, int nv, int K, const int32_t *lim, hipStream_t s);
+hipError_t launch_put_col(double *dst, const double *src, int64_t n, int K, int c, hipStream_t s);
+hipError_t launch_get_col(const double *src, double *dst, int64_t n, int K, int c, hipStream_t s);
+
 int64_t mdot_chunks(int64_t n);
 hipError_t launch_mdot(const double *V, int64_t ldv, int nv, const double *w, int64_t n, double *partial,
                        double *out, int accumulate, hipStream_t s);
@@ -114,6 +121,11 @@ struct hodlr_op {
     void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr,
                     bool add_diag = true);
     void ensure_work(int prec, int nrhs);
+
+    // called by assemble once the exact arena footprint (bytes, all
+    // requested precisions) is known and before it is allocated; may block
+    // until memory is available or throw to abandon the build
+    std::function<void(double)> admit;
 
     // kernel form per (precision, RHS count): kMatvec* bits, indexed
     // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller

@@ -251,6 +251,69 @@ int hodlr_copy_to_host(hodlr_ctx *ctx, void *dst, const void *src, int64_t bytes
 int hodlr_to_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst);
 int hodlr_from_tree_order(hodlr_tree *t, int nrhs, const double *src, double *dst);
 
+/* ---- solve server (no reference counterpart) ----
+ * A continuous-batching request loop on one GPU.  Operators are built by a
+ * builder thread (tree + assembly) and admitted by their exact HBM footprint
+ * (known after the ACA rank pass, before any arena is allocated); idle
+ * resident operators are evicted least-recently-used when memory is needed
+ * and rebuilt from their kept points when asked for again.  Solve requests
+ * are split into columns; every column is an independent FP32-inner /
+ * FP64-refined GMRES instance (restarted GMRES cycles of the reference's
+ * semantics, GMRES.hpp:160-303, on the FP32 operator, each followed by an
+ * FP64 residual) that takes a free slot of its operator's engine as soon as
+ * one opens, at any iteration of the others, so one matvec per iteration
+ * serves every column of an operator.  Engines of different operators run
+ * on their own streams and overlap on the device.  A column's arithmetic
+ * does not depend on the other columns: a result is the same alone and in
+ * any mix. */
+typedef struct hodlr_server hodlr_server;
+
+typedef struct {
+    double hbm_budget;     /* bytes for operators and solver space; <= 0: 90% of free memory */
+    int32_t slots;         /* solve columns per operator engine: 1, 2, 4, 8, 16 or 32 */
+    int32_t restart;       /* inner GMRES cycle length (<= 127) */
+    int32_t max_outer;     /* FP64 refinement steps per column */
+    double tol;            /* relative residual target, FP64-certified */
+    double inner_floor;    /* tightest relative tolerance asked of one inner cycle */
+    double eps_aca;
+    int32_t nmax;
+    int32_t dprime;
+} hodlr_server_config;
+
+typedef struct {
+    int64_t op_id;
+    int32_t nrhs;
+    int32_t status;        /* 0 queued, 1 running, 2 done, < 0 a HODLR_E_* code */
+    int32_t converged;     /* every column reached tol */
+    int32_t iterations;    /* largest inner GMRES iteration count of the columns */
+    int32_t outer;         /* largest FP64 refinement count of the columns */
+    int32_t built;         /* the request waited for a build of its operator */
+    double max_resid;      /* largest FP64 relative residual of the columns */
+    double t_submit, t_start, t_done;  /* server clock, seconds */
+} hodlr_request_info;
+
+typedef struct {
+    int64_t builds, evictions, requests_done, columns_done, ticks;
+    int64_t resident_ops;
+    double used_bytes, peak_bytes, budget_bytes;
+    double build_seconds;  /* summed assembly wall time */
+} hodlr_server_stats;
+
+int hodlr_server_create(int device, const hodlr_server_config *cfg, hodlr_server **out);
+/* stops the threads after the queued work is finished or failed */
+void hodlr_server_destroy(hodlr_server *srv);
+/* queue a new operator: points n x ndim (point order, copied), bounding box,
+ * kernel (weights copied); the id is valid at once */
+int hodlr_server_submit_build(hodlr_server *srv, int ndim, int64_t n, const double *pts, const double *lo,
+                              const double *hi, const hodlr_kernel_desc *k, int64_t *op_id);
+/* queue nrhs right-hand sides (n x nrhs column-major, point order, copied) */
+int hodlr_server_submit_solve(hodlr_server *srv, int64_t op_id, int nrhs, const double *b, int64_t *req_id);
+/* wait up to timeout_s (< 0: forever) for a request; x (n x nrhs, point
+ * order) may be NULL.  Returns HODLR_E_STATE on time-out. */
+int hodlr_server_wait(hodlr_server *srv, int64_t req_id, double timeout_s, double *x, hodlr_request_info *info);
+int hodlr_server_get_stats(hodlr_server *srv, hodlr_server_stats *st);
+const char *hodlr_server_last_error(hodlr_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -153,3 +153,18 @@ def test_cpp_api_example_compiles(tmp_path):
                         "-L" + os.path.join(ROOT, "hodlrnd_amd"), "-lhodlrnd"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+def test_server_config_validation():
+    """hodlr_server_create checks its configuration before touching a GPU."""
+    import ctypes as C
+    bad = [dict(slots=3), dict(restart=0), dict(restart=200), dict(tol=0.0), dict(eps_aca=1.5), dict(nmax=0),
+           dict(max_outer=0)]
+    for b in bad:
+        kw = dict(hbm_budget=0.0, slots=8, restart=100, max_outer=20, tol=1e-8, inner_floor=0.0, eps_aca=1e-8,
+                  nmax=64, dprime=0)
+        kw.update(b)
+        cfg = H.ServerConfig(**kw)
+        h = C.c_void_p()
+        assert H.lib.hodlr_server_create(0, C.byref(cfg), C.byref(h)) == H.E_ARG, b
+    assert H.lib.hodlr_server_create(-1, C.byref(H.ServerConfig()), None) == H.E_ARG
