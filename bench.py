@@ -343,11 +343,20 @@ def bench_serve(args, H):
         plan.append((build, n, nrhs, u[q, 3]))
         nbuild += build
     srv = H.Server(slots=args.slots, tol=args.tol, eps_aca=args.eps, nmax=args.nmax, dprime=0,
-                   hbm_budget=args.hbm_budget * 1e9 if args.hbm_budget > 0 else 0.0)
+                   hbm_budget=args.hbm_budget * 1e9 if args.hbm_budget > 0 else 0.0, builders=args.builders)
     # warm the kernels once (first launches load code objects)
     w = srv.submit_build(Wl.problem_points(999, 2000, 3), H.Kernel(H.KERNEL_INV_R, 3, 1.0 / 2000, 1.0))
     srv.wait(srv.submit_solve(w, Wl.splitmix_uniform(998, 2000)), timeout=120)
     log("serve: %d requests at %s req/s, %d builds" % (nreq, rate if rate > 0 else "inf (all at t=0)", nbuild))
+    try:
+        out = serve_stream(srv, plan, arr, args, H, Wl)
+    finally:
+        srv.close()
+    print(json.dumps(out))
+
+
+def serve_stream(srv, plan, arr, args, H, Wl):
+    nreq, rate = args.requests, args.rate
     ops, nof, rids = [], {}, []
     t0 = time.perf_counter()
     for q, (build, n, nrhs, pick) in enumerate(plan):
@@ -385,7 +394,6 @@ def bench_serve(args, H):
             log("done %d/%d" % (k + 1, nreq))
     st = srv.stats()
     wall = t_last - t_first
-    srv.close()
     lat = np.array(lat)
     out = {
         "metric": METRIC,
@@ -405,7 +413,7 @@ def bench_serve(args, H):
                         "A=I+(1/N)/r, eps_ACA=%g, Nmax=%d, d'=0, 30%% new operators / 70%% reuse, 1-32 RHS, "
                         "tol %g; a step is one request" % (nreq, rate if rate > 0 else "inf", args.eps, args.nmax,
                                                             args.tol),
-            "requests": nreq, "rate": rate, "slots": args.slots,
+            "requests": nreq, "rate": rate, "slots": args.slots, "builders": args.builders,
         },
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -422,7 +430,7 @@ def bench_serve(args, H):
         "server": {k: st[k] for k in ("builds", "evictions", "ticks", "peak_bytes", "budget_bytes",
                                       "build_seconds")},
     }
-    print(json.dumps(out))
+    return out
 
 
 def main():
@@ -443,6 +451,7 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second (0: all at once)")
     ap.add_argument("--slots", type=int, default=16, help="serve: solve columns per operator engine")
     ap.add_argument("--hbm-budget", type=float, default=0.0, help="serve: GB for operators (0: 90%% of free)")
+    ap.add_argument("--builders", type=int, default=4, help="serve: concurrent operator builds")
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--plain-all", action="store_true",
                     help="batch256: time the plain FP64 path on every problem, not 64 spread ones")

@@ -147,7 +147,7 @@ def _f64(a):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_double), ("slots", C.c_int32), ("restart", C.c_int32),
                 ("max_outer", C.c_int32), ("tol", C.c_double), ("inner_floor", C.c_double),
-                ("eps_aca", C.c_double), ("nmax", C.c_int32), ("dprime", C.c_int32)]
+                ("eps_aca", C.c_double), ("nmax", C.c_int32), ("dprime", C.c_int32), ("builders", C.c_int32)]
 
 
 class RequestInfo(C.Structure):
@@ -488,8 +488,9 @@ class Server:
     operator as FP32-inner / FP64-refined GMRES instances."""
 
     def __init__(self, device=0, hbm_budget=0.0, slots=8, restart=100, max_outer=20, tol=1e-8,
-                 inner_floor=0.0, eps_aca=1e-8, nmax=64, dprime=0):
-        self.cfg = ServerConfig(hbm_budget, slots, restart, max_outer, tol, inner_floor, eps_aca, nmax, dprime)
+                 inner_floor=0.0, eps_aca=1e-8, nmax=64, dprime=0, builders=2):
+        self.cfg = ServerConfig(hbm_budget, slots, restart, max_outer, tol, inner_floor, eps_aca, nmax, dprime,
+                                builders)
         self.h = C.c_void_p()
         rc = lib.hodlr_server_create(device, C.byref(self.cfg), C.byref(self.h))
         if rc != OK:

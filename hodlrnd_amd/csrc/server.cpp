@@ -128,7 +128,12 @@ struct Engine {
         upload(chunks, ch);
         upload(pchunk, pc);
         const size_t nk = (size_t)n * K;
-        for (DevBuf *d : {&b, &x, &rr, &w64, &in, &w}) d->alloc(sizeof(double) * nk);
+        // blocks start zeroed: a masked (zero) coefficient times a column
+        // that was never written must give zero, not NaN from stale memory
+        for (DevBuf *d : {&b, &x, &rr, &w64, &in, &w}) {
+            d->alloc(sizeof(double) * nk);
+            HIP_CHECK(hipMemsetAsync(d->p, 0, sizeof(double) * nk, s));
+        }
         x32.alloc(sizeof(float) * nk);
         y32.alloc(sizeof(float) * nk);
         partial.alloc(sizeof(double) * (size_t)std::max(nchunk, 1) * (m + 1) * K);
@@ -163,6 +168,7 @@ struct Engine {
     double *basis(int i) {
         if (!V[i].p) {
             V[i].alloc(sizeof(double) * (size_t)n * K);
+            HIP_CHECK(hipMemsetAsync(V[i].p, 0, sizeof(double) * (size_t)n * K, s));
             vptr[i] = V[i].as<double>();
             bytes += 8.0 * (double)n * K;
             HIP_CHECK(hipMemcpyAsync(vtab.p, vptr.data(), sizeof(double *) * vptr.size(), hipMemcpyHostToDevice, s));
@@ -464,21 +470,22 @@ struct OpEntry {
 struct hodlr_server {
     hodlr_server_config cfg{};
     int device = 0;
-    hodlr_ctx bctx;  // builder's device and stream
+    std::vector<std::unique_ptr<hodlr_ctx>> bctx;  // one device + stream per builder
     std::mutex mu;
     std::condition_variable cv_build, cv_work, cv_done, cv_mem;
     std::map<int64_t, std::unique_ptr<hodlr::OpEntry>> ops;
     std::map<int64_t, std::shared_ptr<hodlr::Request>> reqs;
     int64_t next_op = 1, next_req = 1;
     bool stop = false;
-    std::thread builder, scheduler;
+    std::vector<std::thread> builders;
+    std::thread scheduler;
     hodlr::Clock::time_point t0 = hodlr::Clock::now();
     hodlr_server_stats st{};
     double used = 0;  // admitted operator bytes + engine bytes
     std::string last_error;
 
     double now() const { return std::chrono::duration<double>(hodlr::Clock::now() - t0).count(); }
-    void build_loop();
+    void build_loop(int idx);
     void sched_loop();
     void admit_memory(hodlr::OpEntry &e, double bytes);
     bool evict_one(int64_t keep);
@@ -555,8 +562,9 @@ void hodlr_server::fail_op(OpEntry &e, int code, const std::string &msg) {
     cv_done.notify_all();
 }
 
-void hodlr_server::build_loop() {
+void hodlr_server::build_loop(int idx) {
     (void)hipSetDevice(device);
+    hodlr_ctx &bc = *bctx[idx];
     while (true) {
         OpEntry *e = nullptr;
         {
@@ -578,16 +586,16 @@ void hodlr_server::build_loop() {
         const double tb = now();
         hodlr_tree *t = nullptr;
         hodlr_op *op = nullptr;
-        int rc = hodlr_tree_build(&bctx, e->ndim, e->n, e->pts.data(), e->lo.data(), e->hi.data(), cfg.nmax,
+        int rc = hodlr_tree_build(&bc, e->ndim, e->n, e->pts.data(), e->lo.data(), e->hi.data(), cfg.nmax,
                                   cfg.dprime, &t);
-        std::string err = rc ? bctx.last_error : "";
+        std::string err = rc ? bc.last_error : "";
         if (rc == HODLR_OK) {
             hodlr_kernel_desc kd = e->kd;
             kd.weight = e->weight.empty() ? nullptr : e->weight.data();
             try {
                 (void)hipSetDevice(device);
                 op = new hodlr_op;
-                op->ctx = &bctx;
+                op->ctx = &bc;
                 op->trees.assign(1, t);
                 op->kds.assign(1, kd);
                 op->eps = cfg.eps_aca;
@@ -809,19 +817,22 @@ int hodlr_server_create(int device, const hodlr_server_config *cfg, hodlr_server
     const int sl = cfg->slots;
     if (sl != 1 && sl != 2 && sl != 4 && sl != 8 && sl != 16 && sl != 32) return HODLR_E_ARG;
     if (cfg->restart < 1 || cfg->restart > 127 || !(cfg->tol > 0) || !(cfg->eps_aca > 0 && cfg->eps_aca < 1) ||
-        cfg->nmax < 1 || cfg->max_outer < 1)
+        cfg->nmax < 1 || cfg->max_outer < 1 || cfg->builders < 1 || cfg->builders > 16)
         return HODLR_E_ARG;
     auto *srv = new hodlr_server;
     int rc = sguard(srv, [&] {
         HIP_CHECK(hipSetDevice(device));
         srv->cfg = *cfg;
         srv->device = device;
-        srv->bctx.device = device;
-        HIP_CHECK(hipStreamCreate(&srv->bctx.stream));
+        for (int b = 0; b < cfg->builders; b++) {
+            srv->bctx.push_back(std::make_unique<hodlr_ctx>());
+            srv->bctx.back()->device = device;
+            HIP_CHECK(hipStreamCreate(&srv->bctx.back()->stream));
+        }
         size_t fr = 0, tot = 0;
         HIP_CHECK(hipMemGetInfo(&fr, &tot));
         srv->st.budget_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : 0.9 * (double)fr;
-        srv->builder = std::thread([srv] { srv->build_loop(); });
+        for (int b = 0; b < cfg->builders; b++) srv->builders.emplace_back([srv, b] { srv->build_loop(b); });
         srv->scheduler = std::thread([srv] { srv->sched_loop(); });
     });
     if (rc != HODLR_OK) {
@@ -843,13 +854,15 @@ void hodlr_server_destroy(hodlr_server *srv) {
     srv->cv_build.notify_all();
     srv->cv_work.notify_all();
     srv->cv_mem.notify_all();
-    if (srv->builder.joinable()) srv->builder.join();
+    for (auto &t : srv->builders)
+        if (t.joinable()) t.join();
     if (srv->scheduler.joinable()) srv->scheduler.join();
     {
         std::unique_lock<std::mutex> lk(srv->mu);
         for (auto &kv : srv->ops) srv->free_op(*kv.second);
     }
-    if (srv->bctx.stream) (void)hipStreamDestroy(srv->bctx.stream);
+    for (auto &c : srv->bctx)
+        if (c->stream) (void)hipStreamDestroy(c->stream);
     delete srv;
 }
 

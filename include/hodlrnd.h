@@ -278,6 +278,7 @@ typedef struct {
     double eps_aca;
     int32_t nmax;
     int32_t dprime;
+    int32_t builders;      /* concurrent operator builds (builder threads, each with its own stream) */
 } hodlr_server_config;
 
 typedef struct {

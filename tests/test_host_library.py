@@ -159,10 +159,10 @@ def test_server_config_validation():
     """hodlr_server_create checks its configuration before touching a GPU."""
     import ctypes as C
     bad = [dict(slots=3), dict(restart=0), dict(restart=200), dict(tol=0.0), dict(eps_aca=1.5), dict(nmax=0),
-           dict(max_outer=0)]
+           dict(max_outer=0), dict(builders=0), dict(builders=17)]
     for b in bad:
         kw = dict(hbm_budget=0.0, slots=8, restart=100, max_outer=20, tol=1e-8, inner_floor=0.0, eps_aca=1e-8,
-                  nmax=64, dprime=0)
+                  nmax=64, dprime=0, builders=2)
         kw.update(b)
         cfg = H.ServerConfig(**kw)
         h = C.c_void_p()
