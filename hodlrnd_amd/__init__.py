@@ -100,6 +100,11 @@ SIGNATURES = {
     "hodlr_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_void_p, C.c_void_p]),
     "hodlr_server_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "hodlr_op_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "hodlr_op_load": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p]),
+    "hodlr_op_get_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "hodlr_vec_save": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p]),
+    "hodlr_vec_load": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "hodlr_op_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_op_block_ranks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "hodlr_op_destroy": (None, [C.c_void_p]),
@@ -245,10 +250,20 @@ class Tree:
         self.info = TreeInfo()
         lib.hodlr_tree_get_info(self.h, C.byref(self.info))
 
+    @classmethod
+    def _borrowed(cls, ctx, handle, owner):
+        """View of a tree owned by a loaded operator (not destroyed here)."""
+        t = cls.__new__(cls)
+        t.ctx, t.h, t._owner = ctx, handle, owner
+        t.info = TreeInfo()
+        lib.hodlr_tree_get_info(t.h, C.byref(t.info))
+        t.n, t.ndim = t.info.npoints, t.info.ndim
+        return t
+
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owner", None) is None:
             lib.hodlr_tree_destroy(self.h)
-            self.h = None
+        self.h = None
 
     @property
     def nlevels(self):
@@ -273,6 +288,26 @@ class Tree:
         idx = np.empty(max(int(ptr[-1]), 1), dtype=np.int64)
         lib.hodlr_tree_lists(self.h, which, _p(ptr), _p(idx))
         return ptr, idx[:int(ptr[-1])]
+
+
+def vec_save(path, v):
+    """cereal portable-binary std::vector<double> (the reference's data/*.bin)."""
+    v = _f64(v).ravel()
+    rc = lib.hodlr_vec_save(str(path).encode(), len(v), _p(v))
+    if rc != OK:
+        raise HodlrError(rc, "vec_save: %s" % path)
+
+
+def vec_load(path):
+    n = C.c_int64()
+    rc = lib.hodlr_vec_load(str(path).encode(), 0, None, C.byref(n))
+    if rc != OK:
+        raise HodlrError(rc, "vec_load: %s" % path)
+    v = np.empty(n.value)
+    rc = lib.hodlr_vec_load(str(path).encode(), len(v), _p(v), C.byref(n))
+    if rc != OK:
+        raise HodlrError(rc, "vec_load: %s" % path)
+    return v
 
 
 class Kernel:
@@ -324,6 +359,31 @@ class Operator:
                                                  C.byref(self.h)), "op_assemble")
         self.n = sum(t.n for t in self.trees)
         self.prec = prec
+
+    def save(self, path):
+        """Operator file: points, kernel, ACA ranks and FP64 arenas."""
+        self.ctx.check(lib.hodlr_op_save(self.h, str(path).encode()), "op_save")
+
+    @classmethod
+    def load(cls, path, ctx=None, prec=PREC_F64 | PREC_F32):
+        """Operator from a file written by save (no ACA is run)."""
+        ctx = ctx or default_context()
+        self = cls.__new__(cls)
+        self.h = C.c_void_p()
+        self.ctx = ctx
+        ctx.check(lib.hodlr_op_load(ctx.h, str(path).encode(), prec, C.byref(self.h)), "op_load")
+        self.trees, p = [], 0
+        while True:
+            th = C.c_void_p()
+            if lib.hodlr_op_get_tree(self.h, p, C.byref(th)) != OK:
+                break
+            self.trees.append(Tree._borrowed(ctx, th, self))
+            p += 1
+        self.kernels = [None] * len(self.trees)
+        self.tree, self.kernel = self.trees[0], None
+        self.n = sum(t.n for t in self.trees)
+        self.prec = prec
+        return self
 
     def offsets(self):
         out = np.zeros(len(self.trees) + 1, dtype=np.int64)

@@ -426,6 +426,283 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave form of the same ACA for small blocks (max(m, n) <= 64 and
+// min(m, n) <= CAP <= 32): one 64-lane wave per block, lane l owning row l
+// and column l.  The factors live in registers (lane i holds U[0..r)[i],
+// lane j holds V[0..r)[j]; a pivot's values are broadcast by shuffles), the
+// block's point coordinates in LDS, and the used-index sets are 64-bit
+// masks, so an ACA step has no global-memory round trip and no barrier.
+// The control flow, pivot rules and every reduction (wave_sum trees, the
+// same fma order in the residuals) are those of aca_kernel; the norm
+// estimate adds the 2|<u_q,u_r>||<v_q,v_r>| terms one q at a time, in the
+// order of LowRank_matrix.hpp:335-338.
+namespace {
+
+__device__ __forceinline__ double entry_xy(const KernelDev &k, const double *a, const double *b, int64_t pi,
+                                           int64_t pj, double wj) {
+    if (pi == pj) return k.diag;
+    double s = 0.0;
+    for (int d = 0; d < k.ndim; d++) {
+        double t = a[d] - b[d];
+        double t2 = t * t;
+        s = s + t2;
+    }
+    double r = sqrt(s);
+    double f;
+    if (k.family == 1) f = 1.0 / r;
+    else if (k.family == 0) f = log(r);
+    else f = exp(-r);
+    double v = k.scale * f;
+    if (k.W) v = v * wj;
+    return v;
+}
+
+// wave argmax of |v| over candidate lanes, lowest index on ties
+__device__ __forceinline__ void wave_argmax(double &val, double &av, int &idx) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double ov = __shfl_xor(val, o, kWave);
+        double oa = __shfl_xor(av, o, kWave);
+        int oi = __shfl_xor(idx, o, kWave);
+        if (better(oa, oi, av, idx)) { val = ov; av = oa; idx = oi; }
+    }
+}
+
+__device__ __forceinline__ uint64_t low_bits(int len) { return len >= 64 ? ~0ull : ((1ull << len) - 1ull); }
+
+// k-th (0-based) set bit of mask, 0 when there is none
+__device__ __forceinline__ int kth_bit(uint64_t mask, int64_t k) {
+    for (int64_t t = 0; t < k && mask; t++) mask &= mask - 1;
+    return mask ? __builtin_ctzll(mask) : 0;
+}
+
+// midpoint of the widest gap between the distinct indices in mask
+// (LowRank_matrix.hpp:181-198: sorted list, first strictly widest gap)
+__device__ __forceinline__ int mask_mid_gap(uint64_t mask) {
+    if (!mask) return 0;
+    int prev = __builtin_ctzll(mask), at = prev;
+    unsigned int mx = 0;
+    mask &= mask - 1;
+    while (mask) {
+        const int cur = __builtin_ctzll(mask);
+        const unsigned int d = (unsigned int)(cur - prev);
+        if (d > mx) { mx = d; at = prev; }
+        prev = cur;
+        mask &= mask - 1;
+    }
+    return at + (int)(mx / 2);
+}
+
+}  // namespace
+
+template <int CAP>
+__global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict__ blocks,
+                                                      const KernelDev *__restrict__ Ks, double tol,
+                                                      int32_t *__restrict__ out_rank,
+                                                      int32_t *__restrict__ out_flags, int fill,
+                                                      const FillDest *__restrict__ dest,
+                                                      const int64_t *__restrict__ leaf_start,
+                                                      const int32_t *__restrict__ leaf_cnt,
+                                                      double *__restrict__ UP, double *__restrict__ VP) {
+    __shared__ double xr[kWave * kMaxDimDev], xc[kWave * kMaxDimDev];
+    __shared__ int64_t prow[kWave];
+    const AcaBlock B = blocks[blockIdx.x];
+    const KernelDev K = Ks[B.prob];
+    const int l = threadIdx.x;
+    const int m = B.m, n = B.n, cap = B.cap;
+    const int mn = m < n ? m : n;
+    const int nd = K.ndim;
+    // stage the block's points: row l and column l of lane l
+    const int64_t pi = l < m ? (int64_t)B.ord[B.row0l + l] : -1;
+    const int64_t pj = l < n ? (int64_t)B.ord[B.col0l + l] : -2;
+    for (int d = 0; d < nd; d++) {
+        xr[l * kMaxDimDev + d] = l < m ? K.X[(pi - K.pos0) * nd + d] : 0.0;
+        xc[l * kMaxDimDev + d] = l < n ? K.X[(pj - K.pos0) * nd + d] : 0.0;
+    }
+    prow[l] = pi;
+    const double wj = (K.W && l < n) ? K.W[pj - K.pos0] : 0.0;
+    __syncthreads();
+
+    double Ur[CAP], Vr[CAP];
+#pragma unroll
+    for (int q = 0; q < CAP; q++) { Ur[q] = 0.0; Vr[q] = 0.0; }
+    uint64_t rused = 1ull, cused = 0ull;           // row 0 is the first pivot row
+    const uint64_t rvalid = low_bits(m), cvalid = low_bits(n);
+    // the pivot index lists rind / cind as multisets: lane k counts the
+    // entries equal to k (the gap rule only needs the distinct values, but
+    // a retry pops the last entry, which may also occur earlier)
+    int rcnt = l == 0 ? 1 : 0, ccnt = 0;
+    int rlast = 0;                                 // rind[nri - 1]
+    int rrem = m - 1, crem = n, nci = 0;
+    int r = 0, pivot = 0, flags = 0;
+    double maxv = 0.0, mnorm = 0.0, rnorm = 0.0, cnorm = 0.0, gamma = 0.0;
+    double rowv = 0.0, colv = 0.0;                 // residual row (lane j) and column (lane i)
+    uint64_t rng = B.seed;
+
+    // residual row ri (LowRank_matrix.hpp:126-133)
+    auto eval_row = [&](int ri) {
+        const double *a = xr + ri * kMaxDimDev;
+        double v = l < n ? entry_xy(K, a, xc + l * kMaxDimDev, prow[ri], pj, wj) : 0.0;
+#pragma unroll
+        for (int q = 0; q < CAP; q++)
+            if (q < r) {
+                const double uq = __shfl(Ur[q], ri, kWave);
+                v = fma(-uq, Vr[q], v);
+            }
+        rowv = v;
+        double bv = 0.0, ba = -1.0;
+        int bi = INT_MAX;
+        if (l < n && !((cused >> l) & 1ull)) { bv = v; ba = fabs(v); bi = l; }
+        wave_argmax(bv, ba, bi);
+        if (bi == INT_MAX) {  // no column left: index 0, as *begin() of an empty set
+            bi = 0;
+            bv = __shfl(v, 0, kWave);
+        }
+        maxv = bv;
+        pivot = bi;
+    };
+    // residual column cj (LowRank_matrix.hpp:237-243)
+    auto eval_col = [&](int cj) {
+        const double *b = xc + cj * kMaxDimDev;
+        const int64_t pjc = __shfl(pj, cj, kWave);
+        const double wc = __shfl(wj, cj, kWave);
+        double v = l < m ? entry_xy(K, xr + l * kMaxDimDev, b, pi, pjc, wc) : 0.0;
+#pragma unroll
+        for (int q = 0; q < CAP; q++)
+            if (q < r) {
+                const double vq = __shfl(Vr[q], cj, kWave);
+                v = fma(-vq, Ur[q], v);
+            }
+        colv = v;
+        double bv = 0.0, ba = -1.0;
+        int bi = INT_MAX;
+        if (l < m && !((rused >> l) & 1ull)) { bv = v; ba = fabs(v); bi = l; }
+        wave_argmax(bv, ba, bi);
+        if (bi == INT_MAX) {
+            bi = 0;
+            bv = __shfl(v, 0, kWave);
+        }
+        maxv = bv;
+        pivot = bi;
+    };
+
+    do {
+        eval_row(rlast);
+        int count = 0;
+        bool at_end = false, use_rand = true;
+        while (fabs(maxv) < tol && count < kMaxTries && crem > 0 && rrem > 0) {  // :144-216
+            int nr;
+            const uint64_t freer = ~rused & rvalid;
+            if (r < 3) {
+                nr = at_end ? (freer ? 63 - __builtin_clzll(freer) : 0) : (freer ? __builtin_ctzll(freer) : 0);
+                at_end = !at_end;
+            } else {
+                // the retried row was popped from the list: the gap search
+                // sees the list without it
+                if (use_rand) nr = kth_bit(freer, (int64_t)(splitmix64_next(rng) % (uint64_t)rrem));
+                else nr = mask_mid_gap(__ballot(rcnt - (l == rlast ? 1 : 0) > 0));
+                use_rand = !use_rand;
+            }
+            const bool was_used = (rused >> nr) & 1ull;
+            rcnt += (l == nr ? 1 : 0) - (l == rlast ? 1 : 0);  // pop the failed row, push nr
+            rlast = nr;
+            rused |= 1ull << nr;
+            if (!was_used) rrem--;
+            eval_row(nr);
+            count++;
+        }
+        if (count == kMaxTries || crem == 0 || rrem == 0) break;  // :220-225
+        count = 0;
+        ccnt += l == pivot ? 1 : 0;
+        int clast = pivot;
+        cused |= 1ull << pivot;
+        nci++;
+        crem--;
+        gamma = 1.0 / maxv;  // :233
+        eval_col(pivot);
+        while (fabs(maxv) < tol && count < kMaxTries && crem > 0 && rrem > 0) {  // :245-316
+            nci--;
+            int nc;
+            const uint64_t freec = ~cused & cvalid;
+            if (nci < 3) {
+                // :256-258: *remaining_col_ind.end() reads the set's size in libstdc++
+                nc = at_end ? crem : (freec ? __builtin_ctzll(freec) : 0);
+                at_end = !at_end;
+            } else {
+                if (use_rand) nc = kth_bit(freec, (int64_t)(splitmix64_next(rng) % (uint64_t)crem));
+                else nc = mask_mid_gap(__ballot(ccnt - (l == clast ? 1 : 0) > 0));
+                use_rand = !use_rand;
+            }
+            const bool was_used = (cused >> nc) & 1ull;
+            ccnt += (l == nc ? 1 : 0) - (l == clast ? 1 : 0);  // pop the failed column, push nc
+            clast = nc;
+            cused |= 1ull << nc;
+            nci++;
+            if (!was_used) crem--;
+            eval_col(nc);
+            count++;
+        }
+        {  // :318-319
+            const bool was_used = (rused >> pivot) & 1ull;
+            rcnt += l == pivot ? 1 : 0;
+            rlast = pivot;
+            rused |= 1ull << pivot;
+            if (!was_used) rrem--;
+        }
+        if (r >= cap) {
+            if (cap < mn - 1) flags |= 2;  // rank budget exhausted
+            break;
+        }
+        // u = gamma col, v = row and the norm estimate (:322-338)
+        const double rs = wave_sum(l < n ? rowv * rowv : 0.0);
+        const double cs = wave_sum(l < m ? colv * colv : 0.0);
+        const double unew = l < m ? gamma * colv : 0.0;
+        const double vnew = l < n ? rowv : 0.0;
+#pragma unroll
+        for (int q = 0; q < CAP; q++)
+            if (q == r) { Ur[q] = unew; Vr[q] = vnew; }
+        rnorm = sqrt(rs);
+        cnorm = sqrt(cs);
+        mnorm += fabs(gamma * gamma * rs * cs);
+#pragma unroll
+        for (int q = 0; q < CAP; q++)
+            if (q < r) {
+                const double du = wave_sum(Ur[q] * unew);
+                const double dv = wave_sum(Vr[q] * vnew);
+                mnorm += 2.0 * fabs(du) * fabs(dv);
+            }
+        r++;
+    } while ((double)(r * (m + n)) * rnorm * cnorm > fabs(maxv) * tol * mnorm && r < mn);  // :341-344
+
+    const bool dense = r >= mn - 1;  // :348
+    if (l == 0) {
+        out_rank[B.id] = r;
+        out_flags[B.id] = flags | (dense ? 1 : 0);
+    }
+    if (!fill || dense || (flags & 2)) return;
+    const FillDest D = dest[B.id];
+    if (l < m) {  // U columns into the row node's leaf-blocked panel
+        const int64_t ls = leaf_start[pi];
+        const int64_t nc = leaf_cnt[pi];
+        const int64_t base = D.up_base + (int64_t)D.R * (ls - B.row0) + (pi - ls);
+#pragma unroll
+        for (int q = 0; q < CAP; q++)
+            if (q < r) UP[base + (int64_t)(D.coff + q) * nc] = Ur[q];
+    }
+    if (l < n) {  // V columns into the column node's tile-major panel
+        const int64_t row = pj - B.col0;
+#pragma unroll
+        for (int q = 0; q < CAP; q++)
+            if (q < r) {
+                const int col = D.voff + q;
+                const int tile = col / kProjTile;
+                const int tw = min(kProjTile, D.T - tile * kProjTile);
+                VP[D.vp_base + (int64_t)tile * kProjTile * n + row * tw + (col - tile * kProjTile)] = Vr[q];
+            }
+    }
+}
+
 // dense-fallback blocks written into the row node's U panel
 __global__ __launch_bounds__(256) void dense_fill_kernel(const DenseBlock *__restrict__ blocks,
                                                          const KernelDev *__restrict__ Ks,
@@ -488,6 +765,15 @@ hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const Ker
     if (nblocks == 0) return hipSuccess;
     dim3 g((unsigned)nblocks);
     switch (bs) {
+    case 8:
+        aca_wave_kernel<8><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        break;
+    case 16:
+        aca_wave_kernel<16><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        break;
+    case 32:
+        aca_wave_kernel<32><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        break;
     case 64:
         aca_kernel<64><<<g, 64, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
         break;

@@ -411,6 +411,26 @@ int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes) {
     return HODLR_OK;
 }
 
+int hodlr_op_save(hodlr_op *op, const char *path) {
+    if (!op || !path) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        set_device(op->ctx);
+        save_op(*op, path);
+    });
+}
+
+int hodlr_op_load(hodlr_ctx *ctx, const char *path, int prec_flags, hodlr_op **out) {
+    if (!ctx || !path || !out) return HODLR_E_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] { *out = load_op(ctx, path, prec_flags); });
+}
+
+int hodlr_op_get_tree(hodlr_op *op, int problem, hodlr_tree **tree) {
+    if (!op || !tree || problem < 0 || problem >= (int)op->trees.size()) return HODLR_E_ARG;
+    *tree = op->trees[problem];
+    return HODLR_OK;
+}
+
 int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps) {
     if (!op || reps < 1) return HODLR_E_ARG;
     return guard(op->ctx, [&] {

@@ -12,6 +12,7 @@
 namespace hodlr {
 
 constexpr int kWave = 64;
+constexpr int kMaxDimDev = 8;  // largest NDIM
 
 // One problem's kernel.  Positions are global over a batch of problems;
 // this problem's points occupy [pos0, pos0 + n) and X, W hold them in tree

@@ -30,50 +30,69 @@ struct ArgError {
                                             std::to_string(__LINE__)};              \
     } while (0)
 
-// Device buffer owned by an object; freed in the destructor.
+// Device buffer owned by an object; freed in the destructor.  A buffer
+// allocated with a stream is stream-ordered (hipMallocAsync / hipFreeAsync
+// on that stream): freeing it does not synchronise the device, so
+// temporaries of one assembly never stall work on other streams.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    hipStream_t fs = nullptr;  // stream-ordered when set
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), fs(o.fs) {
         o.p = nullptr;
         o.bytes = 0;
+        o.fs = nullptr;
     }
     DevBuf &operator=(DevBuf &&o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            fs = o.fs;
             o.p = nullptr;
             o.bytes = 0;
+            o.fs = nullptr;
         }
         return *this;
     }
     ~DevBuf() { release(); }
-    void alloc(size_t nb) {
+    void alloc(size_t nb, hipStream_t stream = nullptr) {
         release();
         if (nb == 0) nb = 8;
-        hipError_t e = hipMalloc(&p, nb);
+        hipError_t e = stream ? hipMallocAsync(&p, nb, stream) : hipMalloc(&p, nb);
         if (e != hipSuccess) {
             p = nullptr;
-            throw ArgError{HODLR_E_NOMEM, "hipMalloc of " + std::to_string(nb) + " bytes failed"};
+            (void)hipGetLastError();
+            throw ArgError{HODLR_E_NOMEM, "device allocation of " + std::to_string(nb) + " bytes failed"};
         }
         bytes = nb;
+        fs = stream;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)(fs ? hipFreeAsync(p, fs) : hipFree(p));
         p = nullptr;
         bytes = 0;
+        fs = nullptr;
     }
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// v to a new device buffer.  With a stream the copy is ordered on it (and
+// waited for, so v may go away); temp = true makes the buffer itself
+// stream-ordered.
 template <class T>
-void upload(DevBuf &b, const std::vector<T> &v) {
-    b.alloc(v.size() * sizeof(T));
-    if (!v.empty()) HIP_CHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s = nullptr, bool temp = false) {
+    b.alloc(v.size() * sizeof(T), temp ? s : nullptr);
+    if (v.empty()) return;
+    if (s) {
+        HIP_CHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    } else {
+        HIP_CHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
 }
 
 }  // namespace hodlr

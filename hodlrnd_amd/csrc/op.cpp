@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -43,11 +44,37 @@ bool verbose() {
     return v;
 }
 
+// ACA kernel class of a block: 8, 16, 32 = one wave per block with the
+// factors in registers (rank capacity = class, so max(m, n) <= 64 and
+// min(m, n) <= class); 64, 256, 1024 = workgroup of that size with the
+// factors in the scratch pool
 int aca_block_size(int m, int n) {
-    const int mx = m > n ? m : n;
+    const int mx = m > n ? m : n, mn = m < n ? m : n;
+    if (mx <= 64 && mn <= 32) return mn <= 8 ? 8 : (mn <= 16 ? 16 : 32);
     if (mx <= 128) return 64;
     if (mx <= 4096) return 256;
     return 1024;
+}
+
+bool aca_in_registers(int cls) { return cls < 64; }
+
+// fn(begin, end) over [0, n) on up to 16 host threads (inline for small n)
+template <class F>
+void parallel_for(int64_t n, F &&fn) {
+    const int64_t kMin = 1 << 16;
+    int nt = (int)std::min<int64_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+    nt = (int)std::min<int64_t>(nt, (n + kMin - 1) / kMin);
+    if (nt <= 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t per = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        const int64_t a = t * per, b = std::min(n, a + per);
+        if (a < b) th.emplace_back([&fn, a, b] { fn(a, b); });
+    }
+    for (auto &x : th) x.join();
 }
 
 uint64_t block_seed(int level, int64_t row_node, int64_t col_node) {
@@ -62,46 +89,84 @@ uint64_t block_seed(int level, int64_t row_node, int64_t col_node) {
 void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vector<int32_t> &cap, int fill,
                   DevBuf &pool, int64_t budget, DevBuf &d_rank, DevBuf &d_flags, const DevBuf *d_dest,
                   hipStream_t s) {
-    // group by workgroup size, large blocks first inside a class
-    std::vector<int64_t> order(ids);
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-        int sa = aca_block_size(op.b_m[a], op.b_n[a]), sb = aca_block_size(op.b_m[b], op.b_n[b]);
-        if (sa != sb) return sa > sb;
-        return (int64_t)op.b_m[a] + op.b_n[a] > (int64_t)op.b_m[b] + op.b_n[b];
-    });
+    // group by kernel class (largest class first), large blocks first
+    // inside a class: bucket by class, then a counting sort on m + n
+    static const int kClasses[] = {1024, 256, 64, 32, 16, 8};
+    auto class_index = [](int c) {
+        for (int k = 0; k < 6; k++)
+            if (kClasses[k] == c) return k;
+        return 0;
+    };
+    std::vector<int64_t> order;
+    order.reserve(ids.size());
+    {
+        std::vector<std::vector<int64_t>> bucket(6);
+        for (int64_t b : ids) bucket[class_index(aca_block_size(op.b_m[b], op.b_n[b]))].push_back(b);
+        for (auto &bk : bucket) {
+            int64_t kmax = 0;
+            for (int64_t b : bk) kmax = std::max<int64_t>(kmax, (int64_t)op.b_m[b] + op.b_n[b]);
+            if (kmax > (1 << 16)) {
+                std::stable_sort(bk.begin(), bk.end(), [&](int64_t a, int64_t b) {
+                    return (int64_t)op.b_m[a] + op.b_n[a] > (int64_t)op.b_m[b] + op.b_n[b];
+                });
+                order.insert(order.end(), bk.begin(), bk.end());
+                continue;
+            }
+            std::vector<int64_t> cnt(kmax + 2, 0);
+            for (int64_t b : bk) cnt[kmax - (op.b_m[b] + op.b_n[b])]++;
+            int64_t run = 0;
+            for (auto &c : cnt) {
+                const int64_t t = c;
+                c = run;
+                run += t;
+            }
+            const size_t base = order.size();
+            order.resize(base + bk.size());
+            for (int64_t b : bk) order[base + cnt[kmax - (op.b_m[b] + op.b_n[b])]++] = b;
+        }
+    }
     DevBuf d_blocks;
     size_t i = 0;
     std::vector<AcaBlock> chunk;
+    std::vector<int64_t> offs;
     while (i < order.size()) {
-        chunk.clear();
         const int bs = aca_block_size(op.b_m[order[i]], op.b_n[order[i]]);
+        // the chunk: consecutive blocks of one class within the scratch budget
         int64_t used = 0;
+        const size_t i0 = i;
+        offs.clear();
         while (i < order.size()) {
             const int64_t b = order[i];
             if (aca_block_size(op.b_m[b], op.b_n[b]) != bs) break;
-            const int64_t need = aca_scratch_doubles(op.b_m[b], op.b_n[b], cap[b]);
-            if (!chunk.empty() && used + need > budget) break;
-            const int p = op.b_prob[b];
-            AcaBlock a;
-            a.row0 = op.node_start[op.b_row[b]];
-            a.col0 = op.node_start[op.b_col[b]];
-            a.row0l = a.row0 - op.pos_base[p];
-            a.col0l = a.col0 - op.pos_base[p];
-            a.scratch = used;
-            a.ord = op.ord.as<int32_t>() + op.ord_off[p] + (int64_t)op.b_level[b] * op.trees[p]->n;
-            a.m = op.b_m[b];
-            a.n = op.b_n[b];
-            a.cap = cap[b];
-            a.id = (int32_t)b;
-            a.seed = block_seed(op.b_level[b], op.b_rowj[b], op.b_colj[b]);
-            a.prob = p;
-            a.pad = 0;
-            chunk.push_back(a);
+            const int64_t need = aca_in_registers(bs) ? 0 : aca_scratch_doubles(op.b_m[b], op.b_n[b], cap[b]);
+            if (i > i0 && used + need > budget) break;
+            offs.push_back(used);
             used += need;
             i++;
         }
-        if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double));
-        upload(d_blocks, chunk);
+        chunk.resize(i - i0);
+        parallel_for((int64_t)chunk.size(), [&](int64_t q0, int64_t q1) {
+            for (int64_t q = q0; q < q1; q++) {
+                const int64_t b = order[i0 + q];
+                const int p = op.b_prob[b];
+                AcaBlock &a = chunk[q];
+                a.row0 = op.node_start[op.b_row[b]];
+                a.col0 = op.node_start[op.b_col[b]];
+                a.row0l = a.row0 - op.pos_base[p];
+                a.col0l = a.col0 - op.pos_base[p];
+                a.scratch = offs[q];
+                a.ord = op.ord.as<int32_t>() + op.ord_off[p] + (int64_t)op.b_level[b] * op.trees[p]->n;
+                a.m = op.b_m[b];
+                a.n = op.b_n[b];
+                a.cap = cap[b];
+                a.id = (int32_t)b;
+                a.seed = block_seed(op.b_level[b], op.b_rowj[b], op.b_colj[b]);
+                a.prob = p;
+                a.pad = 0;
+            }
+        });
+        if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double), s);
+        upload(d_blocks, chunk, s, true);
         HIP_CHECK(launch_aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), op.kdev.as<KernelDev>(), op.eps,
                              pool.as<double>(), d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
                              d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
@@ -163,8 +228,9 @@ void assemble(hodlr_op &op) {
                 for (int64_t q = 0; q < t.n; q++) W.push_back(op.kds[p].weight[t.perm[q]]);
             }
         }
-        upload(op.X, X);
-        upload(op.W, W);
+        upload(op.X, X, s);
+        upload(op.W, W, s);
+        op.w_off = woff;
         std::vector<KernelDev> kd(P);
         for (int p = 0; p < P; p++) {
             kd[p].family = op.kds[p].family;
@@ -175,7 +241,7 @@ void assemble(hodlr_op &op) {
             kd[p].W = woff[p] >= 0 ? op.W.as<double>() + woff[p] : nullptr;
             kd[p].pos0 = op.pos_base[p];
         }
-        upload(op.kdev, kd);
+        upload(op.kdev, kd, s);
         std::vector<int64_t> ls(N > 0 ? N : 1, 0);
         std::vector<int32_t> lc(N > 0 ? N : 1, 0);
         std::vector<int32_t> ord;
@@ -204,13 +270,21 @@ void assemble(hodlr_op &op) {
                 }
             }
         }
-        upload(op.leaf_start, ls);
-        upload(op.leaf_cnt, lc);
-        upload(op.ord, ord);
+        upload(op.leaf_start, ls, s);
+        upload(op.leaf_cnt, lc, s);
+        upload(op.ord, ord, s);
     }
 
     mark("inputs");
     // ---- 1. blocks in reference order, problem by problem ----
+    {
+        size_t tot = 0;
+        for (int p = 0; p < P; p++) tot += op.trees[p]->il.size();
+        for (auto *v : {&op.b_row, &op.b_col, &op.b_rowj, &op.b_colj}) v->reserve(tot);
+        for (auto *v : {&op.b_m, &op.b_n}) v->reserve(tot);
+        op.b_level.reserve(tot);
+        op.b_prob.reserve(tot);
+    }
     for (int p = 0; p < P; p++) {
         const hodlr_tree &t = *op.trees[p];
         for (int l = 0; l < t.nlevels; l++)
@@ -238,8 +312,8 @@ void assemble(hodlr_op &op) {
     // ---- 2. rank pass ----
     const double t_rank0 = now_s();
     DevBuf d_rank, d_flags, pool;
-    d_rank.alloc(sizeof(int32_t) * (nb + 1));
-    d_flags.alloc(sizeof(int32_t) * (nb + 1));
+    d_rank.alloc(sizeof(int32_t) * (nb + 1), s);
+    d_flags.alloc(sizeof(int32_t) * (nb + 1), s);
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
     const int64_t budget = std::max<int64_t>((int64_t)(free_b / 16 / sizeof(double)), (int64_t)1 << 24);
@@ -252,12 +326,20 @@ void assemble(hodlr_op &op) {
     }
     op.rank.assign(nb, 0);
     op.flags.assign(nb, 0);
+    if (!op.preset_rank.empty()) {  // ranks of a stored operator
+        if ((int64_t)op.preset_rank.size() != nb || (int64_t)op.preset_flags.size() != nb)
+            throw ArgError{HODLR_E_ARG, "stored block count does not match the rebuilt tree"};
+        op.rank = op.preset_rank;
+        op.flags = op.preset_flags;
+        todo.clear();
+    }
     while (!todo.empty()) {
         run_aca_pass(op, todo, cap, 0, pool, budget, d_rank, d_flags, nullptr, s);
         std::vector<int32_t> rk(nb), fl(nb);
         if (nb) {
-            HIP_CHECK(hipMemcpy(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
-            HIP_CHECK(hipMemcpy(fl.data(), d_flags.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+            HIP_CHECK(hipMemcpyAsync(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipMemcpyAsync(fl.data(), d_flags.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
         }
         std::vector<int64_t> again;
         for (int64_t b : todo) {
@@ -513,12 +595,14 @@ void assemble(hodlr_op &op) {
     op.VP64.alloc(sizeof(double) * op.n_vp);
     HIP_CHECK(hipMemsetAsync(op.VP64.p, 0, sizeof(double) * op.n_vp, s));  // padding columns
     DevBuf d_near;
-    upload(d_near, near);
-    {
+    upload(d_near, near, s, true);
+    if (op.fill_from) {
+        op.fill_from(op);
+    } else {
         DevBuf d_colsrc, d_dense, d_dest;
-        upload(d_colsrc, colsrc);
-        upload(d_dense, dense);
-        upload(d_dest, dest);
+        upload(d_colsrc, colsrc, s, true);
+        upload(d_dense, dense, s, true);
+        upload(d_dest, dest, s, true);
         HIP_CHECK(launch_near_fill(d_near.as<NearLeaf>(), (int64_t)near.size(), d_colsrc.as<int64_t>(),
                                    op.kdev.as<KernelDev>(), op.NP64.as<double>(), s));
         HIP_CHECK(launch_dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), op.kdev.as<KernelDev>(),
@@ -528,10 +612,13 @@ void assemble(hodlr_op &op) {
         for (int64_t b = 0; b < nb; b++)
             if (!(op.flags[b] & 1)) lowrank.push_back(b);
         // the fill pass repeats the rank pass with the same caps
-        HIP_CHECK(hipMemset(d_rank.p, 0xff, sizeof(int32_t) * (nb + 1)));
+        HIP_CHECK(hipMemsetAsync(d_rank.p, 0xff, sizeof(int32_t) * (nb + 1), s));
         run_aca_pass(op, lowrank, cap, 1, pool, budget, d_rank, d_flags, &d_dest, s);
         std::vector<int32_t> rk(nb > 0 ? nb : 1);
-        if (nb) HIP_CHECK(hipMemcpy(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+        if (nb) {
+            HIP_CHECK(hipMemcpyAsync(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
         for (int64_t b : lowrank)
             if (rk[b] != op.rank[b])
                 throw ArgError{HODLR_E_STATE, "ACA fill pass disagrees with the rank pass at block " +
@@ -551,11 +638,11 @@ void assemble(hodlr_op &op) {
         HIP_CHECK(launch_zero_self_diag(d_near.as<NearLeaf>(), (int64_t)near.size(), op.NP32.as<float>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
-    upload(op.proj, proj);
-    upload(op.red, red);
-    upload(op.panel, panel);
-    upload(op.segs, segs);
-    upload(op.tidx, tidx);
+    upload(op.proj, proj, s);
+    upload(op.red, red, s);
+    upload(op.panel, panel, s);
+    upload(op.segs, segs, s);
+    upload(op.tidx, tidx, s);
     op.n_proj = (int64_t)proj.size();
     op.n_red = (int64_t)red.size();
     op.n_panel = (int64_t)panel.size();
@@ -564,7 +651,7 @@ void assemble(hodlr_op &op) {
         std::vector<double> dg(N > 0 ? N : 1, 0.0);
         for (int p = 0; p < P; p++)
             for (int64_t q = op.pos_base[p]; q < op.pos_base[p + 1]; q++) dg[q] = op.kds[p].diag;
-        upload(op.diag, dg);
+        upload(op.diag, dg, s);
         op.uniform_diag = true;
         for (int p = 1; p < P; p++) op.uniform_diag = op.uniform_diag && op.kds[p].diag == op.kds[0].diag;
     }

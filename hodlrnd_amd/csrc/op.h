@@ -66,6 +66,7 @@ struct hodlr_op {
     hodlr_tree *tree = nullptr;           // first problem's tree
     std::vector<hodlr_tree *> trees;      // every problem's tree
     std::vector<hodlr_kernel_desc> kds;   // every problem's kernel (weights not kept)
+    std::vector<int64_t> w_off;           // per problem offset of its weights in W, -1 = none
     std::vector<int64_t> pos_base;        // nprob + 1: first global position of each problem
     hodlr_kernel_desc kd{};
     double eps = 0.0;
@@ -127,6 +128,12 @@ struct hodlr_op {
     // until memory is available or throw to abandon the build
     std::function<void(double)> admit;
 
+    // operator files: the stored ranks / flags replace the ACA rank pass and
+    // fill_from writes the FP64 arenas in place of the fill pass
+    std::vector<int32_t> preset_rank, preset_flags;
+    std::function<void(hodlr_op &)> fill_from;
+    std::vector<std::unique_ptr<hodlr_tree>> owned_trees;  // trees of a loaded operator
+
     // kernel form per (precision, RHS count): kMatvec* bits, indexed
     // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller
     int mv_variant[2][6] = {{0, 0, 2, 2, 2, 3}, {0, 0, 2, 2, 2, 3}};
@@ -138,4 +145,7 @@ struct hodlr_op {
 namespace hodlr {
 // op.trees and op.kds must be set; weights in kds are read during assembly only
 void assemble(hodlr_op &op);
+// operator files (opio.cpp)
+void save_op(hodlr_op &op, const char *path);
+hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags);
 }

@@ -114,7 +114,9 @@ struct Engine {
 
     Engine(hodlr_op &o, int K_, int m_, double tol_, double floor_, int max_outer_)
         : op(o), K(K_), m(m_), n(o.N), tol(tol_), floor(floor_), max_outer(max_outer_) {
-        HIP_CHECK(hipStreamCreate(&s));
+        // non-blocking: the engine's work is ordered on its own stream only,
+        // never behind null-stream calls made by builders or callers
+        HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         std::vector<Chunk> ch;
         for (int64_t r0 = 0; r0 < n; r0 += kChunkRows) {
             Chunk c;
@@ -827,7 +829,7 @@ int hodlr_server_create(int device, const hodlr_server_config *cfg, hodlr_server
         for (int b = 0; b < cfg->builders; b++) {
             srv->bctx.push_back(std::make_unique<hodlr_ctx>());
             srv->bctx.back()->device = device;
-            HIP_CHECK(hipStreamCreate(&srv->bctx.back()->stream));
+            HIP_CHECK(hipStreamCreateWithFlags(&srv->bctx.back()->stream, hipStreamNonBlocking));
         }
         size_t fr = 0, tot = 0;
         HIP_CHECK(hipMemGetInfo(&fr, &tot));

@@ -205,6 +205,22 @@ int hodlr_gmres_batch_device(hodlr_op *op, int nrhs, const double *b, double *x,
 int hodlr_gmres_batch(hodlr_op *op, int nrhs, const double *b, double *x,
                       const hodlr_gmres_params *params, hodlr_gmres_info *infos);
 
+/* ---- operator files and vector files ----
+ * save writes the operator's problems (points, box, tree knobs, kernel,
+ * weights), its ACA ranks and its FP64 arenas; load rebuilds the trees
+ * and lists from the points and takes ranks and arenas from the file (no
+ * ACA), building the FP32 copy when prec_flags asks for it.  A loaded
+ * operator owns its trees (hodlr_op_get_tree; destroyed with the operator).
+ * No reference counterpart: the reference keeps operators in memory. */
+int hodlr_op_save(hodlr_op *op, const char *path);
+int hodlr_op_load(hodlr_ctx *ctx, const char *path, int prec_flags, hodlr_op **out);
+int hodlr_op_get_tree(hodlr_op *op, int problem, hodlr_tree **tree);
+/* cereal portable-binary std::vector<double>, the reference's vector files
+ * (myHeaders.hpp:146-170, the data/ fixtures): byte 0x01, u64 count, doubles.
+ * load with v = NULL only reports the count. */
+int hodlr_vec_save(const char *path, int64_t n, const double *v);
+int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
+
 /* ---- kernel selection (no reference counterpart) ----
  * Each matvec pass has several kernel forms.  variant bits: 1 = matrix-core
  * (MFMA) projection V^T x, 2 = MFMA panel product (both only with 4 or more

@@ -168,3 +168,34 @@ def test_server_config_validation():
         h = C.c_void_p()
         assert H.lib.hodlr_server_create(0, C.byref(cfg), C.byref(h)) == H.E_ARG, b
     assert H.lib.hodlr_server_create(-1, C.byref(H.ServerConfig()), None) == H.E_ARG
+
+
+def test_cereal_vector_files(tmp_path, golden_dir):
+    """hodlr_vec_load reads the reference's fixtures (cereal portable binary,
+    data/x_10000.bin) exactly as the oracle's parser; save/load round trip."""
+    import os
+    x = H.vec_load(os.path.join(golden_dir, "x_10000.bin"))
+    assert np.array_equal(x, O.read_cereal_vec(os.path.join(golden_dir, "x_10000.bin")))
+    p = tmp_path / "v.bin"
+    v = O.splitmix_uniform(3, 777)
+    H.vec_save(p, v)
+    raw = p.read_bytes()
+    assert raw[0] == 1 and int.from_bytes(raw[1:9], "little") == 777 and len(raw) == 9 + 8 * 777
+    assert np.array_equal(H.vec_load(p), v)
+    assert np.array_equal(O.read_cereal_vec(str(p)), v)
+    p.write_bytes(b"\x00" + raw[1:])
+    with pytest.raises(H.HodlrError):
+        H.vec_load(p)
+
+
+def test_operator_file_rejects_bad_files(hctx, tmp_path):
+    """hodlr_op_load checks magic, version and length before any device work."""
+    import ctypes as C
+    h = C.c_void_p()
+    bad = tmp_path / "bad.op"
+    bad.write_bytes(b"NOTANOPERATORFILE")
+    assert H.lib.hodlr_op_load(hctx.h, str(bad).encode(), 3, C.byref(h)) == H.E_ARG
+    trunc = tmp_path / "trunc.op"
+    trunc.write_bytes(b"HODLROP1\x01\x00")
+    assert H.lib.hodlr_op_load(hctx.h, str(trunc).encode(), 3, C.byref(h)) == H.E_ARG
+    assert H.lib.hodlr_op_load(hctx.h, str(tmp_path / "missing.op").encode(), 3, C.byref(h)) == H.E_ARG
