@@ -100,6 +100,7 @@ SIGNATURES = {
     "hodlr_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.c_void_p, C.c_void_p]),
     "hodlr_server_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "hodlr_dense_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hodlr_op_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "hodlr_op_load": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p]),
     "hodlr_op_get_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
@@ -359,6 +360,14 @@ class Operator:
                                                  C.byref(self.h)), "op_assemble")
         self.n = sum(t.n for t in self.trees)
         self.prec = prec
+
+    def dense_rows(self, rows, x):
+        """FP64 direct-sum rows of the dense operator (point ids) times x."""
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        x = _f64(x)
+        y = np.empty(len(rows))
+        self.ctx.check(lib.hodlr_dense_rows(self.h, len(rows), _p(rows), _p(x), _p(y)), "dense_rows")
+        return y
 
     def save(self, path):
         """Operator file: points, kernel, ACA ranks and FP64 arenas."""

@@ -756,6 +756,34 @@ __global__ void convert_f64_f32_kernel(const double *__restrict__ a, float *__re
         b[i] = (float)a[i];
 }
 
+// Direct-sum rows of the dense operator: y[r] = sum_j A(rows[r], j) x[j]
+// over the row's problem, FP64, entries evaluated on the fly (the check of
+// main.cpp:42-57 on chosen rows).  One workgroup per row; fixed-order sums.
+__global__ __launch_bounds__(256) void dense_rows_kernel(const int64_t *__restrict__ rows,
+                                                         const int32_t *__restrict__ rprob,
+                                                         const KernelDev *__restrict__ Ks,
+                                                         const int64_t *__restrict__ pos_base,
+                                                         const double *__restrict__ x, double *__restrict__ y) {
+    __shared__ double red[4];
+    const int64_t i = rows[blockIdx.x];
+    const int p = rprob[blockIdx.x];
+    const KernelDev K = Ks[p];
+    const int64_t j0 = pos_base[p], j1 = pos_base[p + 1];
+    double acc = 0.0;
+    for (int64_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) acc = fma(entry(K, i, j), x[j], acc);
+    acc = wave_sum(acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) y[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+hipError_t launch_dense_rows(const int64_t *rows, const int32_t *rprob, int64_t nrows, const KernelDev *K,
+                             const int64_t *pos_base, const double *x, double *y, hipStream_t s) {
+    if (nrows == 0) return hipSuccess;
+    dense_rows_kernel<<<dim3((unsigned)nrows), 256, 0, s>>>(rows, rprob, K, pos_base, x, y);
+    return hipGetLastError();
+}
+
 // ---- host launchers ----
 
 hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,

@@ -3,6 +3,8 @@
 #include <exception>
 #include <new>
 
+#include <algorithm>
+
 #include "op.h"
 #include "solver.h"
 
@@ -325,6 +327,43 @@ int hodlr_matvec_device(hodlr_op *op, int prec, int nrhs, const void *x, void *y
         if (prec != HODLR_PREC_F64 && prec != HODLR_PREC_F32) throw ArgError{HODLR_E_ARG, "bad precision"};
         set_device(op->ctx);
         op->matvec_dev(prec, nrhs, x, y, stream ? static_cast<hipStream_t>(stream) : op->ctx->stream);
+    });
+}
+
+int hodlr_dense_rows(hodlr_op *op, int64_t nrows, const int64_t *rows, const double *x, double *y) {
+    if (!op || nrows < 0 || (nrows && (!rows || !x || !y))) return HODLR_E_ARG;
+    return guard(op->ctx, [&] {
+        set_device(op->ctx);
+        hipStream_t s = op->ctx->stream;
+        const int64_t n = op->N;
+        std::vector<int64_t> tr(nrows);
+        std::vector<int32_t> pr(nrows);
+        std::vector<std::vector<int64_t>> inv(op->trees.size());  // point id -> tree position
+        for (int64_t r = 0; r < nrows; r++) {  // point ids -> global tree positions and problems
+            const int64_t g = rows[r];
+            if (g < 0 || g >= n) throw ArgError{HODLR_E_ARG, "row index out of range"};
+            const int p = (int)(std::upper_bound(op->pos_base.begin(), op->pos_base.end(), g) -
+                                op->pos_base.begin()) - 1;
+            if (inv[p].empty()) {
+                const auto &perm = op->trees[p]->perm;
+                inv[p].resize(perm.size());
+                for (size_t q = 0; q < perm.size(); q++) inv[p][perm[q]] = (int64_t)q;
+            }
+            tr[r] = op->pos_base[p] + inv[p][g - op->pos_base[p]];
+            pr[r] = p;
+        }
+        std::vector<double> xt;
+        to_tree(op, 1, 1, x, xt);
+        DevBuf drows, dprob, dx, dy, dpb;
+        upload(drows, tr, s);
+        upload(dprob, pr, s);
+        upload(dx, xt, s);
+        upload(dpb, op->pos_base, s);
+        dy.alloc(sizeof(double) * std::max<int64_t>(nrows, 1));
+        HIP_CHECK(launch_dense_rows(drows.as<int64_t>(), dprob.as<int32_t>(), nrows, op->kdev.as<KernelDev>(),
+                                    dpb.as<int64_t>(), dx.as<double>(), dy.as<double>(), s));
+        HIP_CHECK(hipMemcpyAsync(y, dy.p, sizeof(double) * nrows, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
     });
 }
 

@@ -19,6 +19,8 @@ hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev *K
 hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev *K,
                             double *NP, hipStream_t s);
 hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s);
+hipError_t launch_dense_rows(const int64_t *rows, const int32_t *rprob, int64_t nrows, const KernelDev *K,
+                             const int64_t *pos_base, const double *x, double *y, hipStream_t s);
 hipError_t launch_zero_self_diag(const NearLeaf *l, int64_t nl, float *NP, hipStream_t s);
 hipError_t launch_f32_result(const float *y, const double *x, const double *diag, double *out, int64_t n, int k,
                              hipStream_t s);

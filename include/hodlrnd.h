@@ -183,6 +183,12 @@ int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y);
 int hodlr_matvec_device(hodlr_op *op, int prec, int nrhs, const void *x, void *y,
                         void *stream);
 
+/* Rows of the dense operator applied to x, evaluated on the device in FP64
+ * by direct summation (the dense self-check of main.cpp:42-57, on chosen
+ * rows): y[r] = sum_j A(rows[r], j) x[j].  rows are point ids (problems
+ * concatenated), x and y host buffers in point order / row order. */
+int hodlr_dense_rows(hodlr_op *op, int64_t nrows, const int64_t *rows, const double *x, double *y);
+
 /* HDD_matrix.hpp:51-62 with GMRES.hpp:160-303: solve A x = b, host buffers
  * in point order. x holds the initial guess unless params->x0_is_b. */
 int hodlr_gmres(hodlr_op *op, const double *b, double *x,

@@ -1,0 +1,101 @@
+"""The benchmark configurations at their real sizes on the GPU.
+
+  * configuration 1 (NDIM 3, N = 10,000 uniform iid points, A = I + (1/N)/r,
+    Nmax 64, eps_ACA 1e-10, d' = 0): the HODLR matvec against the oracle's
+    (1e-12) and against the dense FP64 product (100 eps_ACA); GMRES to
+    1e-10 against the oracle's solution (100 tol) with the same iteration
+    count (+-1);
+  * configuration 3 (NDIM 4, 16^4 Gauss-Legendre tensor nodes, N = 65,536,
+    A = I + w_j / r, Nmax 64, eps_ACA 1e-8, d' in {-1, 0, 3}): the FP64
+    HODLR matvec against device direct-sum rows of the dense operator (512
+    sampled rows, 100 eps_ACA), FP32 against FP64 (1e-5), and the FP32-inner
+    / FP64-refined solve to a certified 1e-8;
+  * the direct-sum rows themselves against the oracle's dense product.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = pytest.importorskip("hodlrnd_amd")
+from oracle import pyoracle as O  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_dense_rows_match_oracle():
+    say("case dense_rows")
+    n, nd = 3000, 3
+    pts = O.splitmix_uniform(17, n * nd).reshape(n, nd)
+    kern = H.Kernel(H.KERNEL_INV_R, nd, 1.0 / n, 1.0)
+    T = H.Tree(pts, -np.ones(nd), np.ones(nd), 64, 0)
+    A = H.Operator(T, kern, 1e-10)
+    x = O.splitmix_uniform(18, n)
+    y = A.dense_rows(np.arange(n), x)
+    yr = O.dense_matvec(O.KernelDesc(O.K_INV, nd, 1.0 / n, 1.0), pts, x)
+    assert relerr(y, yr) < 1e-13
+
+
+def test_config1_full_size():
+    say("case config1 N=10000")
+    n, nd = 10000, 3
+    pts = O.splitmix_uniform(1, n * nd).reshape(n, nd)
+    lo, hi = -np.ones(nd), np.ones(nd)
+    kern = H.Kernel(H.KERNEL_INV_R, nd, 1.0 / n, 1.0)
+    A = H.HODLRdD_matrix(kern, pts, lo, hi, nmax=64, dprime=0, eps_aca=1e-10)
+    A.Assemble_matrix_operators()
+    ok = O.KernelDesc(O.K_INV, nd, 1.0 / n, 1.0)
+    ref = O.Operator(O.Tree(pts, lo, hi, 64, 0), ok, 1e-10, sys_size=-1)
+    r_gpu, d_gpu = A.op.block_ranks()
+    r_ref, d_ref = ref.block_ranks()
+    agree = np.mean((r_gpu == r_ref) & (d_gpu == d_ref))
+    say("rank agreement", agree, "blocks", len(r_gpu))
+    assert agree == 1.0
+    x = O.splitmix_uniform(2, n)
+    y = A * x
+    e_ref = relerr(y, ref.matvec(x))
+    e_dense = relerr(y, O.dense_matvec(ok, pts, x))
+    say("vs oracle", e_ref, "vs dense", e_dense)
+    assert e_ref < 1e-12 and e_dense < 100 * 1e-10
+    b = O.splitmix_uniform(3, n)
+    xs = A.solve(b, tol=1e-10)
+    xr, ir = ref.gmres(b, tol=1e-10)
+    say("gmres gpu", A.last_info["iterations"], "oracle", ir["iterations"])
+    assert A.last_info["converged"] and abs(A.last_info["iterations"] - ir["iterations"]) <= 1
+    assert relerr(xs, xr) < 100 * 1e-10
+
+
+@pytest.mark.parametrize("dprime", [-1, 0, 3])
+def test_config3_gauss_legendre_4d(dprime):
+    say("case config3 dprime", dprime)
+    gx, gw = O.gauss_legendre(16)
+    pts = O.tensor_grid(4, gx)
+    wts = O.tensor_weights(4, gw)
+    n = len(pts)
+    assert n == 65536
+    T = H.Tree(pts, -np.ones(4), np.ones(4), 64, dprime)
+    A = H.Operator(T, H.Kernel(H.KERNEL_INV_R, 4, 1.0, 1.0, wts), 1e-8)
+    st = A.stats()
+    say("blocks", st["n_blocks"], "dense", st["n_dense"], "max rank", st["max_rank"])
+    x = O.splitmix_uniform(5, n)
+    y = A.matvec(x)
+    rows = np.unique((np.abs(O.splitmix_uniform(6, 512)) * n).astype(np.int64))
+    yd = A.dense_rows(rows, x)
+    e = np.linalg.norm(y[rows] - yd) / np.linalg.norm(yd)
+    say("sampled rows vs dense", e)
+    assert e < 100 * 1e-8
+    assert relerr(A.matvec(x, prec=H.PREC_F32), y) < 1e-5
+    b = O.splitmix_uniform(7, n)
+    xs, info = A.gmres(b, tol=1e-8, mode=H.SOLVE_MIXED, x0=np.zeros(n))
+    say("mixed solve", info["iterations"], info["outer"], info["true_resid"])
+    assert info["converged"] and info["true_resid"] < 1e-8
+    assert np.linalg.norm(b - A.matvec(xs)) / np.linalg.norm(b) < 1.01e-8
