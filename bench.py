@@ -270,7 +270,7 @@ def bench_batch(args, H):
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, problems one at a time)...")
-        cpu = cpu_baseline_batch(specs, args.nmax, args.eps, args.tol)
+        cpu = cpu_baseline_batch(specs, args.nmax, args.eps, args.tol, budget_s=args.cpu_budget)
         log("cpu baseline: %.4g solves/s" % cpu["value"])
     val = NI / latm.mean()
     iters = [i["iterations"] for i in infm]
@@ -320,7 +320,7 @@ def bench_batch(args, H):
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    return out
 
 
 def bench_serve(args, H):
@@ -352,7 +352,7 @@ def bench_serve(args, H):
         out = serve_stream(srv, plan, arr, args, H, Wl)
     finally:
         srv.close()
-    print(json.dumps(out))
+    return out
 
 
 def serve_stream(srv, plan, arr, args, H, Wl):
@@ -428,7 +428,7 @@ def serve_stream(srv, plan, arr, args, H, Wl):
         "all_converged": bool(okc),
         "max_resid": worst,
         "server": {k: st[k] for k in ("builds", "evictions", "ticks", "peak_bytes", "budget_bytes",
-                                      "build_seconds")},
+                                      "build_seconds", "tick_seconds")},
     }
     return out
 
@@ -444,6 +444,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline work per leg")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="grid100: skip the batched and serving legs reported beside the headline")
     ap.add_argument("--workload", choices=["grid100", "batch256", "serve"], default="grid100",
                     help="grid100: one N=1e6 solve per step; batch256: 256 uneven problems x 8 RHS "
                          "solved as one batch per step; serve: a Poisson stream of solve requests")
@@ -475,7 +478,7 @@ def main():
     if args.workload in ("batch256", "serve"):
         if args.nmax == 128:
             args.nmax = 64
-        (bench_batch if args.workload == "batch256" else bench_serve)(args, H)
+        print(json.dumps((bench_batch if args.workload == "batch256" else bench_serve)(args, H)))
         return
 
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)
@@ -533,6 +536,15 @@ def main():
     x[perm] = xt
     r = b - op.matvec(x, prec=H.PREC_F64)
     true_res = float(np.linalg.norm(r) / np.linalg.norm(b))
+    # against the dense operator itself, on 256 sampled rows (device direct
+    # sums): the HODLR approximation error of the matvec, and the residual
+    # of the mixed solution for the exact problem
+    rows = np.unique((np.abs(splitmix_uniform(77, 256)) * n).astype(np.int64))
+    xr = splitmix_uniform(78, n)
+    yd = op.dense_rows(rows, xr)
+    mv_err = float(np.linalg.norm(op.matvec(xr)[rows] - yd) / np.linalg.norm(yd))
+    dres = float(np.linalg.norm(b[rows] - op.dense_rows(rows, x)) / np.linalg.norm(b[rows]))
+    log("sampled dense rows: matvec error %.3g, residual of the mixed solution %.3g" % (mv_err, dres))
 
     # dominant kernel of the timed mixed solves
     kb64 = op.kernel_bytes(H.PREC_F64, 1)
@@ -584,6 +596,7 @@ def main():
         "refinement_sweeps": infm[-1]["outer"],
         "matvecs_mixed": {"f32": infm[-1]["matvecs_f32"], "f64": infm[-1]["matvecs_f64"]},
         "true_resid_mixed": true_res,
+        "dense_check": {"rows": int(len(rows)), "matvec_rel_err": mv_err, "mixed_solution_resid": dres},
         "resid_fp64_estimate": inf64[-1]["resid"],
         "matvec_ms": {"f64": mv64, "f32": mv32},
         "matvec_hbm_gbs": {"f64": kb64[0] / (mv64 * 1e-3) * 1e-9 if mv64 else None,
@@ -607,6 +620,27 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_extra:
+        # the batched and served forms of the same solve, each on its own
+        # workload, reported beside the headline.  Each runs in a child
+        # process of its own (a fresh device-memory state), after this
+        # process has released its operator.
+        del op, db, dx
+        import subprocess
+        legs = (("batched", ["--workload", "batch256", "--steps", "3", "--warmup", "1", "--cpu-budget", "10"]),
+                ("serving", ["--workload", "serve", "--requests", "100"]))
+        for key, extra in legs:
+            cmd = [sys.executable, "-u", os.path.abspath(__file__)] + extra + (["--no-cpu"] if args.no_cpu else [])
+            try:
+                r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=900)
+                line = [l for l in r.stdout.splitlines() if l.startswith("{")]
+                if r.returncode != 0 or not line:
+                    raise RuntimeError("exit status %d" % r.returncode)
+                r2 = json.loads(line[-1])
+                out[key] = {k: r2[k] for k in r2 if k not in ("metric", "higher_is_better", "scaling",
+                                                               "vs_baseline", "n_gpus", "data")}
+            except Exception as ex:  # the headline stands on its own
+                out[key] = {"error": repr(ex)}
     print(json.dumps(out))
 
 

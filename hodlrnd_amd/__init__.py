@@ -167,7 +167,7 @@ class ServerStats(C.Structure):
     _fields_ = [("builds", C.c_int64), ("evictions", C.c_int64), ("requests_done", C.c_int64),
                 ("columns_done", C.c_int64), ("ticks", C.c_int64), ("resident_ops", C.c_int64),
                 ("used_bytes", C.c_double), ("peak_bytes", C.c_double), ("budget_bytes", C.c_double),
-                ("build_seconds", C.c_double)]
+                ("build_seconds", C.c_double), ("tick_seconds", C.c_double)]
 
 
 class Context:

@@ -62,6 +62,26 @@ void from_tree(const hodlr_op *op, int k, int K, const double *src, double *dst)
     }
 }
 
+}  // namespace
+
+namespace hodlr {
+// Stream-ordered temporaries (assembly scratch, work lists) come from the
+XX
+// mapped spares every later assembly the cost of mapping and clearing it
+// again.  Best effort: without the attribute the pool still works.
+void keep_pool_memory(int device) {
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess || !pool) {
+        (void)hipGetLastError();
+        return;
+    }
+    uint64_t keep = (uint64_t)8 << 30;
+    if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) (void)hipGetLastError();
+}
+}  // namespace hodlr
+
+namespace {
+
 void set_device(hodlr_ctx *ctx) {
     if (ctx->device < 0) throw ArgError{HODLR_E_STATE, "host-only context (device -1) cannot run device work"};
     HIP_CHECK(hipSetDevice(ctx->device));
@@ -86,6 +106,7 @@ int hodlr_ctx_create(int device, hodlr_ctx **out) {
         if (device < 0 || device >= ndev) throw ArgError{HODLR_E_ARG, "no such device"};
         HIP_CHECK(hipSetDevice(device));
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        keep_pool_memory(device);
     });
     if (rc != HODLR_OK) {
         delete ctx;
@@ -608,12 +629,20 @@ int hodlr_device_free(hodlr_ctx *ctx, void *ptr) {
 
 int hodlr_copy_to_device(hodlr_ctx *ctx, void *dst, const void *src, int64_t bytes) {
     if (!ctx || !dst || !src || bytes < 0) return HODLR_E_ARG;
-    return guard(ctx, [&] { HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice)); });
+    return guard(ctx, [&] {  // after the work queued on the context's stream
+        set_device(ctx);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    });
 }
 
 int hodlr_copy_to_host(hodlr_ctx *ctx, void *dst, const void *src, int64_t bytes) {
     if (!ctx || !dst || !src || bytes < 0) return HODLR_E_ARG;
-    return guard(ctx, [&] { HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost)); });
+    return guard(ctx, [&] {
+        set_device(ctx);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    });
 }
 
 }  // extern "C"

@@ -58,11 +58,16 @@ int aca_block_size(int m, int n) {
 
 bool aca_in_registers(int cls) { return cls < 64; }
 
-// fn(begin, end) over [0, n) on up to 16 host threads (inline for small n)
+// host threads one assembly may use (set per thread by callers that run
+// several assemblies at once); 0 = up to 16
+thread_local int tl_threads = 0;
+
+// fn(begin, end) over [0, n) on the assembly's host threads (inline for small n)
 template <class F>
 void parallel_for(int64_t n, F &&fn) {
     const int64_t kMin = 1 << 16;
-    int nt = (int)std::min<int64_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+    int nt = tl_threads > 0 ? tl_threads
+                            : (int)std::min<int64_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
     nt = (int)std::min<int64_t>(nt, (n + kMin - 1) / kMin);
     if (nt <= 1) {
         fn(0, n);
@@ -316,7 +321,12 @@ void assemble(hodlr_op &op) {
     d_flags.alloc(sizeof(int32_t) * (nb + 1), s);
     size_t free_b = 0, total_b = 0;
     HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-    const int64_t budget = std::max<int64_t>((int64_t)(free_b / 16 / sizeof(double)), (int64_t)1 << 24);
+    // ACA scratch per launch: at most 1/16 of free memory and 4 GiB, so
+    // concurrent builds neither crowd out operators nor pay for clearing
+    // freshly mapped memory on every chunk
+    const int64_t budget = std::max<int64_t>(
+        std::min<int64_t>((int64_t)(free_b / 16 / sizeof(double)), ((int64_t)4 << 30) / (int64_t)sizeof(double)),
+        (int64_t)1 << 24);
     std::vector<int32_t> cap(nb);
     std::vector<int64_t> todo(nb);
     for (int64_t b = 0; b < nb; b++) {
@@ -812,3 +822,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
         }
     }
 }
+
+namespace hodlr {
+void set_assembly_threads(int n) { tl_threads = n; }
+}  // namespace hodlr

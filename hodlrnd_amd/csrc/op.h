@@ -147,6 +147,10 @@ struct hodlr_op {
 namespace hodlr {
 // op.trees and op.kds must be set; weights in kds are read during assembly only
 void assemble(hodlr_op &op);
+// host threads used by assemblies run from the calling thread (0 = up to 16)
+void set_assembly_threads(int n);
+// keep freed stream-ordered memory of the device's default pool mapped
+void keep_pool_memory(int device);
 // operator files (opio.cpp)
 void save_op(hodlr_op &op, const char *path);
 hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags);

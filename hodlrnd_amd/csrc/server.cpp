@@ -484,6 +484,7 @@ struct hodlr_server {
     hodlr::Clock::time_point t0 = hodlr::Clock::now();
     hodlr_server_stats st{};
     double used = 0;  // admitted operator bytes + engine bytes
+    double tick_t0 = 0;
     std::string last_error;
 
     double now() const { return std::chrono::duration<double>(hodlr::Clock::now() - t0).count(); }
@@ -566,6 +567,8 @@ void hodlr_server::fail_op(OpEntry &e, int code, const std::string &msg) {
 
 void hodlr_server::build_loop(int idx) {
     (void)hipSetDevice(device);
+    // concurrent builds share the host: 16 threads between them
+    hodlr::set_assembly_threads(std::max(1, 16 / (int)bctx.size()));
     hodlr_ctx &bc = *bctx[idx];
     while (true) {
         OpEntry *e = nullptr;
@@ -720,6 +723,7 @@ void hodlr_server::sched_loop() {
             }
             if (!need_engine.empty()) continue;  // admit into the new slots first
             const double t = now();
+            tick_t0 = t;
             for (auto &a : admits) {
                 OpEntry &e = *std::get<0>(a);
                 auto &req = std::get<2>(a);
@@ -751,6 +755,7 @@ void hodlr_server::sched_loop() {
             for (OpEntry *e : run) e->engine->launch_c();
             std::unique_lock<std::mutex> lk(mu);
             st.ticks++;
+            st.tick_seconds += now() - tick_t0;
             for (OpEntry *e : run) e->in_tick = false;
             for (OpEntry *e : run) {  // basis vectors allocated this tick
                 used += e->engine->bytes - e->engine_bytes;
@@ -831,6 +836,7 @@ int hodlr_server_create(int device, const hodlr_server_config *cfg, hodlr_server
             srv->bctx.back()->device = device;
             HIP_CHECK(hipStreamCreateWithFlags(&srv->bctx.back()->stream, hipStreamNonBlocking));
         }
+        hodlr::keep_pool_memory(device);
         size_t fr = 0, tot = 0;
         HIP_CHECK(hipMemGetInfo(&fr, &tot));
         srv->st.budget_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : 0.9 * (double)fr;

@@ -320,6 +320,7 @@ typedef struct {
     int64_t resident_ops;
     double used_bytes, peak_bytes, budget_bytes;
     double build_seconds;  /* summed assembly wall time */
+    double tick_seconds;   /* scheduler time spent in ticks (device work and collection) */
 } hodlr_server_stats;
 
 int hodlr_server_create(int device, const hodlr_server_config *cfg, hodlr_server **out);
