@@ -66,7 +66,7 @@ void from_tree(const hodlr_op *op, int k, int K, const double *src, double *dst)
 
 namespace hodlr {
 // Stream-ordered temporaries (assembly scratch, work lists) come from the
-XX
+// device's default memory pool; keeping up to 8 GiB of freed pool memory
 // mapped spares every later assembly the cost of mapping and clearing it
 // again.  Best effort: without the attribute the pool still works.
 void keep_pool_memory(int device) {
