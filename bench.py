@@ -51,17 +51,17 @@ def splitmix_uniform(seed, n):
     return Wl.splitmix_uniform(seed, n)
 
 
-def measured_traffic(kernel, alg_bytes):
+def measured_traffic(kernel, alg_bytes, name="matvec_traffic_grid100_N1e6.json"):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel factors),
     used only if it was measured on this exact operator (same algorithmic
     bytes per launch); otherwise None."""
-    path = os.path.join(ROOT, "profiles", "matvec_traffic_grid100_N1e6.json")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         d = json.load(open(path))
         e = d["kernels"][kernel]
         if abs(e["algorithmic_bytes_per_launch"] - alg_bytes) <= 1e-3 * alg_bytes:
-            return float(e["hbm_bytes_per_launch"]), "profiles/matvec_traffic_grid100_N1e6.json"
+            return float(e["hbm_bytes_per_launch"]), "profiles/" + name
     except (OSError, KeyError, ValueError):
         pass
     return None, None
@@ -200,6 +200,14 @@ def bench_batch(args, H):
     variants = {("f64" if pr == H.PREC_F64 else "f32") + "_k%d" % k: op.get_variant(pr, k)
                 for pr in (H.PREC_F64, H.PREC_F32) for k in (4, 8, 16)}
     log("kernel forms tuned in %.1fs: %s" % (t_tune, variants))
+    if args.matvec_only:
+        res = {}
+        for prec in (H.PREC_F64, H.PREC_F32):
+            ms = op.matvec_timing(prec, K, args.matvec_only)
+            kb = op.kernel_bytes(prec, K)
+            res["f64" if prec == H.PREC_F64 else "f32"] = {"ms": list(ms), "bytes": list(kb)}
+            log("matvec prec %d K %d: %.3f ms (proj %.3f, panel %.3f)" % (prec, K, ms[0], ms[1], ms[2]))
+        return {"matvec_only": res, "kernel_forms": variants}
     ctx = op.ctx
     B = np.empty((N, K))
     for p in range(len(specs)):
@@ -315,7 +323,9 @@ def bench_batch(args, H):
         "operator_gb": {"f64": st["bytes_f64"] * 1e-9, "f32": st["bytes_f32"] * 1e-9},
         "roofline": {
             "bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "measured_copy_gbs": copy_gbs, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "measured_copy_gbs": copy_gbs,
+            "traffic": measured_traffic(name, bytes_, "matvec_traffic_batch256_k8.json")[0],
+            "traffic_source": measured_traffic(name, bytes_, "matvec_traffic_batch256_k8.json")[1],
             "algorithmic_bytes_per_launch": bytes_, "avg_launch_ms": avg_ms,
         },
         "cpu_baseline": cpu,
@@ -458,6 +468,8 @@ def main():
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--plain-all", action="store_true",
                     help="batch256: time the plain FP64 path on every problem, not 64 spread ones")
+    ap.add_argument("--matvec-only", type=int, default=0, metavar="R",
+                    help="grid100: assemble, then only R FP64 and R FP32 matvecs (counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the library's plain copy kernel (1 GiB, 3 launches): the "
                          "known-size kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -496,6 +508,12 @@ def main():
     st = op.stats()
     log("assembled in %.2fs: e_near %.3g e_far %.3g, %.1f GB FP64 + %.1f GB FP32"
         % (t_asm, st["e_near"], st["e_far"], st["bytes_f64"] * 1e-9, st["bytes_f32"] * 1e-9))
+
+    if args.matvec_only:
+        for prec in (H.PREC_F64, H.PREC_F32):
+            ms = op.matvec_timing(prec, 1, args.matvec_only)
+            log("matvec prec %d: %.3f ms (proj %.3f, panel %.3f)" % (prec, ms[0], ms[1], ms[2]))
+        return
 
     ctx = tree.ctx
     perm = tree.perm()

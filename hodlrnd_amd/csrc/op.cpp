@@ -789,8 +789,13 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             HIP_CHECK(hipMemsetAsync(dx.p, 0, es * N * k, s));
             double best[2] = {1e300, 1e300};
             int pick[2] = {0, 0};
-            std::vector<int> cands = {0, kMatvecPanelFlat};
+            // vector-ALU panel forms only up to 8 RHS: from 16 on their
+            // per-lane accumulators spill past one wave per SIMD and they
+            // run 4-20x slower than the matrix-core form (never chosen)
+            std::vector<int> cands;
+            if (k <= 8) cands = {0, kMatvecPanelFlat};
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
+            if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
             for (int v : cands) {
                 mv_variant[pi][lg] = v;
                 hipEvent_t ev[3];
