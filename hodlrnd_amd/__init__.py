@@ -53,7 +53,8 @@ class OpStats(C.Structure):
                 ("bytes_f64", C.c_double), ("bytes_f32", C.c_double),
                 ("matvec_bytes_f64", C.c_double), ("matvec_bytes_f32", C.c_double),
                 ("assemble_seconds", C.c_double), ("rank_seconds", C.c_double),
-                ("fill_seconds", C.c_double)]
+                ("fill_seconds", C.c_double), ("n_skeleton", C.c_int64), ("e_skel", C.c_double),
+                ("bytes_core", C.c_double)]
 
 
 class GmresParams(C.Structure):
@@ -85,6 +86,8 @@ SIGNATURES = {
                                     C.c_void_p]),
     "hodlr_op_assemble_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                           C.c_int, C.c_void_p]),
+    "hodlr_op_assemble_skeleton": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                             C.c_int, C.c_void_p]),
     "hodlr_op_problem_offsets": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     "hodlr_gmres_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hodlr_gmres_batch_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -340,11 +343,26 @@ def kernel_4d_test(num_points, kii):
 class Operator:
     """Assembled HODLR operator (HDD_matrix.hpp:39-47, GMRES.hpp:160-303).
     With lists of trees and kernels: the block-diagonal operator of a batch
-    of independent problems (hodlr_op_assemble_batch)."""
+    of independent problems (hodlr_op_assemble_batch).  sys_size (an int >= 0)
+    keeps every compressed block whose column node holds more than sys_size
+    points as a skeleton evaluated on the fly in each matvec
+    (hodlr_op_assemble_skeleton; the reference's SYS_SIZE = 2 Nmax,
+    LowRank_matrix.hpp:22-36)."""
 
-    def __init__(self, tree, kernel, eps_aca, prec=PREC_F64 | PREC_F32):
+    def __init__(self, tree, kernel, eps_aca, prec=PREC_F64 | PREC_F32, sys_size=None):
         self.h = C.c_void_p()
-        if isinstance(tree, (list, tuple)):
+        self.sys_size = -1 if sys_size is None else int(sys_size)
+        if self.sys_size >= 0:
+            self.trees = list(tree) if isinstance(tree, (list, tuple)) else [tree]
+            self.kernels = list(kernel) if isinstance(kernel, (list, tuple)) else [kernel]
+            self.ctx = self.trees[0].ctx
+            arr = (C.c_void_p * len(self.trees))(*[t.h.value for t in self.trees])
+            kds = (KernelDesc * len(self.kernels))(*[k.desc() for k in self.kernels])
+            self.ctx.check(lib.hodlr_op_assemble_skeleton(self.ctx.h, len(self.trees), arr, kds, eps_aca,
+                                                          self.sys_size, prec, C.byref(self.h)),
+                           "op_assemble_skeleton")
+            self.tree, self.kernel = self.trees[0], self.kernels[0]
+        elif isinstance(tree, (list, tuple)):
             self.trees, self.kernels = list(tree), list(kernel)
             self.ctx = self.trees[0].ctx
             arr = (C.c_void_p * len(self.trees))(*[t.h.value for t in self.trees])
@@ -493,7 +511,7 @@ class Operator:
         return v.value
 
     def matvec_timing(self, prec=PREC_F64, nrhs=1, reps=10):
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         self.ctx.check(lib.hodlr_matvec_timing(self.h, prec, nrhs, reps, _p(ms)), "matvec_timing")
         return ms
 
@@ -517,10 +535,13 @@ class HODLRdD_matrix:
     """Reference-shaped operator object (HDD_matrix.hpp:10-89)."""
 
     def __init__(self, kernel, points, x1, x2, nmax=64, dprime=0, eps_aca=1e-10, ctx=None,
-                 prec=PREC_F64 | PREC_F32):
+                 prec=PREC_F64 | PREC_F32, sys_size=None):
+        """sys_size: None stores every factor; an int (the reference's
+        SYS_SIZE is 2 * nmax) keeps larger blocks as on-the-fly skeletons."""
         self.kernel = kernel
         self.eps_aca = eps_aca
         self.prec = prec
+        self.sys_size = sys_size
         self.tree = Tree(points, x1, x2, nmax, dprime, ctx)
         self.N = self.tree.n
         self.op = None
@@ -529,7 +550,7 @@ class HODLRdD_matrix:
         return self.N
 
     def Assemble_matrix_operators(self):
-        self.op = Operator(self.tree, self.kernel, self.eps_aca, self.prec)
+        self.op = Operator(self.tree, self.kernel, self.eps_aca, self.prec, sys_size=self.sys_size)
 
     def __mul__(self, x):
         return self.op.matvec(x)

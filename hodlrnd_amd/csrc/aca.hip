@@ -197,7 +197,8 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
                                                  const FillDest *__restrict__ dest,
                                                  const int64_t *__restrict__ leaf_start,
                                                  const int32_t *__restrict__ leaf_cnt,
-                                                 double *__restrict__ UP, double *__restrict__ VP) {
+                                                 double *__restrict__ UP, double *__restrict__ VP,
+                                                 int32_t *__restrict__ SK) {
     __shared__ Shared<BS> sh;
     const AcaBlock B = blocks[blockIdx.x];
     const KernelDev K = Ks[B.prob];
@@ -404,6 +405,13 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     if (!fill || dense || (flags & 2)) return;
     __syncthreads();
     const FillDest D = dest[B.id];
+    if (D.sk_row >= 0) {  // skeleton block: its pivot rows and columns, in step order
+        for (int q = tid; q < r; q += BS) {
+            SK[D.sk_row + q] = B.ord[B.row0l + rind[q]];
+            SK[D.sk_col + q] = B.ord[B.col0l + cind[q]];
+        }
+        return;
+    }
     // U columns into the row node's leaf-blocked panel
     for (int i = tid; i < m; i += BS) {
         const int64_t p = B.ord[B.row0l + i];
@@ -682,6 +690,9 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
     }
     if (!fill || dense || (flags & 2)) return;
     const FillDest D = dest[B.id];
+    // skeleton blocks are routed to aca_kernel, which writes their pivots;
+    // the host finds any pivot slot left unwritten
+    if (D.sk_row >= 0) return;
     if (l < m) {  // U columns into the row node's leaf-blocked panel
         const int64_t ls = leaf_start[pi];
         const int64_t nc = leaf_cnt[pi];
@@ -789,7 +800,7 @@ hipError_t launch_dense_rows(const int64_t *rows, const int32_t *rprob, int64_t 
 hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,
                       double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
                       const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
-                      hipStream_t s) {
+                      int32_t *SK, hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
     dim3 g((unsigned)nblocks);
     switch (bs) {
@@ -803,13 +814,16 @@ hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const Ker
         aca_wave_kernel<32><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
         break;
     case 64:
-        aca_kernel<64><<<g, 64, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        aca_kernel<64><<<g, 64, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP,
+                                                 SK);
         break;
     case 256:
-        aca_kernel<256><<<g, 256, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        aca_kernel<256><<<g, 256, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP,
+                                                 SK);
         break;
     default:
-        aca_kernel<1024><<<g, 1024, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        aca_kernel<1024><<<g, 1024, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP,
+                                                 SK);
         break;
     }
     return hipGetLastError();

@@ -39,6 +39,10 @@ struct FillDest {
     int64_t vp_base;      // V panel of the column node (elements, tile-major)
     int32_t R, coff;      // U panel width and this block's first column
     int32_t T, voff;      // V panel width (padded to 4) and this block's first column
+    // skeleton blocks (>= 0): the block keeps no factors; the fill pass writes
+    // the tree positions of its r pivot rows to SK[sk_row ..) and of its r
+    // pivot columns to SK[sk_col ..) (LowRank_matrix.hpp:685-689)
+    int64_t sk_row, sk_col;
 };
 
 // a block stored dense (LowRank_matrix.hpp:348-363)
@@ -105,6 +109,62 @@ struct Segment {
     int64_t vec;
     int32_t len;
     int32_t arena;
+};
+
+// ---- skeleton blocks (evaluated on the fly) ----
+// Slots: every skeleton block b of rank r owns r row slots (its pivot rows,
+// grouped by column node) and r column slots (its pivot columns, grouped by
+// row node).  SK holds the tree position of every row slot, then of every
+// column slot; t1 / t2 hold one value per row / column slot and RHS.
+constexpr int kSkelChunk = 1024;  // sources of one projection work item
+
+// t1[t0 + l] (+)= sum over sources j in [src0, src0 + nsrc) of A(SK[t0 + l], j) x_j
+struct SkelProj {
+    int64_t t0;           // first row slot (<= 64 targets, one per lane)
+    int64_t src0;         // first source tree position (the column node's points)
+    int64_t out;          // >= 0: first partial slot (64 per item), < 0: direct into t1
+    int32_t ntgt, nsrc;
+    int32_t prob, pad;
+};
+
+// sum of the partial projections of one target tile
+struct SkelReduce {
+    int64_t t0, slot0;
+    int32_t ntgt, nch;
+};
+
+// t2[c0 .. c0 + r) = G t1[t0 .. t0 + r), G the r x r core pseudo-inverse
+// (column-major, at element g of the core arena)
+struct SkelCore {
+    int64_t g, t0, c0;
+    int32_t r, pad;
+};
+
+// y[y0 + l] += sum over column slots q in [c0, c0 + nc) of A(y0 + l, SKC[q]) t2[q]
+struct SkelExpand {
+    int64_t y0, c0;
+    int32_t h, nc;        // rows (<= 64) and column slots
+    int32_t prob, pad;
+};
+
+struct KernelDev;
+
+// the skeleton product's work lists (device pointers into the operator's
+// buffers; ex_lvl is a host array: expand items sorted by tree level,
+// level l at [ex_lvl[l], ex_lvl[l + 1]))
+struct SkeletonPlan {
+    const SkelProj *proj;
+    const SkelReduce *red;
+    const SkelCore *core;
+    const SkelExpand *ex;
+    int64_t nproj, nred, ncore;
+    const int64_t *ex_lvl;
+    int nlvl;
+    const KernelDev *K;
+    const int32_t *SK;    // row slots, then column slots at SK + nrow_slots
+    int64_t nrow_slots;
+    const double *G;
+    int max_ndim;         // largest NDIM of the problems (sizes the LDS source tiles)
 };
 
 // rows of a batched vector that belong to one problem (never crosses a

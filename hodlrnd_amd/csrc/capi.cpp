@@ -232,6 +232,11 @@ int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
 
 int hodlr_op_assemble_batch(hodlr_ctx *ctx, int nprob, hodlr_tree *const *trees, const hodlr_kernel_desc *k,
                             double eps_aca, int prec_flags, hodlr_op **out) {
+    return hodlr_op_assemble_skeleton(ctx, nprob, trees, k, eps_aca, -1, prec_flags, out);
+}
+
+int hodlr_op_assemble_skeleton(hodlr_ctx *ctx, int nprob, hodlr_tree *const *trees, const hodlr_kernel_desc *k,
+                               double eps_aca, int sys_size, int prec_flags, hodlr_op **out) {
     if (!ctx || nprob < 1 || !trees || !k || !out) return HODLR_E_ARG;
     *out = nullptr;
     return guard(ctx, [&] {
@@ -250,6 +255,7 @@ int hodlr_op_assemble_batch(hodlr_ctx *ctx, int nprob, hodlr_tree *const *trees,
         op->trees.assign(trees, trees + nprob);
         op->kds.assign(k, k + nprob);
         op->eps = eps_aca;
+        op->sys_size = sys_size < 0 ? -1 : sys_size;
         op->prec_flags = prec_flags | HODLR_PREC_F64;
         try {
             assemble(*op);
@@ -537,24 +543,26 @@ int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms) 
         dx.alloc(es * n * nrhs);
         dy.alloc(es * n * nrhs);
         HIP_CHECK(hipMemset(dx.p, 0, es * n * nrhs));
-        std::vector<hipEvent_t> ev((size_t)3 * reps);
+        std::vector<hipEvent_t> ev((size_t)4 * reps);
         for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
         op->matvec_dev(prec, nrhs, dx.p, dy.p, s);  // warm-up
-        for (int r = 0; r < reps; r++) op->matvec_dev(prec, nrhs, dx.p, dy.p, s, &ev[(size_t)3 * r]);
+        for (int r = 0; r < reps; r++) op->matvec_dev(prec, nrhs, dx.p, dy.p, s, &ev[(size_t)4 * r]);
         HIP_CHECK(hipStreamSynchronize(s));
-        double tot = 0, a = 0, b = 0;
+        double a = 0, b = 0, c = 0;
         for (int r = 0; r < reps; r++) {
-            float t01 = 0, t12 = 0;
-            HIP_CHECK(hipEventElapsedTime(&t01, ev[3 * r], ev[3 * r + 1]));
-            HIP_CHECK(hipEventElapsedTime(&t12, ev[3 * r + 1], ev[3 * r + 2]));
+            float t01 = 0, t12 = 0, t23 = 0;
+            HIP_CHECK(hipEventElapsedTime(&t01, ev[4 * r], ev[4 * r + 1]));
+            HIP_CHECK(hipEventElapsedTime(&t12, ev[4 * r + 1], ev[4 * r + 2]));
+            HIP_CHECK(hipEventElapsedTime(&t23, ev[4 * r + 2], ev[4 * r + 3]));
             a += t01;
             b += t12;
-            tot += t01 + t12;
+            c += t23;
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
-        ms[0] = tot / reps;
+        ms[0] = (a + b + c) / reps;
         ms[1] = a / reps;
         ms[2] = b / reps;
+        ms[3] = c / reps;
     });
 }
 

@@ -18,10 +18,16 @@
 //                        256-column tiles, each a contiguous row-major block;
 //   4. fill pass: near field, dense fallbacks, and the ACA again, writing
 //      factors into place.
+// With sys_size >= 0, compressed blocks whose column node holds more than
+// sys_size points are skeletons (LowRank_matrix.hpp:22-36): they get no
+// panel columns; the fill pass writes their pivot positions and the host
+// forms G = C^+ of every r x r core (LowRank_matrix.hpp:685-698), and the
+// matvec evaluates their entries on the fly (skeleton.hip).
 #include "op.h"
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <thread>
 #include <cstdio>
 #include <cstdlib>
@@ -58,14 +64,22 @@ int aca_block_size(int m, int n) {
 
 bool aca_in_registers(int cls) { return cls < 64; }
 
+// a block that may become a skeleton runs in aca_kernel in both passes
+// (only that kernel writes pivot positions)
+int aca_class(const hodlr_op &op, int64_t b) {
+    const int c = aca_block_size(op.b_m[b], op.b_n[b]);
+    return (op.sys_size >= 0 && op.b_n[b] > op.sys_size && c < 64) ? 64 : c;
+}
+
 // host threads one assembly may use (set per thread by callers that run
 // several assemblies at once); 0 = up to 16
 thread_local int tl_threads = 0;
 
-// fn(begin, end) over [0, n) on the assembly's host threads (inline for small n)
+// fn(begin, end) over [0, n) on the assembly's host threads, at least
+// `grain` items per thread (inline for small n)
 template <class F>
-void parallel_for(int64_t n, F &&fn) {
-    const int64_t kMin = 1 << 16;
+void parallel_for(int64_t n, F &&fn, int64_t grain = 1 << 16) {
+    const int64_t kMin = grain;
     int nt = tl_threads > 0 ? tl_threads
                             : (int)std::min<int64_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
     nt = (int)std::min<int64_t>(nt, (n + kMin - 1) / kMin);
@@ -89,6 +103,97 @@ uint64_t block_seed(int level, int64_t row_node, int64_t col_node) {
     return s;
 }
 
+// A_ij on the host from tree-order coordinates X and weights W of one
+// problem (user_kernel.hpp:50-58,105-113,171-177; the same arithmetic as the
+// device entry())
+double host_entry(const hodlr_kernel_desc &k, int nd, const double *X, const double *W, int64_t i, int64_t j) {
+    if (i == j) return k.diag;
+    double s = 0.0;
+    for (int d = 0; d < nd; d++) {
+        const double t = X[i * nd + d] - X[j * nd + d];
+        const double t2 = t * t;
+        s = s + t2;
+    }
+    const double r = std::sqrt(s);
+    const double f = k.family == 1 ? 1.0 / r : (k.family == 0 ? std::log(r) : std::exp(-r));
+    double v = k.scale * f;
+    if (W) v = v * W[j];
+    return v;
+}
+
+// G = C^+ for the r x r core C of a skeleton block (column-major, a is
+// destroyed): the linear map of the column-pivoted Householder QR solve the
+// reference applies (ColPivHouseholderQR, LowRank_matrix.hpp:57,698).  The
+// column of largest remaining norm is pivoted first; the rank is cut at the
+// first |R_kk| <= eps r |R_00|; G = P [R11^-1 Q1^T ; 0].
+void core_pinv(int r, std::vector<double> &a, double *G) {
+    std::vector<double> tau(r, 0.0), w(r);
+    std::vector<int> perm(r);
+    for (int j = 0; j < r; j++) perm[j] = j;
+    double maxpiv = 0.0;
+    for (int k = 0; k < r; k++) {
+        int best = k;
+        double bn = -1.0;
+        for (int j = k; j < r; j++) {
+            double s = 0.0;
+            for (int i = k; i < r; i++) s += a[(size_t)j * r + i] * a[(size_t)j * r + i];
+            if (s > bn) {
+                bn = s;
+                best = j;
+            }
+        }
+        if (best != k) {
+            for (int i = 0; i < r; i++) std::swap(a[(size_t)k * r + i], a[(size_t)best * r + i]);
+            std::swap(perm[k], perm[best]);
+        }
+        double *x = &a[(size_t)k * r];
+        const double alpha = x[k];
+        double s = 0.0;
+        for (int i = k + 1; i < r; i++) s += x[i] * x[i];
+        double beta = alpha, t = 0.0;
+        if (s != 0.0) {
+            beta = -std::copysign(std::sqrt(alpha * alpha + s), alpha);
+            t = (beta - alpha) / beta;
+            const double sc = 1.0 / (alpha - beta);
+            for (int i = k + 1; i < r; i++) x[i] *= sc;
+        }
+        x[k] = beta;
+        tau[k] = t;
+        if (k == 0) maxpiv = std::fabs(beta);
+        if (t == 0.0) continue;
+        for (int j = k + 1; j < r; j++) {
+            double *y = &a[(size_t)j * r];
+            double d = y[k];
+            for (int i = k + 1; i < r; i++) d += x[i] * y[i];
+            d *= t;
+            y[k] -= d;
+            for (int i = k + 1; i < r; i++) y[i] -= d * x[i];
+        }
+    }
+    const double thr = 2.220446049250313e-16 * (double)r * maxpiv;
+    int rk = 0;
+    while (rk < r && std::fabs(a[(size_t)rk * r + rk]) > thr) rk++;
+    for (int c = 0; c < r; c++) {
+        std::fill(w.begin(), w.end(), 0.0);
+        w[c] = 1.0;
+        for (int k = 0; k < r; k++) {  // w = Q^T e_c
+            if (tau[k] == 0.0) continue;
+            const double *v = &a[(size_t)k * r];
+            double d = w[k];
+            for (int i = k + 1; i < r; i++) d += v[i] * w[i];
+            d *= tau[k];
+            w[k] -= d;
+            for (int i = k + 1; i < r; i++) w[i] -= d * v[i];
+        }
+        for (int i = rk - 1; i >= 0; i--) {
+            double v = w[i];
+            for (int j = i + 1; j < rk; j++) v -= a[(size_t)j * r + i] * w[j];
+            w[i] = v / a[(size_t)i * r + i];
+        }
+        for (int i = 0; i < r; i++) G[(size_t)c * r + perm[i]] = i < rk ? w[i] : 0.0;
+    }
+}
+
 // Runs one ACA pass over `ids` (block ids) with per-block caps, chunked so
 // that the scratch pool stays within `budget` doubles.
 void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vector<int32_t> &cap, int fill,
@@ -106,7 +211,7 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
     order.reserve(ids.size());
     {
         std::vector<std::vector<int64_t>> bucket(6);
-        for (int64_t b : ids) bucket[class_index(aca_block_size(op.b_m[b], op.b_n[b]))].push_back(b);
+        for (int64_t b : ids) bucket[class_index(aca_class(op, b))].push_back(b);
         for (auto &bk : bucket) {
             int64_t kmax = 0;
             for (int64_t b : bk) kmax = std::max<int64_t>(kmax, (int64_t)op.b_m[b] + op.b_n[b]);
@@ -135,14 +240,14 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
     std::vector<AcaBlock> chunk;
     std::vector<int64_t> offs;
     while (i < order.size()) {
-        const int bs = aca_block_size(op.b_m[order[i]], op.b_n[order[i]]);
+        const int bs = aca_class(op, order[i]);
         // the chunk: consecutive blocks of one class within the scratch budget
         int64_t used = 0;
         const size_t i0 = i;
         offs.clear();
         while (i < order.size()) {
             const int64_t b = order[i];
-            if (aca_block_size(op.b_m[b], op.b_n[b]) != bs) break;
+            if (aca_class(op, b) != bs) break;
             const int64_t need = aca_in_registers(bs) ? 0 : aca_scratch_doubles(op.b_m[b], op.b_n[b], cap[b]);
             if (i > i0 && used + need > budget) break;
             offs.push_back(used);
@@ -175,7 +280,8 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
         HIP_CHECK(launch_aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), op.kdev.as<KernelDev>(), op.eps,
                              pool.as<double>(), d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
                              d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
-                             op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(), s));
+                             op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(),
+                             op.SK.as<int32_t>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
         if (verbose())
             std::fprintf(stderr, "[hodlr] aca %s pass: %zu/%zu blocks (workgroup %d, scratch %.2f GB)\n",
@@ -220,9 +326,9 @@ void assemble(hodlr_op &op) {
 
     // ---- tree-order inputs: coordinates, weights, leaf of every position,
     //      per (problem, level) ascending-id order maps ----
+    std::vector<double> X, W;  // host copies, kept for the skeleton cores
+    std::vector<int64_t> xoff(P), woff(P, -1);
     {
-        std::vector<double> X, W;
-        std::vector<int64_t> xoff(P), woff(P, -1);
         for (int p = 0; p < P; p++) {
             const hodlr_tree &t = *op.trees[p];
             xoff[p] = (int64_t)X.size();
@@ -368,6 +474,16 @@ void assemble(hodlr_op &op) {
     if (verbose()) std::fprintf(stderr, "[hodlr] rank pass %.3f s, %lld blocks\n", t_rank1 - t_rank0, (long long)nb);
 
     mark("rank pass");
+    // skeleton blocks: compressed (not dense) with more than sys_size
+    // columns (LowRank_matrix.hpp:27); a dense fallback stays dense (:348-363)
+    std::vector<uint8_t> skel(nb, 0);
+    op.n_skel = 0;
+    if (op.sys_size >= 0)
+        for (int64_t b = 0; b < nb; b++)
+            if (!(op.flags[b] & 1) && op.b_n[b] > op.sys_size) {
+                skel[b] = 1;
+                op.n_skel++;
+            }
     // ---- 3. layout ----
     // U panel of row node a: R[a] columns, its blocks in block order (a
     // dense-fallback block contributes its n columns of K); t of row node a:
@@ -379,12 +495,12 @@ void assemble(hodlr_op &op) {
     std::vector<int64_t> rb_ptr(nn + 1, 0), rb_list(nb > 0 ? nb : 1);  // blocks per row node, block order
     for (int64_t b = 0; b < nb; b++) {
         const bool dense = op.flags[b] & 1;
-        width[b] = dense ? op.b_n[b] : op.rank[b];
+        width[b] = dense ? op.b_n[b] : (skel[b] ? 0 : op.rank[b]);
         const int64_t a = op.b_row[b];
         coff[b] = (int32_t)R[a];
         R[a] += width[b];
         rb_ptr[a + 1]++;
-        if (!dense) {
+        if (!dense && !skel[b]) {
             tcoff[b] = (int32_t)TR[a];
             TR[a] += op.rank[b];
             voff[b] = (int32_t)Tb[op.b_col[b]];
@@ -415,6 +531,48 @@ void assemble(hodlr_op &op) {
         op.e_v += (double)count[g] * Tb[g];
     }
     op.e_u = (double)op.n_up;
+
+    // skeleton slots: the pivot rows of the blocks (., c) of column node c
+    // are contiguous (row slots), as are the pivot columns of the blocks
+    // (a, .) of row node a (column slots); cores in block order
+    std::vector<int64_t> skr_off(nb, -1), skc_off(nb, -1), g_off(nb, -1);
+    std::vector<int64_t> Sc, Sa, sc_base, sa_base;
+    std::vector<SkelCore> skcore;
+    op.n_sk = op.n_g = 0;
+    op.e_skel = 0;
+    if (op.n_skel) {
+        Sc.assign(nn, 0);
+        Sa.assign(nn, 0);
+        sc_base.assign(nn, 0);
+        sa_base.assign(nn, 0);
+        for (int64_t b = 0; b < nb; b++)
+            if (skel[b]) {
+                Sc[op.b_col[b]] += op.rank[b];
+                Sa[op.b_row[b]] += op.rank[b];
+            }
+        int64_t rc = 0, ra = 0;
+        for (int64_t g = 0; g < nn; g++) {
+            sc_base[g] = rc;
+            rc += Sc[g];
+            sa_base[g] = ra;
+            ra += Sa[g];
+        }
+        op.n_sk = rc;
+        if (op.n_sk > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many skeleton slots"};
+        std::vector<int64_t> fc(sc_base), fa(sa_base);
+        for (int64_t b = 0; b < nb; b++) {
+            if (!skel[b]) continue;
+            const int64_t r = op.rank[b];
+            skr_off[b] = fc[op.b_col[b]];
+            fc[op.b_col[b]] += r;
+            skc_off[b] = fa[op.b_row[b]];
+            fa[op.b_row[b]] += r;
+            g_off[b] = op.n_g;
+            op.n_g += r * r;
+            op.e_skel += (double)r * (double)(op.b_m[b] + op.b_n[b]);
+            if (r) skcore.push_back(SkelCore{g_off[b], skr_off[b], skc_off[b], (int32_t)r, 0});
+        }
+    }
 
     // near field and panel work per non-empty leaf
     std::vector<NearLeaf> near;
@@ -528,6 +686,8 @@ void assemble(hodlr_op &op) {
         fd.vp_base = vp_base[c];
         fd.T = (int32_t)Tpad[c];
         fd.voff = voff[b];
+        fd.sk_row = skel[b] ? skr_off[b] : -1;
+        fd.sk_col = skel[b] ? op.n_sk + skc_off[b] : -1;
         dest[b] = fd;
         if (op.flags[b] & 1) {
             DenseBlock db;
@@ -541,7 +701,7 @@ void assemble(hodlr_op &op) {
             db.prob = op.b_prob[b];
             db.pad = 0;
             dense.push_back(db);
-        } else {
+        } else if (!skel[b]) {
             for (int k = 0; k < op.rank[b]; k++) tidx[vcol_base[c] + voff[b] + k] = t_base[a] + tcoff[b] + k;
             op.t_lowrank += op.rank[b];
         }
@@ -588,10 +748,72 @@ void assemble(hodlr_op &op) {
     if (nvcol > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many V columns for 32-bit tile offsets"};
     op.n_slots = slots;
 
+    // skeleton work: projection items per column node (64 pivot rows x up
+    // to kSkelChunk sources), expand items per row node (64 rows), by level
+    std::vector<SkelProj> skproj;
+    std::vector<SkelReduce> skred;
+    std::vector<SkelExpand> skex;
+    op.n_skslots = 0;
+    op.skex_lvl.assign(1, 0);
+    if (op.n_skel) {
+        std::vector<int32_t> nprob(nn, 0), nlev(nn, 0);
+        int nlevels = 0;
+        for (int p = 0; p < P; p++) {
+            const hodlr_tree &t = *op.trees[p];
+            nlevels = std::max(nlevels, t.nlevels);
+            for (int l = 0; l < t.nlevels; l++)
+                for (int64_t j = 0; j < t.level_size(l); j++) {
+                    nprob[node_base[p] + t.node(l, j)] = p;
+                    nlev[node_base[p] + t.node(l, j)] = l;
+                }
+        }
+        std::vector<std::vector<SkelExpand>> bylev(nlevels);
+        for (int64_t g = 0; g < nn; g++) {
+            if (Sc[g]) {
+                const int64_t ns = count[g];
+                const int64_t nch = (ns + kSkelChunk - 1) / kSkelChunk;
+                for (int64_t q0 = 0; q0 < Sc[g]; q0 += kWave) {
+                    const int nt = (int)std::min<int64_t>(kWave, Sc[g] - q0);
+                    if (nch > 1) skred.push_back(SkelReduce{sc_base[g] + q0, op.n_skslots, nt, (int32_t)nch});
+                    for (int64_t ch = 0; ch < nch; ch++) {
+                        SkelProj it;
+                        it.t0 = sc_base[g] + q0;
+                        it.src0 = op.node_start[g] + ch * kSkelChunk;
+                        it.out = nch > 1 ? op.n_skslots + ch * kWave : -1;
+                        it.ntgt = nt;
+                        it.nsrc = (int32_t)std::min<int64_t>(kSkelChunk, ns - ch * kSkelChunk);
+                        it.prob = nprob[g];
+                        it.pad = 0;
+                        skproj.push_back(it);
+                    }
+                    if (nch > 1) op.n_skslots += nch * kWave;
+                }
+            }
+            if (Sa[g]) {
+                if (Sa[g] > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many skeleton columns in one node"};
+                for (int64_t r0 = 0; r0 < count[g]; r0 += kWave)
+                    bylev[nlev[g]].push_back(SkelExpand{op.node_start[g] + r0, sa_base[g],
+                                                        (int32_t)std::min<int64_t>(kWave, count[g] - r0),
+                                                        (int32_t)Sa[g], nprob[g], 0});
+            }
+        }
+        // longest items first within each launch
+        std::stable_sort(skproj.begin(), skproj.end(), [](const SkelProj &a, const SkelProj &b) {
+            return (int64_t)a.nsrc * a.ntgt > (int64_t)b.nsrc * b.ntgt;
+        });
+        for (auto &lv : bylev) {
+            std::stable_sort(lv.begin(), lv.end(),
+                             [](const SkelExpand &a, const SkelExpand &b) { return (int64_t)a.nc * a.h > (int64_t)b.nc * b.h; });
+            skex.insert(skex.end(), lv.begin(), lv.end());
+            op.skex_lvl.push_back((int64_t)skex.size());
+        }
+    }
+
     mark("layout");
     if (op.admit) {
         double bytes = 8.0 * (double)(op.n_np + op.n_up + op.n_vp);
         if (op.prec_flags & HODLR_PREC_F32) bytes *= 1.5;
+        bytes += 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk;
         op.admit(bytes);
         mark("admission");
     }
@@ -604,6 +826,8 @@ void assemble(hodlr_op &op) {
     op.UP64.alloc(sizeof(double) * op.n_up);
     op.VP64.alloc(sizeof(double) * op.n_vp);
     HIP_CHECK(hipMemsetAsync(op.VP64.p, 0, sizeof(double) * op.n_vp, s));  // padding columns
+    op.SK.alloc(sizeof(int32_t) * (size_t)(2 * op.n_sk));
+    HIP_CHECK(hipMemsetAsync(op.SK.p, 0xff, sizeof(int32_t) * (size_t)(2 * op.n_sk), s));  // -1: unwritten
     DevBuf d_near;
     upload(d_near, near, s, true);
     if (op.fill_from) {
@@ -634,6 +858,52 @@ void assemble(hodlr_op &op) {
                 throw ArgError{HODLR_E_STATE, "ACA fill pass disagrees with the rank pass at block " +
                                                   std::to_string(b)};
     }
+    if (op.n_skel) {
+        // pivot positions (written by the fill pass or read from a file) must
+        // lie in their block's problem before any kernel dereferences them
+        std::vector<int32_t> skh((size_t)(2 * op.n_sk));
+        if (op.n_sk) {
+            HIP_CHECK(hipMemcpyAsync(skh.data(), op.SK.p, sizeof(int32_t) * skh.size(), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
+        std::vector<int64_t> skb;
+        for (int64_t b = 0; b < nb; b++)
+            if (skel[b] && op.rank[b] > 0) skb.push_back(b);
+        for (int64_t b : skb) {
+            const int p = op.b_prob[b];
+            const int32_t *ri = skh.data() + skr_off[b];
+            const int32_t *ci = skh.data() + op.n_sk + skc_off[b];
+            for (int i = 0; i < op.rank[b]; i++)
+                if (ri[i] < op.pos_base[p] || ri[i] >= op.pos_base[p + 1] || ci[i] < op.pos_base[p] ||
+                    ci[i] >= op.pos_base[p + 1])
+                    throw ArgError{HODLR_E_STATE, "skeleton pivot position out of range at block " + std::to_string(b)};
+        }
+        if (!op.fill_from) {  // the cores C = A(pivot rows, pivot columns) and G = C^+
+            std::vector<double> Gh((size_t)op.n_g);
+            parallel_for(
+                (int64_t)skb.size(),
+                [&](int64_t q0, int64_t q1) {
+                    std::vector<double> Cm;
+                    for (int64_t q = q0; q < q1; q++) {
+                        const int64_t b = skb[q];
+                        const int r = op.rank[b], p = op.b_prob[b];
+                        const hodlr_tree &t = *op.trees[p];
+                        const int64_t pb = op.pos_base[p];
+                        const int32_t *ri = skh.data() + skr_off[b];
+                        const int32_t *ci = skh.data() + op.n_sk + skc_off[b];
+                        const double *Xp = X.data() + xoff[p];
+                        const double *Wp = woff[p] >= 0 ? W.data() + woff[p] : nullptr;
+                        Cm.resize((size_t)r * r);
+                        for (int k = 0; k < r; k++)
+                            for (int i = 0; i < r; i++)
+                                Cm[(size_t)k * r + i] = host_entry(op.kds[p], t.ndim, Xp, Wp, ri[i] - pb, ci[k] - pb);
+                        core_pinv(r, Cm, Gh.data() + g_off[b]);
+                    }
+                },
+                16);
+            upload(op.GA, Gh, s);
+        }
+    }
     pool.release();
     if (op.prec_flags & HODLR_PREC_F32) {
         op.NP32.alloc(sizeof(float) * op.n_np);
@@ -653,6 +923,13 @@ void assemble(hodlr_op &op) {
     upload(op.panel, panel, s);
     upload(op.segs, segs, s);
     upload(op.tidx, tidx, s);
+    upload(op.skproj, skproj, s);
+    upload(op.skred, skred, s);
+    upload(op.skcore, skcore, s);
+    upload(op.skex, skex, s);
+    op.n_skproj = (int64_t)skproj.size();
+    op.n_skred = (int64_t)skred.size();
+    op.n_skcore = (int64_t)skcore.size();
     op.n_proj = (int64_t)proj.size();
     op.n_red = (int64_t)red.size();
     op.n_panel = (int64_t)panel.size();
@@ -680,15 +957,20 @@ void assemble(hodlr_op &op) {
         const int64_t rr = dn ? mn : op.rank[b];
         st.n_flop_ref += (double)rr * (double)(op.b_m[b] + op.b_n[b]);
         st.max_rank = std::max<int64_t>(st.max_rank, rr);
-        st.e_far += dn ? (double)op.b_m[b] * op.b_n[b] : (double)op.rank[b] * (op.b_m[b] + op.b_n[b]);
+        if (!skel[b])
+            st.e_far += dn ? (double)op.b_m[b] * op.b_n[b] : (double)op.rank[b] * (op.b_m[b] + op.b_n[b]);
         st.n_dense += dn;
     }
     st.n_blocks = nb;
     op.e_far = st.e_far;
-    st.bytes_f64 = 8.0 * (double)(op.n_np + op.n_up + op.n_vp);
+    st.n_skeleton = op.n_skel;
+    st.e_skel = op.e_skel;
+    // cores (FP64, shared by both precisions) and pivot positions
+    st.bytes_core = 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk;
+    st.bytes_f64 = 8.0 * (double)(op.n_np + op.n_up + op.n_vp) + st.bytes_core;
     st.bytes_f32 = (op.prec_flags & HODLR_PREC_F32) ? 4.0 * (double)(op.n_np + op.n_up + op.n_vp) : 0.0;
-    st.matvec_bytes_f64 = 8.0 * (st.e_near + st.e_far) + 2.0 * 8.0 * (double)N;
-    st.matvec_bytes_f32 = 4.0 * (st.e_near + st.e_far) + 2.0 * 4.0 * (double)N;
+    st.matvec_bytes_f64 = 8.0 * (st.e_near + st.e_far) + st.bytes_core + 2.0 * 8.0 * (double)N;
+    st.matvec_bytes_f32 = 4.0 * (st.e_near + st.e_far) + st.bytes_core + 2.0 * 4.0 * (double)N;
     st.rank_seconds = t_rank1 - t_rank0;
     st.fill_seconds = t_end - t_fill0;
     st.assemble_seconds = t_end - t_start;
@@ -698,6 +980,12 @@ void assemble(hodlr_op &op) {
 
 void hodlr_op::ensure_work(int prec, int nrhs) {
     using namespace hodlr;
+    if (n_skcore && cap_rhs_sk < nrhs) {
+        skt1.alloc(sizeof(double) * (size_t)n_sk * nrhs);
+        skt2.alloc(sizeof(double) * (size_t)n_sk * nrhs);
+        skpart.alloc(sizeof(double) * (size_t)(n_skslots > 0 ? n_skslots : 1) * nrhs);
+        cap_rhs_sk = nrhs;
+    }
     if (prec == HODLR_PREC_F64) {
         if (cap_rhs64 >= nrhs) return;
         t64.alloc(sizeof(double) * (size_t)(n_t > 0 ? n_t : 1) * nrhs);
@@ -718,14 +1006,16 @@ hodlr_op::~hodlr_op() {
 void hodlr_op::profile_collect() {
     using namespace hodlr;
     if (ev_pending.empty()) return;
-    HIP_CHECK(hipEventSynchronize(ev_pool[ev_pending.back().second + 2]));
+    HIP_CHECK(hipEventSynchronize(ev_pool[ev_pending.back().second + 3]));
     for (auto &pe : ev_pending) {
-        float a = 0, b = 0;
+        float a = 0, b = 0, c = 0;
         HIP_CHECK(hipEventElapsedTime(&a, ev_pool[pe.second], ev_pool[pe.second + 1]));
         HIP_CHECK(hipEventElapsedTime(&b, ev_pool[pe.second + 1], ev_pool[pe.second + 2]));
+        HIP_CHECK(hipEventElapsedTime(&c, ev_pool[pe.second + 2], ev_pool[pe.second + 3]));
         const int p = pe.first == HODLR_PREC_F64 ? 0 : 1;
         prof_ms[p][0] += a;
         prof_ms[p][1] += b;
+        prof_ms[p][2] += c;
         prof_count[p]++;
     }
     ev_pending.clear();
@@ -736,13 +1026,13 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
     using namespace hodlr;
     ensure_work(prec, nrhs);
     if (profiling && !ev) {
-        const size_t need = 3 * (ev_pending.size() + 1);
+        const size_t need = 4 * (ev_pending.size() + 1);
         while (ev_pool.size() < need) {
             hipEvent_t e;
             HIP_CHECK(hipEventCreate(&e));
             ev_pool.push_back(e);
         }
-        const size_t first = 3 * ev_pending.size();
+        const size_t first = 4 * ev_pending.size();
         ev_pending.push_back({prec, first});
         ev = &ev_pool[first];
     }
@@ -753,6 +1043,10 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
                                         UP64.as<double>(), static_cast<const double *>(x), t64.as<double>(),
                                         part64.as<double>(), tidx.as<int64_t>(), static_cast<double *>(y),
                                         variant_for(prec, nrhs), ev, s));
+        if (n_skcore)
+            HIP_CHECK(launch_skeleton<double>(nrhs, skel_plan(), static_cast<const double *>(x), skt1.as<double>(),
+                                              skpart.as<double>(), skt2.as<double>(), static_cast<double *>(y), s));
+        if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     } else {
         if (!(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
         HIP_CHECK(launch_matvec<float>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
@@ -761,10 +1055,35 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
                                        static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
                                        tidx.as<int64_t>(), static_cast<float *>(y), variant_for(prec, nrhs), ev,
                                        s));
+        if (n_skcore)
+            HIP_CHECK(launch_skeleton<float>(nrhs, skel_plan(), static_cast<const float *>(x), skt1.as<double>(),
+                                             skpart.as<double>(), skt2.as<double>(), static_cast<float *>(y), s));
+        if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
         if (add_diag)
             HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x),
                                           diag.as<double>(), N, nrhs, s));
     }
+}
+
+hodlr::SkeletonPlan hodlr_op::skel_plan() const {
+    using namespace hodlr;
+    SkeletonPlan p;
+    p.proj = skproj.as<SkelProj>();
+    p.red = skred.as<SkelReduce>();
+    p.core = skcore.as<SkelCore>();
+    p.ex = skex.as<SkelExpand>();
+    p.nproj = n_skproj;
+    p.nred = n_skred;
+    p.ncore = n_skcore;
+    p.ex_lvl = skex_lvl.data();
+    p.nlvl = (int)skex_lvl.size() - 1;
+    p.K = kdev.as<KernelDev>();
+    p.SK = SK.as<int32_t>();
+    p.nrow_slots = n_sk;
+    p.G = GA.as<double>();
+    p.max_ndim = 1;
+    for (auto *t : trees) p.max_ndim = std::max(p.max_ndim, t->ndim);
+    return p;
 }
 
 int hodlr_op::variant_for(int prec, int nrhs) const {
@@ -798,13 +1117,13 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
             for (int v : cands) {
                 mv_variant[pi][lg] = v;
-                hipEvent_t ev[3];
+                hipEvent_t ev[4];
                 for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
                 matvec_dev(prec, k, dx.p, dy.p, s);  // warm-up
                 double ph[2] = {0, 0};
                 for (int r = 0; r < reps; r++) {
                     matvec_dev(prec, k, dx.p, dy.p, s, ev);
-                    HIP_CHECK(hipEventSynchronize(ev[2]));
+                    HIP_CHECK(hipEventSynchronize(ev[3]));
                     float a = 0, b = 0;
                     HIP_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
                     HIP_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));

@@ -13,7 +13,7 @@ namespace hodlr {
 hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,
                       double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
                       const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
-                      hipStream_t s);
+                      int32_t *SK, hipStream_t s);
 hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev *K,
                              const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, hipStream_t s);
 hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev *K,
@@ -32,6 +32,12 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
                          const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
                          T *t, T *part, const int64_t *tidx, T *y, int variant, hipEvent_t *ev,
                          hipStream_t s);
+
+// skeleton blocks' on-the-fly product, added into y (t1, t2: FP64 slot
+// values, part: partial projections; all nrhs-interleaved)
+template <typename T>
+hipError_t launch_skeleton(int nrhs, const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2, T *y,
+                           hipStream_t s);
 
 hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const Chunk *chunks, int nchunk,
                         double *partial, const int32_t *pchunk, int nprob, int K, double *out, hipStream_t s);
@@ -103,24 +109,41 @@ struct hodlr_op {
     int cap_rhs64 = 0, cap_rhs32 = 0;
     hodlr::DevBuf t64, part64, x64, y64, t32, part32, x32, y32;
 
+    // skeleton blocks (LowRank_matrix.hpp:22-36): compressed blocks whose
+    // column node holds more than sys_size points keep only their pivot
+    // positions (SK: n_sk row slots, then n_sk column slots) and the
+    // pseudo-inverse of their core (GA, n_g FP64 entries); their entries are
+    // evaluated in every matvec.  sys_size < 0: none.
+    int sys_size = -1;
+    int64_t n_skel = 0, n_sk = 0, n_g = 0, n_skslots = 0;
+    hodlr::DevBuf SK, GA, skproj, skred, skcore, skex;
+    int64_t n_skproj = 0, n_skred = 0, n_skcore = 0;
+    std::vector<int64_t> skex_lvl;
+    double e_skel = 0;
+    int cap_rhs_sk = 0;
+    hodlr::DevBuf skt1, skt2, skpart;
+    hodlr::SkeletonPlan skel_plan() const;
+
     hodlr_op_stats stats{};
 
     // solver state kept between solves (Krylov basis, mixed-mode vectors)
     std::shared_ptr<void> solver_ws;
     std::shared_ptr<void> batch_ws;
 
-    // live kernel timing: event triplets recorded around the projection and
-    // panel launches of every matvec while profiling is on
+    // live kernel timing: four events per matvec (before the projection,
+    // the panel product, the skeleton product, and after it) recorded on
+    // its stream while profiling is on
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, size_t>> ev_pending;  // (precision, first event index)
-    double prof_ms[2][2] = {{0, 0}, {0, 0}};         // [prec f64/f32][proj/panel]
+    double prof_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};   // [prec f64/f32][proj/panel/skeleton]
     int64_t prof_count[2] = {0, 0};
     void profile_collect();
     ~hodlr_op();
 
     // FP32: with add_diag false the result omits the diagonal (the FP32
-    // copy stores the self blocks with a zero diagonal; callers add diag x)
+    // copy stores the self blocks with a zero diagonal; callers add diag x).
+    // ev (nullable): four events, see profiling
     void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr,
                     bool add_diag = true);
     void ensure_work(int prec, int nrhs);

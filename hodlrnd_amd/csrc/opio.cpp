@@ -5,14 +5,16 @@
 // file lets a compressed operator outlive the process (a restarted server
 // reloads its resident operators without repeating ACA).  Layout, all
 // little-endian:
-//   "HODLROP1"  u32 version (1)  u32 0x01020304
-//   i32 P  f64 eps_aca  i32 0
+//   "HODLROP1"  u32 version (2)  u32 0x01020304
+//   i32 P  f64 eps_aca  i32 sys_size (-1: every factor stored; 0 in version 1)
 //   per problem: i32 ndim  i32 nmax  i32 dprime  i32 family  i64 n
 //                f64 scale  f64 diag  i32 has_weights  i32 0
 //                f64 lo[ndim]  f64 hi[ndim]  f64 points[n * ndim] (point order)
 //                f64 weights[n] (point order, if has_weights)
 //   i64 nblocks  i32 rank[nblocks]  i32 flags[nblocks]
 //   i64 n_np  i64 n_up  i64 n_vp  f64 NP[n_np]  f64 UP[n_up]  f64 VP[n_vp]
+//   if sys_size >= 0 (skeleton blocks):
+//     i64 n_sk  i64 n_g  i32 SK[2 n_sk] (pivot tree positions)  f64 G[n_g] (cores)
 //   u64 FNV-1a hash (8-byte words, per write) of everything above
 // The trees and block lists are rebuilt from the points (the tree and list
 // construction is deterministic), the stored ranks replace the ACA rank
@@ -28,7 +30,7 @@ namespace hodlr {
 namespace {
 
 constexpr char kMagic[8] = {'H', 'O', 'D', 'L', 'R', 'O', 'P', '1'};
-constexpr uint32_t kVersion = 1, kEndian = 0x01020304u;
+constexpr uint32_t kVersion = 2, kEndian = 0x01020304u;
 constexpr size_t kChunk = (size_t)1 << 26;  // bytes per device <-> file transfer
 
 // FNV-1a over 8-byte little-endian words (a zero-padded tail word), chained
@@ -108,7 +110,7 @@ void save_op(hodlr_op &op, const char *path) {
     const int P = (int)op.trees.size();
     w.put<int32_t>(P);
     w.put<double>(op.eps);
-    w.put<int32_t>(0);
+    w.put<int32_t>(op.sys_size);
     std::vector<double> wt;
     for (int p = 0; p < P; p++) {
         const hodlr_tree &t = *op.trees[p];
@@ -145,6 +147,12 @@ void save_op(hodlr_op &op, const char *path) {
     w.device(op.NP64.p, sizeof(double) * op.n_np, buf);
     w.device(op.UP64.p, sizeof(double) * op.n_up, buf);
     w.device(op.VP64.p, sizeof(double) * op.n_vp, buf);
+    if (op.sys_size >= 0) {
+        w.put<int64_t>(op.n_sk);
+        w.put<int64_t>(op.n_g);
+        w.device(op.SK.p, sizeof(int32_t) * 2 * op.n_sk, buf);
+        w.device(op.GA.p, sizeof(double) * op.n_g, buf);
+    }
     const uint64_t h = w.h;
     w.put<uint64_t>(h);
     w.f.close();
@@ -156,16 +164,19 @@ hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags) {
     char magic[8];
     r.raw(magic, 8);
     if (std::memcmp(magic, kMagic, 8) != 0) throw ArgError{HODLR_E_ARG, "not an operator file"};
-    if (r.get<uint32_t>() != kVersion) throw ArgError{HODLR_E_ARG, "unsupported operator file version"};
+    const uint32_t version = r.get<uint32_t>();
+    if (version != 1 && version != kVersion) throw ArgError{HODLR_E_ARG, "unsupported operator file version"};
     if (r.get<uint32_t>() != kEndian) throw ArgError{HODLR_E_ARG, "operator file has the wrong byte order"};
     const int P = r.get<int32_t>();
     const double eps = r.get<double>();
-    (void)r.get<int32_t>();
+    const int32_t sys_field = r.get<int32_t>();
+    const int sys_size = version == 1 ? -1 : (sys_field < 0 ? -1 : sys_field);
     if (P < 1 || P > (1 << 24) || !(eps > 0 && eps < 1)) throw ArgError{HODLR_E_ARG, "corrupt operator header"};
     if (ctx->device < 0) throw ArgError{HODLR_E_STATE, "host-only context"};
     auto op = std::make_unique<hodlr_op>();
     op->ctx = ctx;
     op->eps = eps;
+    op->sys_size = sys_size;
     op->prec_flags = prec_flags | HODLR_PREC_F64;
     std::vector<std::vector<double>> weights(P);
     for (int p = 0; p < P; p++) {
@@ -218,6 +229,14 @@ hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags) {
         r.device(o.NP64.p, sizeof(double) * n_np, buf, s);
         r.device(o.UP64.p, sizeof(double) * n_up, buf, s);
         r.device(o.VP64.p, sizeof(double) * n_vp, buf, s);
+        if (o.sys_size >= 0) {
+            const int64_t n_sk = r.get<int64_t>(), n_g = r.get<int64_t>();
+            if (n_sk != o.n_sk || n_g != o.n_g)
+                throw ArgError{HODLR_E_ARG, "stored skeleton sizes do not match the rebuilt layout"};
+            r.device(o.SK.p, sizeof(int32_t) * 2 * n_sk, buf, s);
+            o.GA.alloc(sizeof(double) * (size_t)n_g);
+            r.device(o.GA.p, sizeof(double) * n_g, buf, s);
+        }
         const uint64_t want = r.h;
         if (r.get<uint64_t>() != want) throw ArgError{HODLR_E_ARG, "operator file checksum mismatch"};
     };

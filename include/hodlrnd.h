@@ -93,6 +93,9 @@ typedef struct {
     double assemble_seconds;
     double rank_seconds;   /* ACA rank pass */
     double fill_seconds;   /* ACA fill pass + near field */
+    int64_t n_skeleton;    /* skeleton blocks (hodlr_op_assemble_skeleton) */
+    double e_skel;         /* entries of skeleton blocks evaluated per matvec: sum r (m + n) */
+    double bytes_core;     /* HBM bytes of the skeleton cores and pivot positions */
 } hodlr_op_stats;
 
 typedef struct {
@@ -166,6 +169,20 @@ int hodlr_op_assemble_batch(hodlr_ctx *ctx, int nprob, hodlr_tree *const *trees,
                             const hodlr_kernel_desc *k, double eps_aca, int prec_flags,
                             hodlr_op **out);
 int hodlr_op_problem_offsets(hodlr_op *op, int64_t *offsets, int cap);
+/* LowRank_matrix.hpp:22-36 (SYS_SIZE < |cols| selects ACA_MEM_EFF),
+ * :55-58 and :685-698: hodlr_op_assemble_batch (nprob = 1 for one problem)
+ * in the reference's memory-efficient form.  A compressed block whose
+ * column node holds more than sys_size points keeps no factors: only the
+ * tree positions of its r pivot rows Is and columns Js and G = C^+, the
+ * pseudo-inverse of its core C = A(Is, Js) (column-pivoted QR, rank cut at
+ * eps r |R_00|).  Every matvec evaluates A(Is, J) and A(I, Js) again:
+ * y_I += A(I, Js) G A(Is, J) x_J, in FP64 for both precisions.  The
+ * reference uses sys_size = 2 Nmax (myHeaders.hpp:35-48); sys_size < 0
+ * stores every factor, as hodlr_op_assemble_batch does.  Ranks, dense
+ * fallbacks and the near field are those of the stored form. */
+int hodlr_op_assemble_skeleton(hodlr_ctx *ctx, int nprob, hodlr_tree *const *trees,
+                               const hodlr_kernel_desc *k, double eps_aca, int sys_size,
+                               int prec_flags, hodlr_op **out);
 /* HDD_matrix.hpp:69-85 */
 int hodlr_op_get_stats(hodlr_op *op, hodlr_op_stats *s);
 /* ranks and dense flags in block order (level, node, interaction index) */
@@ -246,7 +263,8 @@ int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);
 /* ---- measurement ---- */
 /* Times `reps` back-to-back matvecs on the op's stream with HIP events per
  * kernel; ms[0] = whole matvec, ms[1] = projection kernel (V^T x),
- * ms[2] = panel kernel (near field + U t), averages per launch. */
+ * ms[2] = panel kernel (near field + U t), ms[3] = skeleton blocks' on-the-fly
+ * product (0 without skeleton blocks), averages per launch. */
 int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms);
 /* Live timing: while enabled, every matvec records HIP events around its
  * projection and panel launches on the stream it runs on.  read returns the
