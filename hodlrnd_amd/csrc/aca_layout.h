@@ -117,12 +117,13 @@ struct Segment {
 // row node).  SK holds the tree position of every row slot, then of every
 // column slot; t1 / t2 hold one value per row / column slot and RHS.
 constexpr int kSkelChunk = 1024;  // sources of one projection work item
+constexpr int kSkelTile = 128;    // targets of one work item (two per lane)
 
 // t1[t0 + l] (+)= sum over sources j in [src0, src0 + nsrc) of A(SK[t0 + l], j) x_j
 struct SkelProj {
-    int64_t t0;           // first row slot (<= 64 targets, one per lane)
+    int64_t t0;           // first row slot (<= kSkelTile targets)
     int64_t src0;         // first source tree position (the column node's points)
-    int64_t out;          // >= 0: first partial slot (64 per item), < 0: direct into t1
+    int64_t out;          // >= 0: first partial slot (kSkelTile per item), < 0: direct into t1
     int32_t ntgt, nsrc;
     int32_t prob, pad;
 };
@@ -143,28 +144,34 @@ struct SkelCore {
 // y[y0 + l] += sum over column slots q in [c0, c0 + nc) of A(y0 + l, SKC[q]) t2[q]
 struct SkelExpand {
     int64_t y0, c0;
-    int32_t h, nc;        // rows (<= 64) and column slots
+    int32_t h, nc;        // rows (<= kSkelTile) and column slots
     int32_t prob, pad;
 };
 
 struct KernelDev;
 
+// one launch of a specialised proj / expand kernel: items [begin, end) share
+// the kernel family and NDIM (key = 16 family + ndim); expand launches are
+// ordered by tree level, so each row has one writer per launch
+struct SkelLaunch {
+    int64_t begin, end;
+    int32_t key, level;
+};
+
 // the skeleton product's work lists (device pointers into the operator's
-// buffers; ex_lvl is a host array: expand items sorted by tree level,
-// level l at [ex_lvl[l], ex_lvl[l + 1]))
+// buffers; the launch lists are host arrays)
 struct SkeletonPlan {
     const SkelProj *proj;
     const SkelReduce *red;
     const SkelCore *core;
     const SkelExpand *ex;
-    int64_t nproj, nred, ncore;
-    const int64_t *ex_lvl;
-    int nlvl;
+    int64_t nred, ncore;
+    const SkelLaunch *proj_l, *ex_l;
+    int nproj_l, nex_l;
     const KernelDev *K;
     const int32_t *SK;    // row slots, then column slots at SK + nrow_slots
     int64_t nrow_slots;
     const double *G;
-    int max_ndim;         // largest NDIM of the problems (sizes the LDS source tiles)
 };
 
 // rows of a batched vector that belong to one problem (never crosses a

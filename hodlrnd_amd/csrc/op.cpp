@@ -748,13 +748,15 @@ void assemble(hodlr_op &op) {
     if (nvcol > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many V columns for 32-bit tile offsets"};
     op.n_slots = slots;
 
-    // skeleton work: projection items per column node (64 pivot rows x up
-    // to kSkelChunk sources), expand items per row node (64 rows), by level
+    // skeleton work: projection items per column node (kSkelTile pivot rows
+    // x up to kSkelChunk sources), expand items per row node (kSkelTile
+    // rows), by level
     std::vector<SkelProj> skproj;
     std::vector<SkelReduce> skred;
     std::vector<SkelExpand> skex;
     op.n_skslots = 0;
-    op.skex_lvl.assign(1, 0);
+    op.skproj_l.clear();
+    op.skex_l.clear();
     if (op.n_skel) {
         std::vector<int32_t> nprob(nn, 0), nlev(nn, 0);
         int nlevels = 0;
@@ -772,40 +774,60 @@ void assemble(hodlr_op &op) {
             if (Sc[g]) {
                 const int64_t ns = count[g];
                 const int64_t nch = (ns + kSkelChunk - 1) / kSkelChunk;
-                for (int64_t q0 = 0; q0 < Sc[g]; q0 += kWave) {
-                    const int nt = (int)std::min<int64_t>(kWave, Sc[g] - q0);
+                for (int64_t q0 = 0; q0 < Sc[g]; q0 += kSkelTile) {
+                    const int nt = (int)std::min<int64_t>(kSkelTile, Sc[g] - q0);
                     if (nch > 1) skred.push_back(SkelReduce{sc_base[g] + q0, op.n_skslots, nt, (int32_t)nch});
                     for (int64_t ch = 0; ch < nch; ch++) {
                         SkelProj it;
                         it.t0 = sc_base[g] + q0;
                         it.src0 = op.node_start[g] + ch * kSkelChunk;
-                        it.out = nch > 1 ? op.n_skslots + ch * kWave : -1;
+                        it.out = nch > 1 ? op.n_skslots + ch * kSkelTile : -1;
                         it.ntgt = nt;
                         it.nsrc = (int32_t)std::min<int64_t>(kSkelChunk, ns - ch * kSkelChunk);
                         it.prob = nprob[g];
                         it.pad = 0;
                         skproj.push_back(it);
                     }
-                    if (nch > 1) op.n_skslots += nch * kWave;
+                    if (nch > 1) op.n_skslots += nch * kSkelTile;
                 }
             }
             if (Sa[g]) {
                 if (Sa[g] > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many skeleton columns in one node"};
-                for (int64_t r0 = 0; r0 < count[g]; r0 += kWave)
+                for (int64_t r0 = 0; r0 < count[g]; r0 += kSkelTile)
                     bylev[nlev[g]].push_back(SkelExpand{op.node_start[g] + r0, sa_base[g],
-                                                        (int32_t)std::min<int64_t>(kWave, count[g] - r0),
+                                                        (int32_t)std::min<int64_t>(kSkelTile, count[g] - r0),
                                                         (int32_t)Sa[g], nprob[g], 0});
             }
         }
-        // longest items first within each launch
-        std::stable_sort(skproj.begin(), skproj.end(), [](const SkelProj &a, const SkelProj &b) {
+        // one launch per (kernel family, NDIM) group -- per tree level as well
+        // for expand items -- with the longest items first
+        auto key = [&](int p) { return 16 * op.kds[p].family + op.trees[p]->ndim; };
+        std::stable_sort(skproj.begin(), skproj.end(), [&](const SkelProj &a, const SkelProj &b) {
+            const int ka = key(a.prob), kb = key(b.prob);
+            if (ka != kb) return ka < kb;
             return (int64_t)a.nsrc * a.ntgt > (int64_t)b.nsrc * b.ntgt;
         });
-        for (auto &lv : bylev) {
-            std::stable_sort(lv.begin(), lv.end(),
-                             [](const SkelExpand &a, const SkelExpand &b) { return (int64_t)a.nc * a.h > (int64_t)b.nc * b.h; });
+        for (size_t i = 0; i < skproj.size();) {
+            size_t j = i;
+            while (j < skproj.size() && key(skproj[j].prob) == key(skproj[i].prob)) j++;
+            op.skproj_l.push_back(SkelLaunch{(int64_t)i, (int64_t)j, key(skproj[i].prob), 0});
+            i = j;
+        }
+        for (int l = 0; l < nlevels; l++) {
+            auto &lv = bylev[l];
+            std::stable_sort(lv.begin(), lv.end(), [&](const SkelExpand &a, const SkelExpand &b) {
+                const int ka = key(a.prob), kb = key(b.prob);
+                if (ka != kb) return ka < kb;
+                return (int64_t)a.nc * a.h > (int64_t)b.nc * b.h;
+            });
+            for (size_t i = 0; i < lv.size();) {
+                size_t j = i;
+                while (j < lv.size() && key(lv[j].prob) == key(lv[i].prob)) j++;
+                op.skex_l.push_back(SkelLaunch{(int64_t)(skex.size() + i), (int64_t)(skex.size() + j),
+                                               key(lv[i].prob), l});
+                i = j;
+            }
             skex.insert(skex.end(), lv.begin(), lv.end());
-            op.skex_lvl.push_back((int64_t)skex.size());
         }
     }
 
@@ -1072,17 +1094,16 @@ hodlr::SkeletonPlan hodlr_op::skel_plan() const {
     p.red = skred.as<SkelReduce>();
     p.core = skcore.as<SkelCore>();
     p.ex = skex.as<SkelExpand>();
-    p.nproj = n_skproj;
     p.nred = n_skred;
     p.ncore = n_skcore;
-    p.ex_lvl = skex_lvl.data();
-    p.nlvl = (int)skex_lvl.size() - 1;
+    p.proj_l = skproj_l.data();
+    p.nproj_l = (int)skproj_l.size();
+    p.ex_l = skex_l.data();
+    p.nex_l = (int)skex_l.size();
     p.K = kdev.as<KernelDev>();
     p.SK = SK.as<int32_t>();
     p.nrow_slots = n_sk;
     p.G = GA.as<double>();
-    p.max_ndim = 1;
-    for (auto *t : trees) p.max_ndim = std::max(p.max_ndim, t->ndim);
     return p;
 }
 

@@ -118,7 +118,7 @@ struct hodlr_op {
     int64_t n_skel = 0, n_sk = 0, n_g = 0, n_skslots = 0;
     hodlr::DevBuf SK, GA, skproj, skred, skcore, skex;
     int64_t n_skproj = 0, n_skred = 0, n_skcore = 0;
-    std::vector<int64_t> skex_lvl;
+    std::vector<hodlr::SkelLaunch> skproj_l, skex_l;
     double e_skel = 0;
     int cap_rhs_sk = 0;
     hodlr::DevBuf skt1, skt2, skpart;

@@ -9,14 +9,14 @@
 //     y_I += A(I, Js) G A(Is, J) x_J
 //
 // in three phases:
-//   proj    t1 = A(Is, J) x_J   per column node J: one lane per pivot row of
-//                                every skeleton block (., J), the node's
+//   proj    t1 = A(Is, J) x_J   per column node J: two lanes' worth of pivot
+//                                rows of every skeleton block (., J), the node's
 //                                points streamed through LDS as sources;
 //                                nodes longer than kSkelChunk points are cut
 //                                into items whose partial sums are added in
 //                                a fixed order (deterministic, no atomics)
 //   core    t2 = G t1           one wave per block, lanes over G's rows
-//   expand  y_I += A(I, Js) t2  per row node I: one lane per row, the pivot
+//   expand  y_I += A(I, Js) t2  per row node I: two rows per lane, the pivot
 //                                columns of every skeleton block (I, .) as
 //                                sources; one launch per tree level, so each
 //                                row has one writer per launch
@@ -36,57 +36,129 @@ namespace hodlr {
 
 namespace {
 
-// 1/sqrt(s): the hardware estimate and NEWTON Newton steps (the estimate
-// has about 23 good bits: one step gives ~1e-14, two give ~1 ulp)
-template <int NEWTON>
-__device__ __forceinline__ double rsqrt_nr(double s) {
-    double y = __builtin_amdgcn_rsq(s);
-#pragma unroll
-    for (int it = 0; it < NEWTON; it++) {
-        const double e = fma(-(s * y), y, 1.0);  // 1 - s y^2
-        y = fma(0.5 * y, e, y);
-    }
-    return y;
+// 1/sqrt(s) from the hardware estimate y (about 23 good bits) and one
+// refinement with e = 1 - s y^2: the cubic step y (1 + e/2 + 3 e^2/8)
+// (error 5 e^3 / 16, below FP64 rounding) for an FP64 operator, Newton's
+// y (1 + e/2) (~1e-14) for the FP32 operator
+template <bool CUBIC>
+__device__ __forceinline__ double rsqrt_ref(double s) {
+    const double y = __builtin_amdgcn_rsq(s);
+    const double e = fma(-(s * y), y, 1.0);
+    if constexpr (CUBIC) return fma(y * e, fma(e, 0.375, 0.5), y);
+    else return fma(0.5 * y, e, y);
 }
 
 // f(r) of the kernel family from s = r^2 (user_kernel.hpp:46-49,101-104,167-170)
-template <int FAM, int NEWTON>
+template <int FAM, bool CUBIC>
 __device__ __forceinline__ double kfun(double s) {
-    if constexpr (FAM == 1) return rsqrt_nr<NEWTON>(s);          // 1 / r
+    if constexpr (FAM == 1) return rsqrt_ref<CUBIC>(s);          // 1 / r
     else if constexpr (FAM == 0) return 0.5 * log(s);            // log r
-    else return exp(-(s * rsqrt_nr<NEWTON>(s)));                 // exp(-r)
+    else return exp(-(s * rsqrt_ref<CUBIC>(s)));                 // exp(-r)
 }
 
-// FP64 results keep every bit of 1/r; the FP32 operator's results need ~1e-14
 template <typename T>
-constexpr int kNewton = sizeof(T) == 8 ? 2 : 1;
+constexpr bool kCubic = sizeof(T) == 8;
 
-// Both on-the-fly phases are one sum per lane over wave-uniform sources,
-// acc_lane += f(|a_lane - b_s|^2) v_s: the sources are staged 64 at a time
-// into LDS (lane l loads source l: coalesced), then every lane reads each
-// source by a broadcast LDS read.  Source record: its ND coordinates then
-// its NR values, S = ND + NR doubles.  One wave per workgroup, so the
-// barriers around a tile only order this wave's LDS traffic.
-template <int NR, int ND, int FAM, int NEWTON>
-__device__ __forceinline__ void tile_sum(const double *__restrict__ lds, int S, int cnt, const double *a, int nd,
-                                         double *acc) {
+// Both on-the-fly phases are sums over sources for up to 128 targets per
+// wave, two per lane when there are more than 64 (lane l owns targets l and
+// l + 64), else one:
+//     acc_t += f(|a_t - b_s|^2) v_s.
+// The sources are staged 64 at a time into LDS (lane l loads source l:
+// coalesced), then every lane reads each source by a broadcast LDS read that
+// feeds both of its targets.  Source record: ND coordinates then NR values,
+// S = ND + NR doubles.  One wave per workgroup, so the barriers around a
+// tile only order this wave's LDS traffic.
+template <int NR, int ND, int FAM, bool CUBIC, int TPL>
+__device__ __forceinline__ void tile_sum(const double *__restrict__ lds, int S, int cnt, const double *a0,
+                                         const double *a1, int nd, double *acc0, double *acc1) {
     constexpr int MD = ND > 0 ? ND : kMaxDimDev;
 #pragma unroll 2
     for (int s = 0; s < cnt; s++) {
         const double *src = lds + s * S;
-        double d2 = 0.0;
+        double d0 = 0.0, d1 = 0.0;
 #pragma unroll
         for (int d = 0; d < MD; d++)
             if (d < nd) {
-                const double t = a[d] - src[d];
-                d2 = fma(t, t, d2);
+                const double b = src[d];
+                const double t0 = a0[d] - b;
+                d0 = fma(t0, t0, d0);
+                if constexpr (TPL == 2) {
+                    const double t1 = a1[d] - b;
+                    d1 = fma(t1, t1, d1);
+                }
             }
-        const double f = kfun<FAM, NEWTON>(d2);
+        const double f0 = kfun<FAM, CUBIC>(d0);
         const double *v = src + (ND > 0 ? ND : nd);
+        if constexpr (TPL == 2) {
+            const double f1 = kfun<FAM, CUBIC>(d1);
 #pragma unroll
-        for (int c = 0; c < NR; c++) acc[c] = fma(f, v[c], acc[c]);
+            for (int c = 0; c < NR; c++) {
+                const double vc = v[c];
+                acc0[c] = fma(f0, vc, acc0[c]);
+                acc1[c] = fma(f1, vc, acc1[c]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NR; c++) acc0[c] = fma(f0, v[c], acc0[c]);
+        }
     }
 }
+
+// Loads through the global address space: a generic (flat) load also counts
+// against the LDS wait counter, so waiting for the tile's LDS reads would wait
+// for an in-flight prefetch as well.
+template <class T>
+__device__ __forceinline__ T gload(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+// The source loop of one item: tiles of up to 64 sources, staged through
+// LDS.  fetch(t, c, v, w) loads this lane's source of tile t -- coordinates
+// c, raw values v, weight w -- with plain loads only (no arithmetic, so no
+// wait); the record stored to LDS holds c and w v.  With few values per
+// source the next tile is fetched into registers while the current one is
+// summed, so its load latency overlaps the arithmetic.
+template <int NR, int ND, int FAM, bool CUBIC, class V, class Fetch>
+__device__ __forceinline__ void source_loop(int nsrc, int nd, int lane, double *lds, const double *a0,
+                                            const double *a1, double *acc0, double *acc1, bool two,
+                                            Fetch fetch) {
+    constexpr int MD = ND > 0 ? ND : kMaxDimDev;
+    constexpr bool kPrefetch = MD + NR <= 12;
+    const int S = nd + NR;
+    const int ntiles = (nsrc + kWave - 1) / kWave;
+    double c[MD], w = 1.0;
+    V v[NR];
+    if (kPrefetch && ntiles > 0 && lane < min(kWave, nsrc)) fetch(0, c, v, w);
+    for (int t = 0; t < ntiles; t++) {
+        const int cnt = min(kWave, nsrc - t * kWave);
+        __syncthreads();  // the previous tile is consumed
+        if (lane < cnt) {
+            if (!kPrefetch) fetch(t, c, v, w);
+            double *dst = lds + lane * S;
+#pragma unroll
+            for (int d = 0; d < MD; d++)
+                if (d < nd) dst[d] = c[d];
+#pragma unroll
+            for (int k = 0; k < NR; k++) dst[nd + k] = w * (double)v[k];
+        }
+        __syncthreads();
+        if (kPrefetch && t + 1 < ntiles && lane < min(kWave, nsrc - (t + 1) * kWave)) fetch(t + 1, c, v, w);
+        if (two) tile_sum<NR, ND, FAM, CUBIC, 2>(lds, S, cnt, a0, a1, nd, acc0, acc1);
+        else tile_sum<NR, ND, FAM, CUBIC, 1>(lds, S, cnt, a0, a1, nd, acc0, acc1);
+    }
+}
+
+template <int MD>
+__device__ __forceinline__ void load_point(const KernelDev &K, int64_t pos, int nd, double *a) {
+    const double *xa = K.X + (pos - K.pos0) * nd;
+#pragma unroll
+    for (int d = 0; d < MD; d++) a[d] = d < nd ? gload(xa + d) : 0.0;
+}
+
+// every load issued so far has landed: the target coordinates are then not
+// waited for inside the source loop, where that wait would also wait for
+// the in-flight prefetch
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0); }
 
 template <typename T, int NR, int ND, int FAM>
 __device__ __forceinline__ void proj_item(const SkelProj &P, const KernelDev &K, const int32_t *__restrict__ SK,
@@ -94,63 +166,49 @@ __device__ __forceinline__ void proj_item(const SkelProj &P, const KernelDev &K,
                                           double *__restrict__ part, int lane, double *lds) {
     constexpr int MD = ND > 0 ? ND : kMaxDimDev;
     const int nd = ND > 0 ? ND : K.ndim;
-    const int S = nd + NR;
-    const bool act = lane < P.ntgt;
-    const int64_t tp = SK[P.t0 + (act ? lane : 0)];
-    const double *xa = K.X + (tp - K.pos0) * nd;
-    double a[MD];
+    const bool act0 = lane < P.ntgt, act1 = lane + kWave < P.ntgt;
+    double a0[MD], a1[MD];
+    load_point<MD>(K, SK[P.t0 + (act0 ? lane : 0)], nd, a0);
+    load_point<MD>(K, SK[P.t0 + (act1 ? lane + kWave : 0)], nd, a1);
+    wait_loads();
+    double acc0[NR], acc1[NR];
 #pragma unroll
-    for (int d = 0; d < MD; d++) a[d] = d < nd ? xa[d] : 0.0;
-    double acc[NR];
+    for (int c = 0; c < NR; c++) acc0[c] = acc1[c] = 0.0;
+    source_loop<NR, ND, FAM, kCubic<T>, T>(P.nsrc, nd, lane, lds, a0, a1, acc0, acc1, P.ntgt > kWave,
+                                            [&](int t, double *c, T *v, double &w) {
+                                                const int64_t j = P.src0 + (int64_t)t * kWave + lane;
+                                                const double *xs = K.X + (j - K.pos0) * nd;
 #pragma unroll
-    for (int c = 0; c < NR; c++) acc[c] = 0.0;
-    for (int j0 = 0; j0 < P.nsrc; j0 += kWave) {
-        const int cnt = min(kWave, P.nsrc - j0);
-        __syncthreads();
-        if (lane < cnt) {
-            const int64_t j = P.src0 + j0 + lane;  // source tree position
-            const double *xs = K.X + (j - K.pos0) * nd;
-            double *dst = lds + lane * S;
+                                                for (int d = 0; d < MD; d++)
+                                                    if (d < nd) c[d] = gload(xs + d);
+                                                if (K.W) w = gload(K.W + (j - K.pos0));
 #pragma unroll
-            for (int d = 0; d < MD; d++)
-                if (d < nd) dst[d] = xs[d];
-            const double w = K.W ? K.W[j - K.pos0] : 1.0;
+                                                for (int k = 0; k < NR; k++) v[k] = gload(x + j * NR + k);
+                                            });
+    const double sc = K.scale;
+    if (act0) {
+        double *o = P.out < 0 ? t1 + (P.t0 + lane) * NR : part + (P.out + lane) * NR;
 #pragma unroll
-            for (int c = 0; c < NR; c++) dst[nd + c] = w * (double)x[j * NR + c];
-        }
-        __syncthreads();
-        tile_sum<NR, ND, FAM, kNewton<T>>(lds, S, cnt, a, nd, acc);
+        for (int c = 0; c < NR; c++) o[c] = sc * acc0[c];
     }
-    if (!act) return;
-    double *o = P.out < 0 ? t1 + (P.t0 + lane) * NR : part + (P.out + lane) * NR;
+    if (act1) {
+        double *o = P.out < 0 ? t1 + (P.t0 + lane + kWave) * NR : part + (P.out + lane + kWave) * NR;
 #pragma unroll
-    for (int c = 0; c < NR; c++) o[c] = K.scale * acc[c];
-}
-
-template <typename T, int NR, int FAM>
-__device__ __forceinline__ void proj_nd(const SkelProj &P, const KernelDev &K, const int32_t *SK, const T *x,
-                                        double *t1, double *part, int lane, double *lds) {
-    switch (K.ndim) {
-    case 2: proj_item<T, NR, 2, FAM>(P, K, SK, x, t1, part, lane, lds); break;
-    case 3: proj_item<T, NR, 3, FAM>(P, K, SK, x, t1, part, lane, lds); break;
-    case 4: proj_item<T, NR, 4, FAM>(P, K, SK, x, t1, part, lane, lds); break;
-    default: proj_item<T, NR, 0, FAM>(P, K, SK, x, t1, part, lane, lds); break;
+        for (int c = 0; c < NR; c++) o[c] = sc * acc1[c];
     }
 }
 
-// one item per single-wave workgroup; dynamic LDS: 64 (max ndim + NR) doubles
-template <typename T, int NR>
+// one item per single-wave workgroup, one (family, NDIM) per launch (ND = 0:
+// any NDIM up to 8); dynamic LDS: 64 (NDIM + NR) doubles
+template <typename T, int NR, int ND, int FAM>
 __global__ __launch_bounds__(64) void skel_proj_kernel(const SkelProj *__restrict__ items,
                                                        const KernelDev *__restrict__ Ks,
                                                        const int32_t *__restrict__ SK, const T *__restrict__ x,
                                                        double *__restrict__ t1, double *__restrict__ part) {
     extern __shared__ double lds[];
-    const int lane = threadIdx.x;
     const SkelProj P = items[blockIdx.x];
     const KernelDev K = Ks[P.prob];
-    if (K.family == 1) proj_nd<T, NR, 1>(P, K, SK, x, t1, part, lane, lds);
-    else if (K.family == 0) proj_nd<T, NR, 0>(P, K, SK, x, t1, part, lane, lds);
-    else proj_nd<T, NR, 2>(P, K, SK, x, t1, part, lane, lds);
+    proj_item<T, NR, ND, FAM>(P, K, SK, x, t1, part, threadIdx.x, lds);
 }
 
 __device__ __forceinline__ int64_t wave_item() {
@@ -164,13 +222,13 @@ __global__ __launch_bounds__(256) void skel_reduce_kernel(const SkelReduce *__re
     if (it >= n) return;
     const int lane = threadIdx.x & (kWave - 1);
     const SkelReduce R = items[it];
-    if (lane >= R.ntgt) return;
+    for (int l = lane; l < R.ntgt; l += kWave)
 #pragma unroll
-    for (int c = 0; c < NR; c++) {
-        double s = 0.0;
-        for (int ch = 0; ch < R.nch; ch++) s += part[(R.slot0 + (int64_t)ch * kWave + lane) * NR + c];
-        t1[(R.t0 + lane) * NR + c] = s;
-    }
+        for (int c = 0; c < NR; c++) {
+            double s = 0.0;
+            for (int ch = 0; ch < R.nch; ch++) s += part[(R.slot0 + (int64_t)ch * kSkelTile + l) * NR + c];
+            t1[(R.t0 + l) * NR + c] = s;
+        }
 }
 
 template <int NR>
@@ -187,6 +245,7 @@ __global__ __launch_bounds__(256) void skel_core_kernel(const SkelCore *__restri
         double acc[NR];
 #pragma unroll
         for (int c = 0; c < NR; c++) acc[c] = 0.0;
+#pragma unroll 8
         for (int k = 0; k < C.r; k++) {
             const double g = act ? G[C.g + (int64_t)k * C.r + i] : 0.0;
 #pragma unroll
@@ -204,80 +263,93 @@ __device__ __forceinline__ void expand_item(const SkelExpand &E, const KernelDev
                                             T *__restrict__ y, int lane, double *lds) {
     constexpr int MD = ND > 0 ? ND : kMaxDimDev;
     const int nd = ND > 0 ? ND : K.ndim;
-    const int S = nd + NR;
-    const bool act = lane < E.h;
-    const int64_t ip = E.y0 + (act ? lane : 0);
-    const double *xa = K.X + (ip - K.pos0) * nd;
-    double a[MD];
+    const bool act0 = lane < E.h, act1 = lane + kWave < E.h;
+    const int64_t ip0 = E.y0 + (act0 ? lane : 0), ip1 = E.y0 + (act1 ? lane + kWave : 0);
+    double a0[MD], a1[MD];
+    load_point<MD>(K, ip0, nd, a0);
+    load_point<MD>(K, ip1, nd, a1);
+    wait_loads();
+    double acc0[NR], acc1[NR];
 #pragma unroll
-    for (int d = 0; d < MD; d++) a[d] = d < nd ? xa[d] : 0.0;
-    double acc[NR];
+    for (int c = 0; c < NR; c++) acc0[c] = acc1[c] = 0.0;
+    source_loop<NR, ND, FAM, kCubic<T>, double>(E.nc, nd, lane, lds, a0, a1, acc0, acc1, E.h > kWave,
+                                                 [&](int t, double *c, double *v, double &w) {
+                                                     const int64_t q = E.c0 + (int64_t)t * kWave + lane;  // slot
+                                                     const int64_t cp = gload(SKC + q) - K.pos0;
+                                                     const double *xb = K.X + cp * nd;
 #pragma unroll
-    for (int c = 0; c < NR; c++) acc[c] = 0.0;
-    for (int q0 = 0; q0 < E.nc; q0 += kWave) {
-        const int cnt = min(kWave, E.nc - q0);
-        __syncthreads();
-        if (lane < cnt) {
-            const int64_t q = E.c0 + q0 + lane;  // column slot
-            const int64_t cp = SKC[q] - K.pos0;
-            const double *xb = K.X + cp * nd;
-            double *dst = lds + lane * S;
+                                                     for (int d = 0; d < MD; d++)
+                                                         if (d < nd) c[d] = gload(xb + d);
+                                                     if (K.W) w = gload(K.W + cp);
 #pragma unroll
-            for (int d = 0; d < MD; d++)
-                if (d < nd) dst[d] = xb[d];
-            const double w = K.W ? K.W[cp] : 1.0;
+                                                     for (int k = 0; k < NR; k++) v[k] = gload(t2 + q * NR + k);
+                                                 });
+    const double sc = K.scale;
+    if (act0) {
+        T *o = y + ip0 * NR;
 #pragma unroll
-            for (int c = 0; c < NR; c++) dst[nd + c] = w * t2[q * NR + c];
-        }
-        __syncthreads();
-        tile_sum<NR, ND, FAM, kNewton<T>>(lds, S, cnt, a, nd, acc);
+        for (int c = 0; c < NR; c++) o[c] = (T)((double)o[c] + sc * acc0[c]);
     }
-    if (!act) return;
-    T *o = y + ip * NR;
+    if (act1) {
+        T *o = y + ip1 * NR;
 #pragma unroll
-    for (int c = 0; c < NR; c++) o[c] = (T)((double)o[c] + K.scale * acc[c]);
-}
-
-template <typename T, int NR, int FAM>
-__device__ __forceinline__ void expand_nd(const SkelExpand &E, const KernelDev &K, const int32_t *SKC,
-                                          const double *t2, T *y, int lane, double *lds) {
-    switch (K.ndim) {
-    case 2: expand_item<T, NR, 2, FAM>(E, K, SKC, t2, y, lane, lds); break;
-    case 3: expand_item<T, NR, 3, FAM>(E, K, SKC, t2, y, lane, lds); break;
-    case 4: expand_item<T, NR, 4, FAM>(E, K, SKC, t2, y, lane, lds); break;
-    default: expand_item<T, NR, 0, FAM>(E, K, SKC, t2, y, lane, lds); break;
+        for (int c = 0; c < NR; c++) o[c] = (T)((double)o[c] + sc * acc1[c]);
     }
 }
 
-template <typename T, int NR>
+template <typename T, int NR, int ND, int FAM>
 __global__ __launch_bounds__(64) void skel_expand_kernel(const SkelExpand *__restrict__ items,
                                                          const KernelDev *__restrict__ Ks,
                                                          const int32_t *__restrict__ SKC,
                                                          const double *__restrict__ t2, T *__restrict__ y) {
     extern __shared__ double lds[];
-    const int lane = threadIdx.x;
     const SkelExpand E = items[blockIdx.x];
     const KernelDev K = Ks[E.prob];
-    if (K.family == 1) expand_nd<T, NR, 1>(E, K, SKC, t2, y, lane, lds);
-    else if (K.family == 0) expand_nd<T, NR, 0>(E, K, SKC, t2, y, lane, lds);
-    else expand_nd<T, NR, 2>(E, K, SKC, t2, y, lane, lds);
+    expand_item<T, NR, ND, FAM>(E, K, SKC, t2, y, threadIdx.x, lds);
 }
 
 inline dim3 wave_grid(int64_t n) { return dim3((unsigned)((n + 3) / 4)); }
 
+template <typename T, int NR, int ND, int FAM>
+void launch_group(bool proj, const SkeletonPlan &p, const SkelLaunch &L, size_t lds, const T *x, double *t1,
+                  double *part, const double *t2, T *y, hipStream_t s) {
+    const dim3 g((unsigned)(L.end - L.begin));
+    if (proj)
+        skel_proj_kernel<T, NR, ND, FAM><<<g, 64, lds, s>>>(p.proj + L.begin, p.K, p.SK, x, t1, part);
+    else
+        skel_expand_kernel<T, NR, ND, FAM><<<g, 64, lds, s>>>(p.ex + L.begin, p.K, p.SK + p.nrow_slots, t2, y);
+}
+
+template <typename T, int NR, int FAM>
+void launch_nd(bool proj, const SkeletonPlan &p, const SkelLaunch &L, size_t lds, const T *x, double *t1,
+               double *part, const double *t2, T *y, hipStream_t s) {
+    switch (L.key % 16) {
+    case 2: launch_group<T, NR, 2, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    case 3: launch_group<T, NR, 3, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    case 4: launch_group<T, NR, 4, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    default: launch_group<T, NR, 0, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    }
+}
+
+template <typename T, int NR>
+void launch_key(bool proj, const SkeletonPlan &p, const SkelLaunch &L, const T *x, double *t1, double *part,
+                const double *t2, T *y, hipStream_t s) {
+    if (L.end <= L.begin) return;
+    const size_t lds = sizeof(double) * kWave * (size_t)(L.key % 16 + NR);
+    switch (L.key / 16) {
+    case 0: launch_nd<T, NR, 0>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    case 1: launch_nd<T, NR, 1>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    default: launch_nd<T, NR, 2>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    }
+}
+
 template <typename T, int NR>
 hipError_t launch_skeleton_k(const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2, T *y,
                              hipStream_t s) {
-    const size_t lds = sizeof(double) * kWave * (size_t)(p.max_ndim + NR);
-    if (p.nproj) skel_proj_kernel<T, NR><<<dim3((unsigned)p.nproj), 64, lds, s>>>(p.proj, p.K, p.SK, x, t1, part);
+    for (int i = 0; i < p.nproj_l; i++) launch_key<T, NR>(true, p, p.proj_l[i], x, t1, part, t2, y, s);
     if (p.nred) skel_reduce_kernel<NR><<<wave_grid(p.nred), 256, 0, s>>>(p.red, p.nred, part, t1);
     if (p.ncore) skel_core_kernel<NR><<<wave_grid(p.ncore), 256, 0, s>>>(p.core, p.ncore, p.G, t1, t2);
-    for (int l = 0; l < p.nlvl; l++) {
-        const int64_t a = p.ex_lvl[l], b = p.ex_lvl[l + 1];
-        if (b > a)
-            skel_expand_kernel<T, NR><<<dim3((unsigned)(b - a)), 64, lds, s>>>(p.ex + a, p.K, p.SK + p.nrow_slots,
-                                                                               t2, y);
-    }
+    for (int i = 0; i < p.nex_l; i++) launch_key<T, NR>(false, p, p.ex_l[i], x, t1, part, t2, y, s);
     return hipGetLastError();
 }
 
