@@ -33,6 +33,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s + p99 latency at 1e-8 residual, mixed vs FP64 HODLR; matvec HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# FP64 vector issue rate of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+# (78.6 TFLOP/s counts each of these FMAs as 2 flops)
+VALU_F64_LANE_OPS = 256 * 4 * 16 * 2.4e9
+# issue slots per on-the-fly entry of the 3-D 1/r skeleton kernels (from
+# their gfx950 code): 3 v_add_f64 + 3 v_fma_f64 (distance), v_rsq_f64 (a
+# quarter-rate transcendental: 4 slots), 5 for the refinement (cubic step,
+# FP64 operator) or 4 (Newton step, FP32 operator), 1 FMA per RHS
+SKEL_SLOTS = {"f64": 16, "f32": 15}
 
 
 def log(*a):
@@ -333,6 +341,122 @@ def bench_batch(args, H):
     return out
 
 
+def bench_skeleton(args, H):
+    """One N = n1d^3 solve per step (the headline's problem) with the
+    reference's memory-efficient operator (LowRank_matrix.hpp:22-36,55-58,
+    685-698): every compressed block whose column node holds more than
+    sys_size points (default 2 Nmax, the reference's SYS_SIZE) keeps only its
+    pivot rows and columns and the pseudo-inverse of its core; each matvec
+    evaluates its entries again on the GPU."""
+    n1 = args.n1d
+    pts = make_points(n1)
+    n = len(pts)
+    sys_size = args.sys_size if args.sys_size >= 0 else 2 * args.nmax
+    kern = H.Kernel(H.KERNEL_INV_R, 3, 1.0 / n, 1.0)
+    tree = H.Tree(pts, -np.ones(3), np.ones(3), args.nmax, 0)
+    t0 = time.time()
+    op = H.Operator(tree, kern, args.eps, H.PREC_F64 | H.PREC_F32, sys_size=sys_size)
+    t_asm = time.time() - t0
+    st = op.stats()
+    # the stored form of the same blocks: their r (m + n) factor entries in
+    # place of the cores and pivot positions
+    stored64 = st["bytes_f64"] - st["bytes_core"] + 8.0 * st["e_skel"]
+    stored32 = st["bytes_f32"] + 4.0 * st["e_skel"]
+    log("skeleton: N=%d sys_size=%d, assembled in %.2fs: %d skeleton blocks, %.3g entries on the fly per "
+        "matvec, %.1f GB FP64 + %.1f GB FP32 (stored form: %.1f + %.1f GB)"
+        % (n, sys_size, t_asm, st["n_skeleton"], st["e_skel"], st["bytes_f64"] * 1e-9, st["bytes_f32"] * 1e-9,
+           stored64 * 1e-9, stored32 * 1e-9))
+    mvt = {}
+    for prec, key in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32")):
+        ms = op.matvec_timing(prec, 1, 10)
+        mvt[key] = {"total": ms[0], "proj": ms[1], "panel": ms[2], "skeleton": ms[3]}
+        log("matvec %s: %.3f ms (stored part: proj %.3f, panel %.3f; skeleton %.3f)"
+            % (key, ms[0], ms[1], ms[2], ms[3]))
+    ctx = tree.ctx
+    perm = tree.perm()
+    b = splitmix_uniform(1, n)
+    db = H.DeviceArray(ctx, 8 * n)
+    dx = H.DeviceArray(ctx, 8 * n)
+    db.upload(np.ascontiguousarray(b[perm]))
+
+    def run(mode, k):
+        x0 = "b" if mode == H.SOLVE_F64 else "zero"
+        lat, infos = [], []
+        for _ in range(k):
+            t1 = time.perf_counter()
+            infos.append(op.gmres_device(db.ptr, dx.ptr, restart=100, max_iter=n, tol=args.tol, mode=mode,
+                                         inner_tol=args.inner_tol, x0=x0))
+            lat.append(time.perf_counter() - t1)
+        return np.array(lat), infos
+
+    run(H.SOLVE_F64, args.warmup)
+    lat64, inf64 = run(H.SOLVE_F64, args.steps)
+    run(H.SOLVE_MIXED, args.warmup)
+    latm, infm = run(H.SOLVE_MIXED, args.steps)
+    log("skeleton fp64: %.2f ms/solve (%d iterations), mixed: %.2f ms/solve (%d inner, %d sweeps)"
+        % (lat64.mean() * 1e3, inf64[-1]["iterations"], latm.mean() * 1e3, infm[-1]["iterations"],
+           infm[-1]["outer"]))
+    xt = dx.download(np.empty(n))
+    x = np.empty(n)
+    x[perm] = xt
+    rows = np.unique((np.abs(splitmix_uniform(77, 256)) * n).astype(np.int64))
+    xr = splitmix_uniform(78, n)
+    yd = op.dense_rows(rows, xr)
+    mv_err = float(np.linalg.norm(op.matvec(xr)[rows] - yd) / np.linalg.norm(yd))
+    dres = float(np.linalg.norm(b[rows] - op.dense_rows(rows, x)) / np.linalg.norm(b[rows]))
+    true_res = float(np.linalg.norm(b - op.matvec(x)) / np.linalg.norm(b))
+    log("sampled dense rows: matvec error %.3g, residual of the mixed solution %.3g" % (mv_err, dres))
+    sk = mvt["f64"]["skeleton"]
+    ent = st["e_skel"] / (sk * 1e-3) if sk > 0 else 0.0
+    achieved = ent * SKEL_SLOTS["f64"]
+    return {
+        "metric": "solves/s at 1e-8 residual, mixed vs FP64, on-the-fly skeleton operator",
+        "value": args.steps / latm.sum(),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": latm.mean() * 1e3,
+        "higher_is_better": True,
+        "dtype": "fp32 matvec (fp64 on-the-fly entries) + fp64 refinement",
+        "data": "synthetic",
+        "config": {"workload": "3D %d^3 uniform grid, N=%d, A=I+(1/N)/r, Nmax=%d, eps_ACA=%g, d'=0, "
+                               "skeleton blocks for |cols| > %d, GMRES(100) to rel. residual %g, 1 RHS"
+                               % (n1, n, args.nmax, args.eps, sys_size, args.tol),
+                   "n": n, "nmax": args.nmax, "sys_size": sys_size},
+        "p50_ms": float(np.percentile(latm, 50) * 1e3),
+        "p99_ms": float(np.percentile(latm, 99) * 1e3),
+        "fp64_solves_per_s": args.steps / lat64.sum(),
+        "fp64_p50_ms": float(np.percentile(lat64, 50) * 1e3),
+        "fp64_p99_ms": float(np.percentile(lat64, 99) * 1e3),
+        "iterations_fp64": inf64[-1]["iterations"],
+        "iterations_mixed_inner": infm[-1]["iterations"],
+        "refinement_sweeps": infm[-1]["outer"],
+        "true_resid_mixed": true_res,
+        "dense_check": {"rows": int(len(rows)), "matvec_rel_err": mv_err, "mixed_solution_resid": dres},
+        "matvec_ms": mvt,
+        "skeleton_blocks": st["n_skeleton"],
+        "entries_on_the_fly_per_matvec": st["e_skel"],
+        "entries_per_s_f64": ent,
+        "entries_per_s_f32": st["e_skel"] / (mvt["f32"]["skeleton"] * 1e-3) if mvt["f32"]["skeleton"] else 0.0,
+        "operator_gb": {"f64": st["bytes_f64"] * 1e-9, "f32": st["bytes_f32"] * 1e-9,
+                        "cores": st["bytes_core"] * 1e-9},
+        "stored_form_gb": {"f64": stored64 * 1e-9, "f32": stored32 * 1e-9},
+        "assemble_s": t_asm,
+        "roofline": {
+            "bound": "valu",
+            "kernel": "skel_proj + skel_reduce + skel_core + skel_expand (FP64 operator, 1 RHS)",
+            "achieved": achieved,
+            "peak": VALU_F64_LANE_OPS,
+            "unit": "FP64 VALU lane-ops/s",
+            "frac": achieved / VALU_F64_LANE_OPS,
+            "slots_per_entry": SKEL_SLOTS["f64"],
+            "avg_phase_ms": sk,
+            "traffic": None,
+        },
+    }
+
+
 def bench_serve(args, H):
     """Serving stream: Poisson arrivals of solve requests, NDIM = 3, N
     log-uniform in [1000, 200000], points uniform in [-1,1]^3, A = I + (1/N)/r
@@ -457,9 +581,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline work per leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="grid100: skip the batched and serving legs reported beside the headline")
-    ap.add_argument("--workload", choices=["grid100", "batch256", "serve"], default="grid100",
+    ap.add_argument("--workload", choices=["grid100", "batch256", "serve", "skeleton"], default="grid100",
                     help="grid100: one N=1e6 solve per step; batch256: 256 uneven problems x 8 RHS "
-                         "solved as one batch per step; serve: a Poisson stream of solve requests")
+                         "solved as one batch per step; serve: a Poisson stream of solve requests; "
+                         "skeleton: the grid100 solve with on-the-fly skeleton blocks")
+    ap.add_argument("--sys-size", type=int, default=-1,
+                    help="skeleton: blocks with more columns are skeletons (default 2 Nmax)")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second (0: all at once)")
     ap.add_argument("--slots", type=int, default=16, help="serve: solve columns per operator engine")
@@ -491,6 +618,9 @@ def main():
         if args.nmax == 128:
             args.nmax = 64
         print(json.dumps((bench_batch if args.workload == "batch256" else bench_serve)(args, H)))
+        return
+    if args.workload == "skeleton":
+        print(json.dumps(bench_skeleton(args, H)))
         return
 
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)
@@ -640,13 +770,19 @@ def main():
     }
     if not args.no_extra:
         # the batched and served forms of the same solve, each on its own
-        # workload, reported beside the headline.  Each runs in a child
+        # workload; the same solve with the reference's memory-efficient
+        # (skeleton) operator; and a 160^3 grid (N = 4.1M) whose stored
+        # FP64 + FP32 operator (~330 GB) exceeds HBM, in skeleton form --
+        # reported beside the headline.  Each runs in a child
         # process of its own (a fresh device-memory state), after this
         # process has released its operator.
         del op, db, dx
         import subprocess
         legs = (("batched", ["--workload", "batch256", "--steps", "3", "--warmup", "1", "--cpu-budget", "10"]),
-                ("serving", ["--workload", "serve", "--requests", "100"]))
+                ("serving", ["--workload", "serve", "--requests", "100"]),
+                ("skeleton", ["--workload", "skeleton", "--steps", "5", "--warmup", "1"]),
+                ("skeleton_beyond_hbm", ["--workload", "skeleton", "--n1d", "160", "--sys-size", "128",
+                                         "--steps", "3", "--warmup", "1"]))
         for key, extra in legs:
             cmd = [sys.executable, "-u", os.path.abspath(__file__)] + extra + (["--no-cpu"] if args.no_cpu else [])
             try:

@@ -2,7 +2,8 @@
 //
 // Same shapes as kandapva/HODLRnD so a caller can switch by changing the
 // include:
-//   knobs NDIM, Nmax, eps_ACA, INTERACTION_TYPE_ALLOWED   (myHeaders.hpp:35-49)
+//   knobs NDIM, Nmax, eps_ACA, INTERACTION_TYPE_ALLOWED,  (myHeaders.hpp:35-49)
+//         SYS_SIZE
 //   ptsnD, uniform_nodes, Vec_ops::relative_error         (points_dt.hpp:9-13,
 //                                                          myHeaders.hpp:111-136)
 //   user kernels with getMatrixEntry / get_points          (user_kernel.hpp:10-179)
@@ -43,13 +44,19 @@
 #ifndef HODLR_DPRIME
 #define HODLR_DPRIME 0
 #endif
+#ifndef HODLR_SYS_SIZE
+#define HODLR_SYS_SIZE (2 * HODLR_NMAX)
+#endif
 
 // ---- knobs (myHeaders.hpp:35-49) ----
 constexpr int NDIM = HODLR_NDIM;
 constexpr int Nmax = HODLR_NMAX;
 constexpr int INTERACTION_TYPE_ALLOWED = HODLR_DPRIME;  // d'
 const double eps_ACA = HODLR_EPS_ACA;
-constexpr int SYS_SIZE = 2 * Nmax;  // kept for source compatibility; every factor is stored
+// blocks with more columns keep only their skeleton and are evaluated on
+// the fly (LowRank_matrix.hpp:22-36); define HODLR_SYS_SIZE -1 to store
+// every factor
+constexpr int SYS_SIZE = HODLR_SYS_SIZE;
 inline int numPoints = 10;
 inline size_t N = 1;
 const double PI = 4.0 * std::atan(1.0);
@@ -321,7 +328,9 @@ public:
     size_t get_size() { return N_; }
     void Assemble_matrix_operators() {
         hodlr_kernel_desc kd = kernel_->device_kernel();
-        check(hodlr_op_assemble(ctx_, tree_, &kd, eps_ACA, HODLR_PREC_F64 | HODLR_PREC_F32, &op_), "op_assemble");
+        check(hodlr_op_assemble_skeleton(ctx_, 1, &tree_, &kd, eps_ACA, SYS_SIZE, HODLR_PREC_F64 | HODLR_PREC_F32,
+                                         &op_),
+              "op_assemble");
         std::printf("Matrix operators formed...\n");
     }
     Vec operator*(const Vec &x) {
