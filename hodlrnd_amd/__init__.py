@@ -110,6 +110,7 @@ SIGNATURES = {
     "hodlr_vec_save": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p]),
     "hodlr_vec_load": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "hodlr_op_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hodlr_op_set_solver_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "hodlr_op_block_ranks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
     "hodlr_op_destroy": (None, [C.c_void_p]),
     "hodlr_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -509,6 +510,10 @@ class Operator:
         v = C.c_int()
         self.ctx.check(lib.hodlr_op_get_matvec_variant(self.h, prec, nrhs, C.byref(v)), "get_variant")
         return v.value
+
+    def set_graphs(self, enable=True):
+        """Replay captured Arnoldi steps as hipGraphs in the batched solver."""
+        self.ctx.check(lib.hodlr_op_set_solver_graphs(self.h, 1 if enable else 0), "set_solver_graphs")
 
     def matvec_timing(self, prec=PREC_F64, nrhs=1, reps=10):
         ms = np.zeros(4)

@@ -10,6 +10,14 @@
 // problem boundary (so an instance's arithmetic does not depend on the rest
 // of the batch).  Converged instances stop changing: their next basis
 // vectors are zero and their coefficients are dropped.
+//
+// The device work of Arnoldi step i -- the matvec, both CGS2 passes, the
+// norm of the new vector and the copies of the Hessenberg column to pinned
+// host memory -- depends only on i (basis vectors, work buffers and the
+// pinned staging are fixed per workspace), so it is captured once into a
+// hipGraph per (operator precision, i) and replayed as one launch followed
+// by one wait; the host then applies the Givens rotations.  A graph is
+// recaptured when the operator's work buffers or kernel forms change.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -57,9 +65,13 @@ struct BatchWork {
     int P = 0, nchunk = 0;
     std::vector<DevBuf> V;       // basis vectors, allocated on first use
     std::vector<double *> vptr;
-    DevBuf vtab, w, r, d, rr, w64, x32, y32, partial, coef, chunks, pchunk, tab1;
+    DevBuf vtab, w, r, d, rr, w64, x32, y32, partial, coef, chunks, pchunk, tab1, tabw;
     double *hbuf = nullptr;      // pinned staging
     size_t hcap = 0;
+    // captured Arnoldi steps [operator precision: 0 FP64, 1 FP32][step] and
+    // the operator work generation each was captured at
+    std::vector<hipGraphExec_t> gexec[2];
+    std::vector<uint64_t> ggen[2];
 
     BatchWork(const hodlr_op &op, int K_, int restart_) : K(K_), restart(restart_) {
         n = op.N;
@@ -94,10 +106,20 @@ struct BatchWork {
         partial.alloc(sizeof(double) * (size_t)std::max(nchunk, 1) * (restart + 1) * K);
         coef.alloc(sizeof(double) * (size_t)P * (restart + 1) * K * 3);
         tab1.alloc(sizeof(double *));
+        tabw.alloc(sizeof(double *));
+        double *wp = w.as<double>();
+        HIP_CHECK(hipMemcpy(tabw.p, &wp, sizeof(double *), hipMemcpyHostToDevice));
         hcap = (size_t)P * (restart + 1) * K * 3 + 16;
         HIP_CHECK(hipHostMalloc(&hbuf, sizeof(double) * hcap));
+        for (int k = 0; k < 2; k++) {
+            gexec[k].assign(restart, nullptr);
+            ggen[k].assign(restart, 0);
+        }
     }
     ~BatchWork() {
+        for (auto &g : gexec)
+            for (auto &e : g)
+                if (e) (void)hipGraphExecDestroy(e);
         if (hbuf) (void)hipHostFree(hbuf);
     }
     double *basis(int i) {
@@ -141,10 +163,13 @@ struct Inst {
 
 }  // namespace
 
-// Batched GMRES: A applies the operator to n x K device vectors.
+// Batched GMRES: A applies the operator to n x K device vectors; gkey names
+// the operator precision A uses (the step graphs are cached per key), < 0:
+// no graphs.
 void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWork &bw, const double *b,
                  double *x, bool x0_zero, const std::vector<double> &tol, int max_iter, hipStream_t s,
-                 std::vector<int> &iters, std::vector<int> &conv, std::vector<double> &resid, int &matvecs) {
+                 std::vector<int> &iters, std::vector<int> &conv, std::vector<double> &resid, int &matvecs,
+                 int gkey, uint64_t gen) {
     const int m = bw.restart, K = bw.K, P = bw.P;
     const int NI = P * K;
     const int64_t nk = bw.n * K;
@@ -222,20 +247,54 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
         }
         int i = 0;
         for (; i < m && j <= max_iter && nactive > 0; i++, j++) {
-            A(bw.basis(i), w);
-            matvecs++;
             const int nv = i + 1;
-            HIP_CHECK(launch_bmdot(bw.table(), nv, w, bw.chunks.as<Chunk>(), bw.nchunk, bw.partial.as<double>(),
-                                   bw.pchunk.as<int32_t>(), P, K, cf, s));
-            HIP_CHECK(launch_bgemv(bw.table(), nv, cf, -1.0, w, bw.chunks.as<Chunk>(), bw.nchunk, K, s));
-            HIP_CHECK(launch_bmdot(bw.table(), nv, w, bw.chunks.as<Chunk>(), bw.nchunk, bw.partial.as<double>(),
-                                   bw.pchunk.as<int32_t>(), P, K, cf + cfs, s));
-            HIP_CHECK(launch_bgemv(bw.table(), nv, cf + cfs, -1.0, w, bw.chunks.as<Chunk>(), bw.nchunk, K, s));
             const size_t nh = (size_t)P * nv * K;
-            HIP_CHECK(hipMemcpyAsync(bw.hbuf, cf, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipMemcpyAsync(bw.hbuf + nh, cf + cfs, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
-            std::vector<double> hn2;
-            inst_dot(bw, w, w, hn2, s);  // synchronizes; hbuf[0, 2 nh) is then valid
+            bw.basis(i);
+            bw.basis(i + 1);  // both allocated (and tabled) before any capture
+            // step i's device work: new vector, CGS2, its norm, and the copies
+            // of h1, h2 and |w|^2 to hbuf[0, nh), [nh, 2 nh), [2 cfs, 2 cfs + NI)
+            auto step = [&] {
+                A(bw.vptr[i], w);
+                HIP_CHECK(launch_bmdot(bw.table(), nv, w, bw.chunks.as<Chunk>(), bw.nchunk, bw.partial.as<double>(),
+                                       bw.pchunk.as<int32_t>(), P, K, cf, s));
+                HIP_CHECK(launch_bgemv(bw.table(), nv, cf, -1.0, w, bw.chunks.as<Chunk>(), bw.nchunk, K, s));
+                HIP_CHECK(launch_bmdot(bw.table(), nv, w, bw.chunks.as<Chunk>(), bw.nchunk, bw.partial.as<double>(),
+                                       bw.pchunk.as<int32_t>(), P, K, cf + cfs, s));
+                HIP_CHECK(launch_bgemv(bw.table(), nv, cf + cfs, -1.0, w, bw.chunks.as<Chunk>(), bw.nchunk, K, s));
+                HIP_CHECK(launch_bmdot(bw.tabw.as<const double *>(), 1, w, bw.chunks.as<Chunk>(), bw.nchunk,
+                                       bw.partial.as<double>(), bw.pchunk.as<int32_t>(), P, K, cf + 2 * cfs, s));
+                HIP_CHECK(hipMemcpyAsync(bw.hbuf, cf, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
+                HIP_CHECK(hipMemcpyAsync(bw.hbuf + nh, cf + cfs, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
+                HIP_CHECK(hipMemcpyAsync(bw.hbuf + 2 * cfs, cf + 2 * cfs, sizeof(double) * NI, hipMemcpyDeviceToHost,
+                                         s));
+            };
+            if (gkey >= 0) {
+                hipGraphExec_t &ex = bw.gexec[gkey][i];
+                if (!ex || bw.ggen[gkey][i] != gen) {
+                    if (ex) HIP_CHECK(hipGraphExecDestroy(ex));
+                    ex = nullptr;
+                    hipGraph_t g = nullptr;
+                    HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    try {
+                        step();
+                    } catch (...) {
+                        (void)hipStreamEndCapture(s, &g);
+                        if (g) (void)hipGraphDestroy(g);
+                        throw;
+                    }
+                    HIP_CHECK(hipStreamEndCapture(s, &g));
+                    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    HIP_CHECK(e);
+                    bw.ggen[gkey][i] = gen;
+                }
+                HIP_CHECK(hipGraphLaunch(ex, s));
+            } else {
+                step();
+            }
+            matvecs++;
+            HIP_CHECK(hipStreamSynchronize(s));
+            std::vector<double> hn2(bw.hbuf + 2 * cfs, bw.hbuf + 2 * cfs + NI);
             std::vector<double> h1(bw.hbuf, bw.hbuf + nh), h2(bw.hbuf + nh, bw.hbuf + 2 * nh);
             std::vector<int> ks(NI, -1);
             int nconv = 0;
@@ -336,12 +395,16 @@ void solve_batch(hodlr_op &op, int K, const double *b, double *x, const hodlr_gm
     const bool x0_zero = p.x0_is_b == 2;
     if (x0_zero) HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * nk, s));
     auto A64 = [&](const double *in, double *out) { op.matvec_dev(HODLR_PREC_F64, K, in, out, s); };
+    // work buffers exist before any step is captured (a capture may not allocate)
+    op.ensure_work(HODLR_PREC_F64, K);
+    if (op.prec_flags & HODLR_PREC_F32) op.ensure_work(HODLR_PREC_F32, K);
+    const bool graphs = op.solver_graphs && !op.profiling;
     std::vector<int> it, cv;
     std::vector<double> rs;
     int mv64 = 0, mv32 = 0;
     if (p.mode == HODLR_SOLVE_F64) {
         std::vector<double> tol(NI, p.tol);
-        gmres_batch(A64, bw, b, x, x0_zero, tol, max_iter, s, it, cv, rs, mv64);
+        gmres_batch(A64, bw, b, x, x0_zero, tol, max_iter, s, it, cv, rs, mv64, graphs ? 0 : -1, op.work_gen);
         for (int a = 0; a < NI; a++) {
             infos[a].iterations = it[a];
             infos[a].converged = cv[a];
@@ -386,7 +449,7 @@ void solve_batch(hodlr_op &op, int K, const double *b, double *x, const hodlr_gm
             for (int a = 0; a < NI; a++)  // a finished instance gets an unreachable-to-miss target
                 target[a] = (res_f64 && res[a] < p.tol) ? 1e300 : std::min(0.5, std::max(floor, 0.3 * p.tol / std::max(res[a], 1e-300)));
             HIP_CHECK(hipMemsetAsync(d, 0, sizeof(double) * nk, s));
-            gmres_batch(A32, bw, rr, d, true, target, max_iter, s, it, cv, rs, mv32);
+            gmres_batch(A32, bw, rr, d, true, target, max_iter, s, it, cv, rs, mv32, graphs ? 1 : -1, op.work_gen);
             for (int a = 0; a < NI; a++) total[a] += it[a];
             HIP_CHECK(launch_add(x, d, nk, s));
             A64(x, w64);

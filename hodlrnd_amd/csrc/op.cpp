@@ -1007,6 +1007,7 @@ void hodlr_op::ensure_work(int prec, int nrhs) {
         skt2.alloc(sizeof(double) * (size_t)n_sk * nrhs);
         skpart.alloc(sizeof(double) * (size_t)(n_skslots > 0 ? n_skslots : 1) * nrhs);
         cap_rhs_sk = nrhs;
+        work_gen++;
     }
     if (prec == HODLR_PREC_F64) {
         if (cap_rhs64 >= nrhs) return;
@@ -1019,6 +1020,7 @@ void hodlr_op::ensure_work(int prec, int nrhs) {
         part32.alloc(sizeof(float) * (size_t)(n_slots > 0 ? n_slots : 1) * kProjTile * nrhs);
         cap_rhs32 = nrhs;
     }
+    work_gen++;
 }
 
 hodlr_op::~hodlr_op() {
@@ -1160,6 +1162,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                     }
             }
             mv_variant[pi][lg] = pick[0] | pick[1];
+            work_gen++;
             if (verbose())
                 std::fprintf(stderr, "[hodlr] tune %s K=%d: proj %s %.3f ms, panel %s %.3f ms\n",
                              pi == 0 ? "f64" : "f32", k, pick[0] ? "mfma" : "valu", best[0] / reps,

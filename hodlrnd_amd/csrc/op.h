@@ -107,6 +107,13 @@ struct hodlr_op {
 
     // per-precision work buffers (grown on demand)
     int cap_rhs64 = 0, cap_rhs32 = 0;
+    // bumped whenever a work buffer is reallocated or a kernel form changes:
+    // graphs captured from matvecs are valid only for the generation they
+    // were captured at
+    uint64_t work_gen = 1;
+    // solvers replay captured Arnoldi steps (hipGraph) unless disabled or
+    // while live kernel timing is on
+    bool solver_graphs = true;
     hodlr::DevBuf t64, part64, x64, y64, t32, part32, x32, y32;
 
     // skeleton blocks (LowRank_matrix.hpp:22-36): compressed blocks whose

@@ -227,6 +227,12 @@ int hodlr_gmres_batch_device(hodlr_op *op, int nrhs, const double *b, double *x,
                              const hodlr_gmres_params *params, hodlr_gmres_info *infos);
 int hodlr_gmres_batch(hodlr_op *op, int nrhs, const double *b, double *x,
                       const hodlr_gmres_params *params, hodlr_gmres_info *infos);
+/* No reference counterpart: the batched solver captures the device work of
+ * each Arnoldi step (matvec, CGS2 passes, norm, copies of the Hessenberg
+ * column) into a hipGraph the first time the step runs and replays it as
+ * one launch afterwards (default on; off while live kernel timing is on).
+ * enable = 0 launches every kernel separately; results are identical. */
+int hodlr_op_set_solver_graphs(hodlr_op *op, int enable);
 
 /* ---- operator files and vector files ----
  * save writes the operator's problems (points, box, tree knobs, kernel,

@@ -146,3 +146,38 @@ def test_batch_device_entry_zero_rhs(batch):
         assert infos[2 * p + 1]["iterations"] == (0 if p != 1 else infos[3]["iterations"])
     assert infos[3]["iterations"] > 0
     assert not np.any(X[:, 0])
+
+
+def test_batch_step_graphs_replay_exactly(batch):
+    """hipGraph replays of the captured Arnoldi steps (hodlr_op_set_solver_graphs)
+    against the same solves with every kernel launched separately: bit-identical
+    solutions and iteration counts on every replay, with eager work and
+    synchronisations between the replays, after a recapture forced by grown
+    work buffers, and with a skeleton-form batch operator."""
+    say("case step_graphs")
+    probs, op, singles = batch
+    K = 4
+    B = np.stack([O.splitmix_uniform(700 + c, op.n) for c in range(K)], axis=1)
+    for mode, x0 in ((H.SOLVE_F64, "b"), (H.SOLVE_MIXED, "zero")):
+        op.set_graphs(False)
+        Xe, ie = op.gmres_batch(B, tol=1e-8, mode=mode, x0=x0)
+        op.set_graphs(True)
+        for rep in range(3):
+            Xg, ig = op.gmres_batch(B, tol=1e-8, mode=mode, x0=x0)
+            say("mode", mode, "replay", rep, "max diff", float(np.max(np.abs(Xg - Xe))))
+            assert np.array_equal(Xg, Xe), (mode, rep)
+            assert [i["iterations"] for i in ig] == [i["iterations"] for i in ie]
+            op.matvec(B[:, 0])  # eager work between replays
+        op.matvec(np.concatenate([B] * 4, axis=1))  # 16 RHS: work buffers regrow, graphs recapture
+        Xg, _ = op.gmres_batch(B, tol=1e-8, mode=mode, x0=x0)
+        assert np.array_equal(Xg, Xe), (mode, "after regrow")
+    trees = [p[1] for p in probs]
+    kerns = [p[2] for p in probs]
+    sk = H.Operator(trees, kerns, 1e-8, sys_size=64)
+    assert sk.stats()["n_skeleton"] > 0
+    sk.set_graphs(False)
+    Xe, _ = sk.gmres_batch(B, tol=1e-8, mode=H.SOLVE_MIXED, x0="zero")
+    sk.set_graphs(True)
+    for rep in range(2):
+        Xg, _ = sk.gmres_batch(B, tol=1e-8, mode=H.SOLVE_MIXED, x0="zero")
+        assert np.array_equal(Xg, Xe), ("skeleton", rep)
