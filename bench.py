@@ -336,9 +336,28 @@ def bench_batch(args, H):
             "traffic_source": measured_traffic(name, bytes_, "matvec_traffic_batch256_k8.json")[1],
             "algorithmic_bytes_per_launch": bytes_, "avg_launch_ms": avg_ms,
         },
+        "mfma": mfma_counters(variants, K),
         "cpu_baseline": cpu,
     }
     return out
+
+
+def mfma_counters(variants, K):
+    """MFMA utilisation of the FP32 K-RHS matrix-core kernels the tuner picked,
+    from the rocprofv3 counter summary committed for this workload (None when
+    no matrix-core form was picked or it was not measured)."""
+    v = variants.get("f32_k%d" % K, 0)
+    names = (["proj_mfma_kernel<float, %d>" % K] if v & 1 else []) + \
+            (["panel_mfma_kernel<float, %d>" % K] if v & 2 else [])
+    path = os.path.join(ROOT, "profiles", "mfma_util_batch256_k8.json")
+    if not names or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tab = json.load(f)
+    out = {n: {"util": tab[n]["mfma_util"], "tflops_issued": tab[n]["mfma_tflops"],
+               "peak_tflops": tab[n]["peak_tflops"]} for n in names if n in tab}
+    return {"kernels": out, "source": "profiles/mfma_util_batch256_k8.json (rocprofv3 SQ_VALU_MFMA_BUSY_CYCLES, "
+                                      "GRBM_GUI_ACTIVE, SQ_INSTS_VALU_MFMA_MOPS_F32)"} if out else None
 
 
 def bench_skeleton(args, H):
