@@ -75,6 +75,52 @@ def measured_traffic(kernel, alg_bytes, name="matvec_traffic_grid100_N1e6.json")
     return None, None
 
 
+def host_cores():
+    """Cores this process may use (capped at the 16 of one GPU's share) and
+    the host CPU model, read at run time."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(n, 16)), model
+
+
+def cpu_parallel_speedup(pts, nmax, eps, nthreads):
+    """Speedup of the oracle's OpenMP matvec (parallel over nodes) on the
+    full level-4 node set of the headline tree's sample: nthreads vs 1."""
+    from oracle import pyoracle as O
+    n = len(pts)
+    T = O.Tree(pts, -np.ones(3), np.ones(3), nmax, 0)
+    k = O.KernelDesc(O.K_INV, 3, 1.0 / n, 1.0)
+    L = T.nlevels
+    lvl = min(4, L - 1)
+    ns = T.level_size(lvl)
+    stride = np.zeros(L, dtype=np.int64)
+    off = np.zeros(L, dtype=np.int64)
+    stride[lvl] = max(1, ns // (16 * nthreads))
+    op = O.Operator(T, k, eps, sys_size=2 * nmax, nthreads=nthreads, sample=(stride, off))
+    x = splitmix_uniform(2, n)
+    t = {}
+    for nt in (1, nthreads):
+        op.matvec(x, nthreads=nt)
+        t1 = time.time()
+        op.matvec(x, nthreads=nt)
+        t[nt] = time.time() - t1
+    return t[1] / t[nthreads]
+
+
 def cpu_baseline(pts, n_mv, nmax, eps, budget_s=20.0):
     """Times the FP64 CPU restatement on a stratified sample of row nodes."""
     from oracle import pyoracle as O
@@ -142,7 +188,7 @@ def from_tree_interleaved(trees, off, M):
     return out
 
 
-def cpu_baseline_batch(specs, nmax, eps, tol, budget_s=25.0):
+def cpu_baseline_batch(specs, nmax, eps, tol, budget_s=25.0, omp=False):
     """The FP64 CPU restatement solving problems of the batch one at a time
     (one thread, reference execution model), over problems spread across the
     batch until the budget is used; solves/s = problems solved / time."""
@@ -165,6 +211,29 @@ def cpu_baseline_batch(specs, nmax, eps, tol, budget_s=25.0):
         del op
         if time.time() - t0 > budget_s:
             break
+    if omp:
+        # all host cores: independent problems solved concurrently, one per
+        # thread (the oracle's calls release the interpreter lock); assembly
+        # untimed as above
+        from concurrent.futures import ThreadPoolExecutor
+        nthreads, model = host_cores()
+        probs = order[:max(len(done), 2 * nthreads)]  # enough problems to keep every thread busy
+        ops = []
+        for p in probs:
+            n, nd = specs[p]
+            X = Wl.problem_points(p, n, nd)
+            T = O.Tree(X, -np.ones(nd), np.ones(nd), nmax, 0)
+            k = O.KernelDesc(O.K_EXP, nd, 1.0 / n, 1.0 + 1.0 / n)
+            ops.append((O.Operator(T, k, eps, sys_size=2 * nmax, nthreads=nthreads),
+                        Wl.splitmix_uniform(11_000 + p, n)))
+        t1 = time.time()
+        with ThreadPoolExecutor(nthreads) as ex:
+            list(ex.map(lambda ob: ob[0].gmres(ob[1], tol=tol, nthreads=1), ops))
+        t_omp = time.time() - t1
+        return {"value": len(probs) / t_omp, "unit": "solves/s", "cores": nthreads, "cpu_model": model,
+                "kind": "port",
+                "sample": "%d problems spread over the batch (the 1-thread sample's first) solved "
+                          "concurrently, one per thread on %d threads, assembly untimed" % (len(probs), nthreads)}
     return {
         "value": len(done) / t_solve,
         "unit": "solves/s",
@@ -284,9 +353,13 @@ def bench_batch(args, H):
     mv32 = (ms[2] + ms[3]) / max(cnt[1], 1)
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)
     cpu = None
+    cpu_omp = None
     if not args.no_cpu:
         log("cpu baseline (oracle, problems one at a time)...")
         cpu = cpu_baseline_batch(specs, args.nmax, args.eps, args.tol, budget_s=args.cpu_budget)
+        log("cpu baseline on all host cores...")
+        cpu_omp = cpu_baseline_batch(specs, args.nmax, args.eps, args.tol, budget_s=args.cpu_budget, omp=True)
+        log("cpu baseline: %.3f solves/s on %d cores" % (cpu_omp["value"], cpu_omp["cores"]))
         log("cpu baseline: %.4g solves/s" % cpu["value"])
     val = NI / latm.mean()
     iters = [i["iterations"] for i in infm]
@@ -338,6 +411,7 @@ def bench_batch(args, H):
         },
         "mfma": mfma_counters(variants, K),
         "cpu_baseline": cpu,
+        "cpu_baseline_omp": cpu_omp,
     }
     return out
 
@@ -726,11 +800,18 @@ def main():
     mv64 = (ms[0] + ms[1]) / max(cnt[0], 1)
 
     n_mv64 = inf64[-1]["matvecs_f64"]
-    cpu = None
+    cpu = cpu_omp = None
     if not args.no_cpu:
         log("cpu baseline (oracle sample)...")
         cpu = cpu_baseline(pts, n_mv64, args.nmax, args.eps)
         log("cpu baseline: %.4g solves/s (%.1fs of CPU work)" % (cpu["value"], cpu["wall_s"]))
+        nthreads, model = host_cores()
+        sp = cpu_parallel_speedup(pts, args.nmax, args.eps, nthreads)
+        cpu_omp = {"value": cpu["value"] * sp, "unit": "solves/s", "cores": nthreads, "cpu_model": model,
+                   "kind": "port",
+                   "sample": "the 1-thread estimate x the oracle's OpenMP matvec speedup %.2f measured on %d "
+                             "threads over a sample of the level-4 nodes" % (sp, nthreads)}
+        log("cpu baseline: %.4g solves/s on %d cores (speedup %.2f)" % (cpu_omp["value"], nthreads, sp))
 
     val = args.steps / latm.sum()
     out = {
@@ -786,6 +867,7 @@ def main():
             "avg_launch_ms": avg_ms,
         },
         "cpu_baseline": cpu,
+        "cpu_baseline_omp": cpu_omp,
     }
     if not args.no_extra:
         # the batched and served forms of the same solve, each on its own
