@@ -33,7 +33,7 @@ class HodlrError(RuntimeError):
         self.code = code
 
 
-MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT = 1, 2, 4
+MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT, MV_PANEL_VEC = 1, 2, 4, 8
 
 
 class KernelDesc(C.Structure):

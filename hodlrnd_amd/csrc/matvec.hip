@@ -195,6 +195,77 @@ __global__ __launch_bounds__(256) void panel_seg_kernel(const PanelItem *__restr
     }
 }
 
+// Vector-ALU panel product with 16-byte loads: lane (o, g) owns rows
+// R o .. R o + R - 1 of the tile (R = 16 / sizeof(T): two FP64 or four FP32
+// rows) in column group g, G = 64 / ceil(h / R) groups, and reads its R
+// rows of a column with one dword-aligned 16-byte load.  The last lane of a
+// column may read up to R - 1 elements past the tile (the next column, or
+// the arena's tail padding) into rows >= h, which the reduction drops.
+// Segments one at a time, the 4 waves interleaving column groups, kU
+// predicated column steps in flight per lane.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void panel_vec_kernel(const PanelItem *__restrict__ items,
+                                                        const Segment *__restrict__ segs,
+                                                        const T *__restrict__ NP, const T *__restrict__ UP,
+                                                        const T *__restrict__ x, const T *__restrict__ t,
+                                                        T *__restrict__ y) {
+    constexpr int R = 16 / (int)sizeof(T), NW = 4, kU = 8;
+    typedef T vec_t __attribute__((ext_vector_type(R), aligned(sizeof(T))));
+    __shared__ T red[NW][kWave][R][K];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const int h = P.h;
+    const int hl = (h + R - 1) / R;  // lanes per column
+    const int G = kWave / hl;
+    const int o = lane % hl, g = lane / hl;
+    const bool active = g < G;
+    const int gg = wid * G + g, GG = NW * G;
+    const int64_t ld = P.ld;
+    T acc[R][K];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[r][k] = T(0);
+    for (int s = 0; s < P.nseg; s++) {
+        const Segment S = segs[P.seg0 + s];
+        const T *M = (S.arena == 0 ? NP : UP) + S.mat + P.r0 + (int64_t)o * R;
+        const T *v = (S.arena == 1 ? t : x) + S.vec * K;
+        if (!active) continue;
+        const int len = S.len;
+        for (int q = gg; q < len; q += kU * GG) {
+            vec_t a[kU];
+            T vv[kU][K];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const int qu = q + u * GG;
+                const bool ok = qu < len;
+                a[u] = ok ? *reinterpret_cast<const vec_t *>(M + (int64_t)qu * ld) : vec_t(T(0));
+#pragma unroll
+                for (int k = 0; k < K; k++) vv[u][k] = ok ? v[(int64_t)qu * K + k] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+#pragma unroll
+                for (int r = 0; r < R; r++)
+#pragma unroll
+                    for (int k = 0; k < K; k++) acc[r][k] = fma(a[u][r], vv[u][k], acc[r][k]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int k = 0; k < K; k++) red[wid][lane][r][k] = active ? acc[r][k] : T(0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < h * K; e += blockDim.x) {
+        const int row = e / K, k = e % K;
+        const int ol = row / R, r = row % R;
+        T sum = T(0);
+        for (int w = 0; w < NW; w++)
+            for (int gi = 0; gi < G; gi++) sum += red[w][gi * hl + ol][r][k];
+        y[(P.y0 + row) * K + k] = sum;
+    }
+}
+
 // Vector-ALU panel product, flattened walk.  Lane (o, g) of a wave owns row o of the tile
 // and column group g (G = 64 / h groups of h lanes).  The leaf's segments
 // are walked as one list of G-column steps (ceil(len / G) per segment); the
@@ -447,6 +518,11 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
         if constexpr (kMfma)
             if (variant & kMatvecPanelMfma) {
                 panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                done = true;
+            }
+        if constexpr (K <= 4)
+            if (!done && (variant & kMatvecPanelVec)) {
+                panel_vec_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
                 done = true;
             }
         if (!done) {

@@ -844,8 +844,12 @@ void assemble(hodlr_op &op) {
     if (verbose())
         std::fprintf(stderr, "[hodlr] layout: near %.3g, U %.3g, V %.3g entries (%.2f GB FP64)\n", (double)op.n_np,
                      (double)op.n_up, (double)op.n_vp, 8e-9 * (double)(op.n_np + op.n_up + op.n_vp));
-    op.NP64.alloc(sizeof(double) * op.n_np);
-    op.UP64.alloc(sizeof(double) * op.n_up);
+    // near-field and U arenas carry zeroed tail padding for the 16-byte
+    // panel loads that run past a tile's last row
+    op.NP64.alloc(sizeof(double) * (op.n_np + kArenaPad));
+    op.UP64.alloc(sizeof(double) * (op.n_up + kArenaPad));
+    HIP_CHECK(hipMemsetAsync(op.NP64.as<double>() + op.n_np, 0, sizeof(double) * kArenaPad, s));
+    HIP_CHECK(hipMemsetAsync(op.UP64.as<double>() + op.n_up, 0, sizeof(double) * kArenaPad, s));
     op.VP64.alloc(sizeof(double) * op.n_vp);
     HIP_CHECK(hipMemsetAsync(op.VP64.p, 0, sizeof(double) * op.n_vp, s));  // padding columns
     op.SK.alloc(sizeof(int32_t) * (size_t)(2 * op.n_sk));
@@ -928,8 +932,10 @@ void assemble(hodlr_op &op) {
     }
     pool.release();
     if (op.prec_flags & HODLR_PREC_F32) {
-        op.NP32.alloc(sizeof(float) * op.n_np);
-        op.UP32.alloc(sizeof(float) * op.n_up);
+        op.NP32.alloc(sizeof(float) * (op.n_np + kArenaPad));
+        op.UP32.alloc(sizeof(float) * (op.n_up + kArenaPad));
+        HIP_CHECK(hipMemsetAsync(op.NP32.as<float>() + op.n_np, 0, sizeof(float) * kArenaPad, s));
+        HIP_CHECK(hipMemsetAsync(op.UP32.as<float>() + op.n_up, 0, sizeof(float) * kArenaPad, s));
         op.VP32.alloc(sizeof(float) * op.n_vp);
         HIP_CHECK(launch_convert(op.NP64.as<double>(), op.NP32.as<float>(), op.n_np, s));
         HIP_CHECK(launch_convert(op.UP64.as<double>(), op.UP32.as<float>(), op.n_up, s));
@@ -1136,6 +1142,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             // run 4-20x slower than the matrix-core form (never chosen)
             std::vector<int> cands;
             if (k <= 8) cands = {0, kMatvecPanelFlat};
+            if (k <= 4) cands.push_back(kMatvecPanelVec);
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
             for (int v : cands) {
@@ -1154,7 +1161,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                     ph[1] += b;
                 }
                 for (auto &e : ev) (void)hipEventDestroy(e);
-                const int bits[2] = {kMatvecProjMfma, kMatvecPanelMfma | kMatvecPanelFlat};
+                const int bits[2] = {kMatvecProjMfma, kMatvecPanelMfma | kMatvecPanelFlat | kMatvecPanelVec};
                 for (int q = 0; q < 2; q++)
                     if (ph[q] < best[q]) {
                         best[q] = ph[q];
@@ -1166,7 +1173,9 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             if (verbose())
                 std::fprintf(stderr, "[hodlr] tune %s K=%d: proj %s %.3f ms, panel %s %.3f ms\n",
                              pi == 0 ? "f64" : "f32", k, pick[0] ? "mfma" : "valu", best[0] / reps,
-                             pick[1] & kMatvecPanelMfma ? "mfma" : (pick[1] ? "flat" : "seg"), best[1] / reps);
+                             pick[1] & kMatvecPanelMfma ? "mfma"
+                                 : (pick[1] & kMatvecPanelVec ? "vec" : (pick[1] ? "flat" : "seg")),
+                             best[1] / reps);
         }
     }
 }

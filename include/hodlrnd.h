@@ -255,13 +255,16 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * (MFMA) projection V^T x, 2 = MFMA panel product (both only with 4 or more
  * RHS), 4 = vector-ALU panel product walking each leaf's segments as one
  * flattened column list (else one segment at a time; suits many short
- * segments).  tune times the forms of each pass on this operator for
+ * segments), 8 = vector-ALU panel product reading 16 bytes per lane and
+ * column (two FP64 / four FP32 rows; up to 4 RHS; MFMA takes precedence
+ * when also set, and 8 over 4).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
  * results agree up to rounding order.  get/set read or force the choice for
  * one (prec, nrhs). */
 #define HODLR_MV_PROJ_MFMA 1
 #define HODLR_MV_PANEL_MFMA 2
 #define HODLR_MV_PANEL_FLAT 4
+#define HODLR_MV_PANEL_VEC 8
 int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps);
 int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant);
 int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);
