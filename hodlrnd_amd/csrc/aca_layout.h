@@ -16,7 +16,7 @@ constexpr int kMatvecProjMfma = 1;
 constexpr int kMatvecPanelMfma = 2;
 constexpr int kMatvecPanelFlat = 4;
 // vector-ALU panel product with one 16-byte load per lane and column (up to
-// 4 right-hand sides); the near-field and U arenas carry kArenaPad elements
+// 8 right-hand sides); the near-field and U arenas carry kArenaPad elements
 // of tail padding for its reads past a tile's last row
 constexpr int kMatvecPanelVec = 8;
 constexpr int64_t kArenaPad = 16;

@@ -209,7 +209,8 @@ __global__ __launch_bounds__(256) void panel_vec_kernel(const PanelItem *__restr
                                                         const T *__restrict__ NP, const T *__restrict__ UP,
                                                         const T *__restrict__ x, const T *__restrict__ t,
                                                         T *__restrict__ y) {
-    constexpr int R = 16 / (int)sizeof(T), NW = 4, kU = 8;
+    // column steps in flight per lane: fewer for more RHS (registers)
+    constexpr int R = 16 / (int)sizeof(T), NW = 4, kU = K <= 2 ? 8 : (K <= 4 ? 4 : 2);
     typedef T vec_t __attribute__((ext_vector_type(R), aligned(sizeof(T))));
     __shared__ T red[NW][kWave][R][K];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
@@ -520,7 +521,7 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
                 panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
                 done = true;
             }
-        if constexpr (K <= 4)
+        if constexpr (K <= 8)
             if (!done && (variant & kMatvecPanelVec)) {
                 panel_vec_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
                 done = true;

@@ -1142,7 +1142,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             // run 4-20x slower than the matrix-core form (never chosen)
             std::vector<int> cands;
             if (k <= 8) cands = {0, kMatvecPanelFlat};
-            if (k <= 4) cands.push_back(kMatvecPanelVec);
+            if (k <= 8) cands.push_back(kMatvecPanelVec);
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
             for (int v : cands) {

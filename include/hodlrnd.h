@@ -256,7 +256,7 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * RHS), 4 = vector-ALU panel product walking each leaf's segments as one
  * flattened column list (else one segment at a time; suits many short
  * segments), 8 = vector-ALU panel product reading 16 bytes per lane and
- * column (two FP64 / four FP32 rows; up to 4 RHS; MFMA takes precedence
+ * column (two FP64 / four FP32 rows; up to 8 RHS; MFMA takes precedence
  * when also set, and 8 over 4).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
  * results agree up to rounding order.  get/set read or force the choice for

@@ -219,9 +219,9 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
     op = A.op
     X = np.stack([O.splitmix_uniform(50 + c, n) for c in range(32)], axis=1)
     y1 = np.stack([op.matvec(X[:, c]) for c in range(4)], axis=1)
-    # the 16-byte-load panel form against the per-segment one, 1..4 RHS
+    # the 16-byte-load panel form against the per-segment one, 1..8 RHS
     for prec, tol in ((H.PREC_F64, 1e-14), (H.PREC_F32, 1e-6)):
-        for k in (1, 2, 4):
+        for k in (1, 2, 4, 8):
             Xk = X[:, :k] if k > 1 else X[:, 0]
             op.set_variant(prec, k, 0)
             a = op.matvec(Xk, prec=prec)
