@@ -33,7 +33,7 @@ class HodlrError(RuntimeError):
         self.code = code
 
 
-MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT, MV_PANEL_VEC = 1, 2, 4, 8
+MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT, MV_PANEL_VEC, MV_PROJ_PACK = 1, 2, 4, 8, 16
 
 
 class KernelDesc(C.Structure):
@@ -73,6 +73,7 @@ class GmresInfo(C.Structure):
 SIGNATURES = {
     "hodlr_version": (C.c_char_p, []),
     "hodlr_ctx_create": (C.c_int, [C.c_int, C.c_void_p]),
+    "hodlr_ctx_set_assembly_staging": (C.c_int, [C.c_void_p, C.c_int64]),
     "hodlr_ctx_destroy": (None, [C.c_void_p]),
     "hodlr_last_error": (C.c_char_p, [C.c_void_p]),
     "hodlr_tree_build": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p,
@@ -184,6 +185,12 @@ class Context:
     def check(self, rc, what=""):
         if rc != OK:
             raise HodlrError(rc, "%s: %s" % (what, lib.hodlr_last_error(self.h).decode()))
+
+    def set_assembly_staging(self, max_bytes=-1):
+        """Cap on the buffer that keeps the ACA rank pass's factors for the
+        fill step: -1 default (a quarter of free memory), 0 = recompute every
+        block in a second ACA pass, > 0 = at most that many bytes."""
+        self.check(lib.hodlr_ctx_set_assembly_staging(self.h, int(max_bytes)), "set_assembly_staging")
 
     def close(self):
         if self.h:

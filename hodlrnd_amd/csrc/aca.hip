@@ -30,6 +30,7 @@ struct Shared {
     double du[BS / kWave][kWave], dv[BS / kWave][kWave];
     int64_t cnt[BS];
     int bcast;
+    int64_t bcast64;
 };
 
 __device__ __forceinline__ bool better(double aa, int ai, double ba, int bi) {
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
                                                  const int64_t *__restrict__ leaf_start,
                                                  const int32_t *__restrict__ leaf_cnt,
                                                  double *__restrict__ UP, double *__restrict__ VP,
-                                                 int32_t *__restrict__ SK) {
+                                                 int32_t *__restrict__ SK, AcaStage S) {
     __shared__ Shared<BS> sh;
     const AcaBlock B = blocks[blockIdx.x];
     const KernelDev K = Ks[B.prob];
@@ -402,6 +403,22 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
         out_rank[B.id] = r;
         out_flags[B.id] = flags | (dense ? 1 : 0);
     }
+    if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors
+        const int64_t len = (int64_t)r * (m + n);
+        if (tid == 0) {
+            const int64_t o = (int64_t)atomicAdd(S.cursor, (unsigned long long)len);
+            const int64_t off = o + len <= S.cap ? o : -1;
+            S.off[B.id] = off;
+            sh.bcast64 = off;
+        }
+        __syncthreads();
+        const int64_t off = sh.bcast64;
+        if (off >= 0) {  // U and V columns 0..r-1 are contiguous in the scratch
+            for (int64_t e = tid; e < (int64_t)r * m; e += BS) S.buf[off + e] = U[e];
+            for (int64_t e = tid; e < (int64_t)r * n; e += BS) S.buf[off + (int64_t)r * m + e] = V[e];
+        }
+        return;
+    }
     if (!fill || dense || (flags & 2)) return;
     __syncthreads();
     const FillDest D = dest[B.id];
@@ -512,7 +529,8 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
                                                       const FillDest *__restrict__ dest,
                                                       const int64_t *__restrict__ leaf_start,
                                                       const int32_t *__restrict__ leaf_cnt,
-                                                      double *__restrict__ UP, double *__restrict__ VP) {
+                                                      double *__restrict__ UP, double *__restrict__ VP,
+                                                      AcaStage S) {
     __shared__ double xr[kWave * kMaxDimDev], xc[kWave * kMaxDimDev];
     __shared__ int64_t prow[kWave];
     const AcaBlock B = blocks[blockIdx.x];
@@ -688,6 +706,23 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
         out_rank[B.id] = r;
         out_flags[B.id] = flags | (dense ? 1 : 0);
     }
+    if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors
+        const int64_t len = (int64_t)r * (m + n);
+        unsigned long long o = 0;
+        if (l == 0) o = atomicAdd(S.cursor, (unsigned long long)len);
+        const int64_t ob = (int64_t)__shfl(o, 0, kWave);
+        const int64_t off = ob + len <= S.cap ? ob : -1;
+        if (l == 0) S.off[B.id] = off;
+        if (off >= 0) {
+#pragma unroll
+            for (int q = 0; q < CAP; q++)
+                if (q < r) {
+                    if (l < m) S.buf[off + (int64_t)q * m + l] = Ur[q];
+                    if (l < n) S.buf[off + (int64_t)r * m + (int64_t)q * n + l] = Vr[q];
+                }
+        }
+        return;
+    }
     if (!fill || dense || (flags & 2)) return;
     const FillDest D = dest[B.id];
     // skeleton blocks are routed to aca_kernel, which writes their pivots;
@@ -711,6 +746,36 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
                 const int tw = min(kProjTile, D.T - tile * kProjTile);
                 VP[D.vp_base + (int64_t)tile * kProjTile * n + row * tw + (col - tile * kProjTile)] = Vr[q];
             }
+    }
+}
+
+// staged factors into their panels: the same destinations as the fill pass
+// (U rows into the row node's leaf-blocked panel, V rows into the column
+// node's tile-major panel); one workgroup per piece of a block
+__global__ __launch_bounds__(256) void stage_copy_kernel(const StageCopy *__restrict__ items,
+                                                         const double *__restrict__ buf,
+                                                         const int64_t *__restrict__ leaf_start,
+                                                         const int32_t *__restrict__ leaf_cnt,
+                                                         double *__restrict__ UP, double *__restrict__ VP) {
+    const StageCopy C = items[blockIdx.x];
+    const int64_t nu = (int64_t)C.r * C.m;
+    for (int64_t e = C.e0 + threadIdx.x; e < C.e1; e += blockDim.x) {
+        const double v = buf[C.src + e];
+        if (e < nu) {
+            const int q = (int)(e / C.m), i = (int)(e % C.m);
+            const int64_t p = C.ord[C.row0l + i];
+            const int64_t ls = leaf_start[p];
+            const int64_t nc = leaf_cnt[p];
+            UP[C.up_base + (int64_t)C.R * (ls - C.row0) + (p - ls) + (int64_t)(C.coff + q) * nc] = v;
+        } else {
+            const int64_t ev = e - nu;
+            const int q = (int)(ev / C.n), j = (int)(ev % C.n);
+            const int64_t row = C.ord[C.col0l + j] - C.col0;
+            const int col = C.voff + q;
+            const int tile = col / kProjTile;
+            const int tw = min(kProjTile, C.T - tile * kProjTile);
+            VP[C.vp_base + (int64_t)tile * kProjTile * C.n + row * tw + (col - tile * kProjTile)] = v;
+        }
     }
 }
 
@@ -800,32 +865,39 @@ hipError_t launch_dense_rows(const int64_t *rows, const int32_t *rprob, int64_t 
 hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,
                       double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
                       const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
-                      int32_t *SK, hipStream_t s) {
+                      int32_t *SK, const AcaStage &S, hipStream_t s) {
     if (nblocks == 0) return hipSuccess;
     dim3 g((unsigned)nblocks);
     switch (bs) {
     case 8:
-        aca_wave_kernel<8><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        aca_wave_kernel<8><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP, S);
         break;
     case 16:
-        aca_wave_kernel<16><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        aca_wave_kernel<16><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP, S);
         break;
     case 32:
-        aca_wave_kernel<32><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP);
+        aca_wave_kernel<32><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP, S);
         break;
     case 64:
         aca_kernel<64><<<g, 64, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP,
-                                                 SK);
+                                                 SK, S);
         break;
     case 256:
         aca_kernel<256><<<g, 256, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP,
-                                                 SK);
+                                                 SK, S);
         break;
     default:
         aca_kernel<1024><<<g, 1024, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP,
-                                                 SK);
+                                                 SK, S);
         break;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_copy(const StageCopy *items, int64_t n, const double *buf, const int64_t *leaf_start,
+                             const int32_t *leaf_cnt, double *UP, double *VP, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    stage_copy_kernel<<<dim3((unsigned)n), 256, 0, s>>>(items, buf, leaf_start, leaf_cnt, UP, VP);
     return hipGetLastError();
 }
 

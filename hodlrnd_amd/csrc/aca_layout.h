@@ -19,6 +19,9 @@ constexpr int kMatvecPanelFlat = 4;
 // 8 right-hand sides); the near-field and U arenas carry kArenaPad elements
 // of tail padding for its reads past a tile's last row
 constexpr int kMatvecPanelVec = 8;
+// row-packed vector-ALU projection (1-2 right-hand sides): narrow V-panel
+// tiles put several rows in one wave load
+constexpr int kMatvecProjPack = 16;
 constexpr int64_t kArenaPad = 16;
 
 // one admissible block for the ACA kernel
@@ -35,7 +38,32 @@ struct AcaBlock {
     int32_t cap;          // rank capacity of the scratch
     int32_t id;           // block id
     uint64_t seed;        // retry stream (replaces rand(), LowRank_matrix.hpp:175,274)
-    int32_t prob, pad;    // problem of the block (kernel descriptor index)
+    int32_t prob;         // problem of the block (kernel descriptor index)
+    int32_t stage;        // rank pass: keep the factors in the staging buffer
+};
+
+// Factor staging of the rank pass.  A compressed block whose factors fit
+// takes r(m + n) doubles of buf at an offset handed out by an atomic
+// cursor: U (m x r, column-major, rows in ACA order) then V (n x r); off[id]
+// records the offset, -1 when the block was not staged (dense, skeleton,
+// out of room).  The fill step then copies staged factors into their panels
+// instead of repeating the ACA.
+struct AcaStage {
+    double *buf;                  // nullptr: no staging
+    unsigned long long *cursor;   // doubles handed out so far (may exceed cap)
+    int64_t cap;                  // doubles in buf
+    int64_t *off;                 // per block id
+};
+
+// one piece of a staged block's factors to copy into the panels: elements
+// [e0, e1) of its r(m + n) staged values (U first, then V)
+struct StageCopy {
+    int64_t src;           // staging offset of the block's U
+    int64_t row0, col0, row0l, col0l;
+    const int32_t *ord;    // ascending-id order map (as AcaBlock)
+    int64_t up_base, vp_base;
+    int32_t m, n, r, R, coff, T, voff;
+    int32_t e0, e1, pad;
 };
 
 // where a block's factors go in the operator (fill pass)

@@ -93,6 +93,12 @@ extern "C" {
 
 const char *hodlr_version(void) { return "hodlrnd-mi355x 0.1 (gfx950)"; }
 
+int hodlr_ctx_set_assembly_staging(hodlr_ctx *ctx, int64_t max_bytes) {
+    if (!ctx || max_bytes < -1) return HODLR_E_ARG;
+    ctx->stage_bytes = max_bytes;
+    return HODLR_OK;
+}
+
 int hodlr_ctx_create(int device, hodlr_ctx **out) {
     if (!out) return HODLR_E_ARG;
     *out = nullptr;
@@ -522,7 +528,7 @@ int hodlr_op_set_solver_graphs(hodlr_op *op, int enable) {
 }
 
 int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant) {
-    if (!op || variant < 0 || variant > 15) return HODLR_E_ARG;
+    if (!op || variant < 0 || variant > 31) return HODLR_E_ARG;
     const int lg = variant_slot(prec, nrhs);
     if (lg < 0) return HODLR_E_ARG;
     op->mv_variant[prec == HODLR_PREC_F64 ? 0 : 1][lg] = variant;

@@ -101,6 +101,9 @@ struct hodlr_ctx {
     int device = 0;
     std::string last_error;
     hipStream_t stream = nullptr;
+    // assembly staging of the rank pass's factors: -1 default room, 0 off,
+    // > 0 at most that many bytes
+    int64_t stage_bytes = -1;
 };
 
 // Uniform-depth 2^d tree (HDD_tree.hpp:23-74).  Nodes are numbered level by

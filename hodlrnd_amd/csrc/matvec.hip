@@ -119,6 +119,82 @@ __global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ 
     }
 }
 
+// Row-packed projection for narrow tiles (1-2 RHS).  A tile of width tw has
+// Q = tw/4 column quads per row; a wave reads RW = 64/Q consecutive rows per
+// load (lane l: quad l % Q, row offset l / Q), so a 100-column tile keeps 50
+// lanes busy instead of 25, and every wave load is one contiguous run of
+// RW*tw elements.  Waves take blocks of RW rows in turn, kU loads in flight;
+// x_j is a per-lane load.  The partial sums of the RW row offsets and the 4
+// waves are added through LDS in a fixed order (one thread per column).
+template <typename T, int K>
+__global__ __launch_bounds__(256) void proj_pack_kernel(const ProjItem *__restrict__ items, int64_t nitems,
+                                                        const T *__restrict__ VP, const T *__restrict__ x,
+                                                        T *__restrict__ t, T *__restrict__ part,
+                                                        const int64_t *__restrict__ tidx) {
+    constexpr int kU = 8;
+    __shared__ T red[4 * kWave][4][K];
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const ProjItem P = items[blockIdx.x];
+    const int ld = P.ld;
+    const int Q = ld >> 2, RW = kWave / Q;
+    const int quad = lane % Q, roff = lane / Q;
+    const bool live = roff < RW && 4 * quad < P.w;
+    T acc[4][K];
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[e][k] = T(0);
+    const T *base = VP + P.vp + 4 * quad;
+    const T *xv = x + P.x0 * K;
+    const int stride = 4 * RW;  // rows between a wave's consecutive loads
+    for (int j0 = wid * RW; j0 < P.rows; j0 += kU * stride) {  // wave-uniform
+        Vec4<T> a[kU];
+        T xj[kU][K];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int j = j0 + u * stride + roff;
+            const bool ok = live && j < P.rows;
+            if (ok) {
+                a[u] = load4(base + (int64_t)j * ld);
+#pragma unroll
+                for (int k = 0; k < K; k++) xj[u][k] = xv[(int64_t)j * K + k];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) a[u].v[e] = T(0);
+#pragma unroll
+                for (int k = 0; k < K; k++) xj[u][k] = T(0);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[e][k] = fma(a[u].v[e], xj[u][k], acc[e][k]);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int k = 0; k < K; k++) red[threadIdx.x][e][k] = acc[e][k];
+    __syncthreads();
+    const int col = threadIdx.x;
+    if (col < P.w) {
+        const int cq = col >> 2, ce = col & 3;
+        T s[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) s[k] = T(0);
+        for (int w = 0; w < 4; w++)
+            for (int r = 0; r < RW; r++)
+#pragma unroll
+                for (int k = 0; k < K; k++) s[k] += red[w * kWave + r * Q + cq][ce][k];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (P.out >= 0) part[(P.out * kProjTile + col) * K + k] = s[k];
+            else t[tidx[P.tcol + col] * K + k] = s[k];
+        }
+    }
+}
+
 template <typename T, int K>
 __global__ __launch_bounds__(256) void proj_reduce_kernel(const ProjReduce *__restrict__ items,
                                                           const T *__restrict__ part, T *__restrict__ t,
@@ -508,6 +584,11 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
         if constexpr (kMfma)
             if (variant & kMatvecProjMfma) {
                 proj_mfma_kernel<T, K><<<dim3((unsigned)(4 * npi)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+                done = true;
+            }
+        if constexpr (K <= 2)
+            if (!done && (variant & kMatvecProjPack)) {
+                proj_pack_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
                 done = true;
             }
         if (!done) proj_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);

@@ -198,7 +198,8 @@ void core_pinv(int r, std::vector<double> &a, double *G) {
 // that the scratch pool stays within `budget` doubles.
 void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vector<int32_t> &cap, int fill,
                   DevBuf &pool, int64_t budget, DevBuf &d_rank, DevBuf &d_flags, const DevBuf *d_dest,
-                  hipStream_t s) {
+                  hipStream_t s, const AcaStage &stage = AcaStage{nullptr, nullptr, 0, nullptr},
+                  const std::vector<uint8_t> *stageable = nullptr) {
     // group by kernel class (largest class first), large blocks first
     // inside a class: bucket by class, then a counting sort on m + n
     static const int kClasses[] = {1024, 256, 64, 32, 16, 8};
@@ -272,7 +273,7 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
                 a.id = (int32_t)b;
                 a.seed = block_seed(op.b_level[b], op.b_rowj[b], op.b_colj[b]);
                 a.prob = p;
-                a.pad = 0;
+                a.stage = (stage.buf && stageable && (*stageable)[b]) ? 1 : 0;
             }
         });
         if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double), s);
@@ -281,7 +282,7 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
                              pool.as<double>(), d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
                              d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
                              op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(),
-                             op.SK.as<int32_t>(), s));
+                             op.SK.as<int32_t>(), stage, s));
         HIP_CHECK(hipStreamSynchronize(s));
         if (verbose())
             std::fprintf(stderr, "[hodlr] aca %s pass: %zu/%zu blocks (workgroup %d, scratch %.2f GB)\n",
@@ -449,8 +450,44 @@ void assemble(hodlr_op &op) {
         op.flags = op.preset_flags;
         todo.clear();
     }
+    // Factor staging: the rank pass keeps the factors of the compressed
+    // blocks that fit (skeleton candidates excluded), and the fill step
+    // copies them into their panels instead of running the ACA again.  The
+    // buffer takes at most a quarter of free memory (1/16 under an
+    // admission hook, where builds run concurrently) or the context's cap;
+    // blocks beyond it are recomputed by the fill pass as before.
+    DevBuf stage_buf, stage_cur, stage_off;
+    AcaStage stage{nullptr, nullptr, 0, nullptr};
+    std::vector<uint8_t> stageable(nb, 0);
+    if (!todo.empty() && op.ctx->stage_bytes != 0) {
+        int64_t need = 0;
+        for (int64_t b = 0; b < nb; b++) {
+            if (op.sys_size >= 0 && op.b_n[b] > op.sys_size) continue;
+            stageable[b] = 1;
+            need += (int64_t)cap[b] * (op.b_m[b] + op.b_n[b]);
+        }
+        const int64_t room = op.ctx->stage_bytes > 0
+                                 ? op.ctx->stage_bytes / (int64_t)sizeof(double)
+                                 : (int64_t)(free_b / (op.admit ? 16 : 4) / sizeof(double));
+        const int64_t capd = std::min(need, room);
+        if (capd > 0) {
+            try {
+                stage_buf.alloc(sizeof(double) * (size_t)capd, s);
+                stage_cur.alloc(sizeof(unsigned long long), s);
+                stage_off.alloc(sizeof(int64_t) * (size_t)(nb > 0 ? nb : 1), s);
+                HIP_CHECK(hipMemsetAsync(stage_cur.p, 0, sizeof(unsigned long long), s));
+                HIP_CHECK(hipMemsetAsync(stage_off.p, 0xff, sizeof(int64_t) * (size_t)(nb > 0 ? nb : 1), s));
+                stage = AcaStage{stage_buf.as<double>(), stage_cur.as<unsigned long long>(), capd,
+                                 stage_off.as<int64_t>()};
+            } catch (const ArgError &) {  // no room: the fill pass recomputes every block
+                stage_buf.release();
+                stage_cur.release();
+                stage_off.release();
+            }
+        }
+    }
     while (!todo.empty()) {
-        run_aca_pass(op, todo, cap, 0, pool, budget, d_rank, d_flags, nullptr, s);
+        run_aca_pass(op, todo, cap, 0, pool, budget, d_rank, d_flags, nullptr, s, stage, &stageable);
         std::vector<int32_t> rk(nb), fl(nb);
         if (nb) {
             HIP_CHECK(hipMemcpyAsync(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));
@@ -469,6 +506,12 @@ void assemble(hodlr_op &op) {
             }
         }
         todo.swap(again);
+    }
+    std::vector<int64_t> soff;  // staging offset per block, -1: not staged
+    if (stage.buf) {
+        soff.resize(nb);
+        HIP_CHECK(hipMemcpyAsync(soff.data(), stage.off, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
     }
     const double t_rank1 = now_s();
     if (verbose()) std::fprintf(stderr, "[hodlr] rank pass %.3f s, %lld blocks\n", t_rank1 - t_rank0, (long long)nb);
@@ -740,11 +783,41 @@ void assemble(hodlr_op &op) {
             if (nch > 1) slots += nch;
         }
     }
-    // largest items first: the hardware's round-robin over XCDs then
-    // balances bytes per XCD
-    std::stable_sort(proj.begin(), proj.end(), [](const ProjItem &a, const ProjItem &b) {
-        return (int64_t)a.rows * a.ld > (int64_t)b.rows * b.ld;
-    });
+    // largest items first, by power-of-two size class: the hardware's
+    // round-robin over XCDs then balances bytes per XCD, and inside a class
+    // the items keep arena order, so concurrently running workgroups read
+    // nearby memory
+    {
+        auto size_class = [](const ProjItem &a) {
+            int64_t v = (int64_t)a.rows * a.ld;
+            int l = 0;
+            while (v > 1) {
+                v >>= 1;
+                l++;
+            }
+            return l;
+        };
+        std::stable_sort(proj.begin(), proj.end(),
+                         [&](const ProjItem &a, const ProjItem &b) { return size_class(a) > size_class(b); });
+    }
+    if (verbose()) {  // projection work by tile height and width
+        const int rcut[4] = {64, 256, 2048, 1 << 30};
+        for (int full = 1; full >= 0; full--)
+            for (int rc = 0; rc < 4; rc++) {
+                int64_t cnt = 0;
+                double el = 0, lanes = 0;
+                for (const ProjItem &it : proj) {
+                    if ((it.ld == kProjTile) != (full == 1) || it.rows > rcut[rc] || (rc && it.rows <= rcut[rc - 1]))
+                        continue;
+                    cnt++;
+                    el += (double)it.rows * it.ld;
+                    lanes += (double)it.rows * kProjTile / 4;
+                }
+                if (cnt)
+                    std::fprintf(stderr, "[hodlr] proj items %s rows<=%d: %lld, %.3g elements, lane use %.2f\n",
+                                 full ? "full-width" : "narrow", rcut[rc], (long long)cnt, el, el / 4 / lanes);
+            }
+    }
     if (nvcol > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many V columns for 32-bit tile offsets"};
     op.n_slots = slots;
 
@@ -841,6 +914,20 @@ void assemble(hodlr_op &op) {
     }
     // ---- 4. allocate and fill ----
     const double t_fill0 = now_s();
+    if (stage.buf) {  // the staged factors must leave room for the arenas
+        size_t fb = 0, tb = 0;
+        HIP_CHECK(hipStreamSynchronize(s));
+        HIP_CHECK(hipMemGetInfo(&fb, &tb));
+        double need = 8.0 * (double)(op.n_np + op.n_up + op.n_vp + 2 * kArenaPad);
+        if (op.prec_flags & HODLR_PREC_F32) need *= 1.5;
+        need += 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk + (double)(1 << 30);
+        if ((double)fb < need) {
+            stage_buf.release();
+            stage.buf = nullptr;
+            soff.clear();
+            if (verbose()) std::fprintf(stderr, "[hodlr] staged factors released for the arenas\n");
+        }
+    }
     if (verbose())
         std::fprintf(stderr, "[hodlr] layout: near %.3g, U %.3g, V %.3g entries (%.2f GB FP64)\n", (double)op.n_np,
                      (double)op.n_up, (double)op.n_vp, 8e-9 * (double)(op.n_np + op.n_up + op.n_vp));
@@ -868,9 +955,75 @@ void assemble(hodlr_op &op) {
         HIP_CHECK(launch_dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), op.kdev.as<KernelDev>(),
                                     op.leaf_start.as<int64_t>(), op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
-        std::vector<int64_t> lowrank;
+        std::vector<int64_t> lowrank, staged;
         for (int64_t b = 0; b < nb; b++)
-            if (!(op.flags[b] & 1)) lowrank.push_back(b);
+            if (!(op.flags[b] & 1)) {
+                if (stage.buf && soff[b] >= 0 && !skel[b]) staged.push_back(b);
+                else lowrank.push_back(b);
+            }
+        if (!staged.empty()) {  // copy staged factors, in pieces of at most 64 Ki values
+            constexpr int64_t kPiece = 1 << 16;
+            std::vector<int64_t> first(staged.size() + 1, 0);
+            for (size_t q = 0; q < staged.size(); q++) {
+                const int64_t b = staged[q];
+                const int64_t len = (int64_t)op.rank[b] * (op.b_m[b] + op.b_n[b]);
+                first[q + 1] = first[q] + std::max<int64_t>(1, (len + kPiece - 1) / kPiece);
+            }
+            constexpr int64_t kBatch = (int64_t)1 << 20;  // items per launch
+            std::vector<StageCopy> items;
+            DevBuf d_items;
+            for (size_t q0 = 0; q0 < staged.size();) {
+                size_t q1 = q0;
+                while (q1 < staged.size() && first[q1 + 1] - first[q0] <= kBatch) q1++;
+                if (q1 == q0) q1 = q0 + 1;
+                items.resize((size_t)(first[q1] - first[q0]));
+                parallel_for(
+                    (int64_t)(q1 - q0),
+                    [&](int64_t a0, int64_t a1) {
+                        for (int64_t a = a0; a < a1; a++) {
+                            const size_t q = q0 + (size_t)a;
+                            const int64_t b = staged[q];
+                            const int p = op.b_prob[b];
+                            const FillDest &D = dest[b];
+                            StageCopy c;
+                            c.src = soff[b];
+                            c.row0 = op.node_start[op.b_row[b]];
+                            c.col0 = op.node_start[op.b_col[b]];
+                            c.row0l = c.row0 - op.pos_base[p];
+                            c.col0l = c.col0 - op.pos_base[p];
+                            c.ord = op.ord.as<int32_t>() + op.ord_off[p] + (int64_t)op.b_level[b] * op.trees[p]->n;
+                            c.up_base = D.up_base;
+                            c.vp_base = D.vp_base;
+                            c.m = op.b_m[b];
+                            c.n = op.b_n[b];
+                            c.r = op.rank[b];
+                            c.R = D.R;
+                            c.coff = D.coff;
+                            c.T = D.T;
+                            c.voff = D.voff;
+                            c.pad = 0;
+                            const int64_t len = (int64_t)c.r * (c.m + c.n);
+                            for (int64_t k = first[q]; k < first[q + 1]; k++) {
+                                const int64_t e0 = (k - first[q]) * kPiece;
+                                c.e0 = (int32_t)e0;
+                                c.e1 = (int32_t)std::min(len, e0 + kPiece);
+                                items[(size_t)(k - first[q0])] = c;
+                            }
+                        }
+                    },
+                    1 << 14);
+                upload(d_items, items, s, true);
+                HIP_CHECK(launch_stage_copy(d_items.as<StageCopy>(), (int64_t)items.size(), stage.buf,
+                                            op.leaf_start.as<int64_t>(), op.leaf_cnt.as<int32_t>(),
+                                            op.UP64.as<double>(), op.VP64.as<double>(), s));
+                HIP_CHECK(hipStreamSynchronize(s));
+                q0 = q1;
+            }
+            if (verbose())
+                std::fprintf(stderr, "[hodlr] staged factors of %zu blocks copied, %zu blocks recomputed\n",
+                             staged.size(), lowrank.size());
+        }
+        stage_buf.release();
         // the fill pass repeats the rank pass with the same caps
         HIP_CHECK(hipMemsetAsync(d_rank.p, 0xff, sizeof(int32_t) * (nb + 1), s));
         run_aca_pass(op, lowrank, cap, 1, pool, budget, d_rank, d_flags, &d_dest, s);
@@ -1143,6 +1296,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             std::vector<int> cands;
             if (k <= 8) cands = {0, kMatvecPanelFlat};
             if (k <= 8) cands.push_back(kMatvecPanelVec);
+            if (k <= 2) cands.push_back(kMatvecProjPack | kMatvecPanelVec);
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
             for (int v : cands) {
@@ -1161,7 +1315,8 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                     ph[1] += b;
                 }
                 for (auto &e : ev) (void)hipEventDestroy(e);
-                const int bits[2] = {kMatvecProjMfma, kMatvecPanelMfma | kMatvecPanelFlat | kMatvecPanelVec};
+                const int bits[2] = {kMatvecProjMfma | kMatvecProjPack,
+                                     kMatvecPanelMfma | kMatvecPanelFlat | kMatvecPanelVec};
                 for (int q = 0; q < 2; q++)
                     if (ph[q] < best[q]) {
                         best[q] = ph[q];
@@ -1172,7 +1327,9 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             work_gen++;
             if (verbose())
                 std::fprintf(stderr, "[hodlr] tune %s K=%d: proj %s %.3f ms, panel %s %.3f ms\n",
-                             pi == 0 ? "f64" : "f32", k, pick[0] ? "mfma" : "valu", best[0] / reps,
+                             pi == 0 ? "f64" : "f32", k,
+                             pick[0] & kMatvecProjMfma ? "mfma" : (pick[0] & kMatvecProjPack ? "pack" : "valu"),
+                             best[0] / reps,
                              pick[1] & kMatvecPanelMfma ? "mfma"
                                  : (pick[1] & kMatvecPanelVec ? "vec" : (pick[1] ? "flat" : "seg")),
                              best[1] / reps);

@@ -13,7 +13,9 @@ namespace hodlr {
 hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,
                       double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
                       const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
-                      int32_t *SK, hipStream_t s);
+                      int32_t *SK, const AcaStage &S, hipStream_t s);
+hipError_t launch_stage_copy(const StageCopy *items, int64_t n, const double *buf, const int64_t *leaf_start,
+                             const int32_t *leaf_cnt, double *UP, double *VP, hipStream_t s);
 hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev *K,
                              const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, hipStream_t s);
 hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev *K,

@@ -134,6 +134,14 @@ int hodlr_ctx_create(int device, hodlr_ctx **out);
 void hodlr_ctx_destroy(hodlr_ctx *ctx);
 const char *hodlr_last_error(hodlr_ctx *ctx);
 const char *hodlr_version(void);
+/* No reference counterpart.  Assembly staging: the ACA rank pass keeps each
+ * compressed block's factors in a staging buffer and the fill step copies
+ * them into the operator instead of repeating the ACA; blocks that do not
+ * fit are recomputed.  Both ways give bit-identical operators.  max_bytes
+ * caps the buffer for the operators this context assembles afterwards:
+ * -1 (default) = a quarter of free device memory (1/16 in a server's
+ * builds), 0 = no staging, > 0 = at most that many bytes. */
+int hodlr_ctx_set_assembly_staging(hodlr_ctx *ctx, int64_t max_bytes);
 
 /* HDD_matrix.hpp:20-33 -> HDD_tree.hpp:23-74: 2^d tree and lists.
  * pts is n x ndim row-major (ptsnD order); lo/hi the bounding box. */
@@ -257,7 +265,8 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * flattened column list (else one segment at a time; suits many short
  * segments), 8 = vector-ALU panel product reading 16 bytes per lane and
  * column (two FP64 / four FP32 rows; up to 8 RHS; MFMA takes precedence
- * when also set, and 8 over 4).  tune times the forms of each pass on this operator for
+ * when also set, and 8 over 4), 16 = vector-ALU projection packing several
+ * rows of a narrow V-panel tile into one wave load (1-2 RHS).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
  * results agree up to rounding order.  get/set read or force the choice for
  * one (prec, nrhs). */
@@ -265,6 +274,7 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
 #define HODLR_MV_PANEL_MFMA 2
 #define HODLR_MV_PANEL_FLAT 4
 #define HODLR_MV_PANEL_VEC 8
+#define HODLR_MV_PROJ_PACK 16
 int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps);
 int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant);
 int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);

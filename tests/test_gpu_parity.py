@@ -229,6 +229,11 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
             b = op.matvec(Xk, prec=prec)
             op.set_variant(prec, k, 0)
             assert relerr(b, a) < tol, (prec, k, relerr(b, a))
+            if k <= 2:  # row-packed projection
+                op.set_variant(prec, k, H.MV_PROJ_PACK)
+                c = op.matvec(Xk, prec=prec)
+                op.set_variant(prec, k, 0)
+                assert relerr(c, a) < tol, (prec, k, relerr(c, a))
     for prec, ks, tol in ((H.PREC_F64, (4, 8, 16), 1e-14), (H.PREC_F32, (4, 8, 16, 32), 1e-6)):
         for k in ks:
             out = {}
@@ -251,4 +256,35 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
     op.tune(reps=1)
     for prec, ks in ((H.PREC_F64, (1, 2, 4, 8, 16)), (H.PREC_F32, (1, 2, 4, 8, 16, 32))):
         for k in ks:
-            assert op.get_variant(prec, k) in range(16)
+            assert op.get_variant(prec, k) in range(32)
+
+
+@pytest.mark.parametrize("nd,n,nmax,eps,sys_size,cap", [
+    (3, 6000, 64, 1e-8, None, -1),        # every compressed block staged
+    (2, 5000, 32, 1e-10, None, 1 << 20),  # 1 MiB: most blocks recomputed by the fill pass
+    (3, 40000, 64, 1e-8, None, -1),       # workgroup-1024 ACA blocks (5000 x 5000) staged
+    (3, 5000, 32, 1e-8, 64, -1),          # skeleton blocks are never staged
+])
+def test_assembly_staging_identical(nd, n, nmax, eps, sys_size, cap):
+    """Factors kept by the ACA rank pass and copied into the panels give the
+    same operator, bit for bit, as a second ACA pass (ranks, FP64 and FP32
+    matvecs, 1 and 4 right-hand sides)."""
+    say("case staging", nd, n, nmax, eps, sys_size, cap)
+    pts, fam, sc, dg, w = problem(nd, n, 700 + n)
+    lo, hi = -np.ones(nd), np.ones(nd)
+    x = O.splitmix_uniform(9, n)
+    X = np.stack([O.splitmix_uniform(20 + c, n) for c in range(4)], axis=1)
+    out = []
+    for staging in (cap, 0):
+        ctx = H.Context(0)
+        ctx.set_assembly_staging(staging)
+        tree = H.Tree(pts, lo, hi, nmax, 0, ctx=ctx)
+        op = H.Operator(tree, H.Kernel(fam, nd, sc, dg), eps, sys_size=sys_size)
+        r, d = op.block_ranks()
+        out.append((r, d, op.matvec(x), op.matvec(x, prec=H.PREC_F32), op.matvec(X), op.matvec(X, prec=H.PREC_F32)))
+        del op, tree
+        ctx.close()
+    a, b = out
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    for u, v in zip(a[2:], b[2:]):
+        assert np.array_equal(u, v), relerr(u, v)
