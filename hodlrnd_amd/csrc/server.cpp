@@ -43,6 +43,10 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr int64_t kChunkRows = 2048;
+// An FP64 residual pass reads the whole FP64 operator for all slots, so it
+// waits until it serves a quarter of the engine's solving columns, or until
+// the oldest waiting column has waited this many ticks (or nothing else runs)
+constexpr int kRefineWait = 4;
 
 void gen_rot(double dx, double dy, double &cs, double &sn) {  // GMRES.hpp:83-110
     if (dy == 0.0) {
@@ -83,6 +87,8 @@ struct Column {
     int phase = 0;   // 0 free, 1 start (x = 0), 2 needs FP64 residual, 3 inner cycle
     bool v0 = false; // inner cycle starts this tick: V_0 = r / beta
     bool finish = false;
+    bool in_a = false;  // its FP64 residual / start is computed this tick
+    int wait = 0;       // ticks waited for an FP64 residual pass
     int i = 0;       // Krylov step inside the cycle
     double normb = 1.0, beta = 0.0, res = 0.0, target = 0.0, hn = 0.0;
     int iterations = 0, outer = 0, mv32 = 0, mv64 = 0;
@@ -237,10 +243,22 @@ struct Engine {
         HIP_CHECK(hipStreamSynchronize(s));  // colbuf is reused by the next admission
     }
 
-    bool needs_a() const {
-        for (auto &c : cols)
-            if (c.phase == 1 || c.phase == 2) return true;
-        return false;
+    // columns starting (x = 0: residual b, no matvec) join every phase A;
+    // refining columns join when the FP64 pass is due (kRefineWait)
+    bool plan_a() {
+        int n1 = 0, n2 = 0, n3 = 0, maxw = 0;
+        for (auto &c : cols) {
+            n1 += c.phase == 1;
+            n2 += c.phase == 2;
+            n3 += c.phase == 3;
+            if (c.phase == 2) maxw = std::max(maxw, c.wait);
+        }
+        const bool refine = n2 > 0 && (n3 == 0 || 4 * n2 >= n2 + n3 || maxw >= kRefineWait);
+        for (auto &c : cols) {
+            c.in_a = c.phase == 1 || (c.phase == 2 && refine);
+            if (c.phase == 2) c.wait = refine ? 0 : c.wait + 1;
+        }
+        return n1 > 0 || refine;
     }
     bool needs_b() const {
         for (auto &c : cols)
@@ -251,16 +269,16 @@ struct Engine {
     // ---- phase A: FP64 residual rr = b - A64 x of starting / refining columns ----
     void launch_a() {
         launched_a = false;
-        if (!needs_a()) return;
+        if (!plan_a()) return;
         HIP_CHECK(hipStreamSynchronize(s));  // staging buffers of the previous tick
         const int64_t nk = n * (int64_t)K;
         bool refine = false;
-        for (auto &c : cols) refine = refine || c.phase == 2;
+        for (auto &c : cols) refine = refine || (c.phase == 2 && c.in_a);
         if (refine) {  // columns just admitted have x = 0: their residual is b exactly
             op.matvec_dev(HODLR_PREC_F64, K, x.p, w64.p, s);
             HIP_CHECK(launch_sub(b.as<double>(), w64.as<double>(), rr.as<double>(), nk, s));
             for (auto &c : cols)
-                if (c.phase == 2) c.mv64++;
+                if (c.phase == 2 && c.in_a) c.mv64++;
         } else {
             HIP_CHECK(hipMemcpyAsync(rr.p, b.p, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
         }
@@ -278,7 +296,7 @@ struct Engine {
         const double *r2 = dn_norm(), *b2 = dn_norm() + K;
         for (int c = 0; c < K; c++) {
             Column &C = cols[c];
-            if (C.phase != 1 && C.phase != 2) continue;
+            if ((C.phase != 1 && C.phase != 2) || !C.in_a) continue;
             if (C.phase == 1) C.normb = std::sqrt(b2[c]) == 0.0 ? 1.0 : std::sqrt(b2[c]);
             else C.outer++;
             C.beta = std::sqrt(r2[c]);
