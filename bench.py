@@ -685,8 +685,6 @@ def main():
     ap.add_argument("--slots", type=int, default=16, help="serve: solve columns per operator engine")
     ap.add_argument("--hbm-budget", type=float, default=0.0, help="serve: GB for operators (0: 90%% of free)")
     ap.add_argument("--builders", type=int, default=4, help="serve: concurrent operator builds")
-    ap.add_argument("--settle", type=int, default=20,
-                    help="seconds to wait before the serving leg (device memory freed by the previous leg)")
     ap.add_argument("--nprob", type=int, default=256)
     ap.add_argument("--plain-all", action="store_true",
                     help="batch256: time the plain FP64 path on every problem, not 64 spread ones")
@@ -887,15 +885,6 @@ def main():
                 ("skeleton_beyond_hbm", ["--workload", "skeleton", "--n1d", "160", "--sys-size", "128",
                                          "--steps", "3", "--warmup", "1"]))
         for key, extra in legs:
-            if key == "serving":
-                # The driver clears device memory a process frees in the
-                # background (~60 GB/s measured); until it is done, new
-                # allocations wait for it, and a server building operators
-                # and engines in a stream of requests runs ~3x slower.  The
-                # batched leg has just freed ~120 GB: let it settle.
-                log("serving leg: waiting %d s for the previous leg's freed device memory to be cleared"
-                    % args.settle)
-                time.sleep(args.settle)
             cmd = [sys.executable, "-u", os.path.abspath(__file__)] + extra + (["--no-cpu"] if args.no_cpu else [])
             try:
                 r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, timeout=900)

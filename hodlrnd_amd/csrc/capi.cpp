@@ -4,6 +4,9 @@
 #include <new>
 
 #include <algorithm>
+#include <iterator>
+#include <map>
+#include <mutex>
 
 #include "op.h"
 #include "solver.h"
@@ -65,6 +68,90 @@ void from_tree(const hodlr_op *op, int k, int K, const double *src, double *dst)
 }  // namespace
 
 namespace hodlr {
+thread_local DevSlab *tl_slab = nullptr;
+
+struct DevSlab::Impl {
+    std::mutex mu;
+    std::map<size_t, size_t> free_run;  // offset -> length, disjoint, never adjacent
+    std::map<size_t, size_t> used;      // offset -> length
+    size_t in_use = 0;
+};
+
+DevSlab::DevSlab(size_t bytes) : impl(new Impl) {
+    bytes &= ~(size_t)255;
+    if (bytes == 0 || hipMalloc(reinterpret_cast<void **>(&base), bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        base = nullptr;
+        delete impl;
+        throw ArgError{HODLR_E_NOMEM, "device slab of " + std::to_string(bytes) + " bytes"};
+    }
+    size = bytes;
+    impl->free_run[0] = bytes;
+}
+
+DevSlab::~DevSlab() {
+    if (base) (void)hipFree(base);
+    delete impl;
+}
+
+void *DevSlab::alloc(size_t nb) {
+    nb = (nb + 255) & ~(size_t)255;
+    std::lock_guard<std::mutex> lk(impl->mu);
+    for (auto it = impl->free_run.begin(); it != impl->free_run.end(); ++it) {
+        if (it->second < nb) continue;
+        const size_t off = it->first, len = it->second;
+        impl->free_run.erase(it);
+        if (len > nb) impl->free_run[off + nb] = len - nb;
+        impl->used[off] = nb;
+        impl->in_use += nb;
+        return base + off;
+    }
+    return nullptr;
+}
+
+void DevSlab::free(void *p) {
+    const size_t off = (size_t)(static_cast<char *>(p) - base);
+    std::lock_guard<std::mutex> lk(impl->mu);
+    auto u = impl->used.find(off);
+    if (u == impl->used.end()) return;  // not from this slab: nothing to do
+    size_t start = off, len = u->second;
+    impl->in_use -= len;
+    impl->used.erase(u);
+    auto nx = impl->free_run.lower_bound(off);
+    if (nx != impl->free_run.end() && nx->first == start + len) {  // merge with the next run
+        len += nx->second;
+        nx = impl->free_run.erase(nx);
+    }
+    if (nx != impl->free_run.begin()) {  // merge with the previous run
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == start) {
+            start = pv->first;
+            len += pv->second;
+            impl->free_run.erase(pv);
+        }
+    }
+    impl->free_run[start] = len;
+}
+
+size_t DevSlab::largest_free() {
+    std::lock_guard<std::mutex> lk(impl->mu);
+    size_t m = 0;
+    for (auto &kv : impl->free_run) m = std::max(m, kv.second);
+    return m;
+}
+
+size_t DevSlab::in_use() {
+    std::lock_guard<std::mutex> lk(impl->mu);
+    return impl->in_use;
+}
+
+size_t device_free_bytes() {
+    if (tl_slab) return tl_slab->largest_free();
+    size_t fr = 0, tot = 0;
+    HIP_CHECK(hipMemGetInfo(&fr, &tot));
+    return fr;
+}
+
 // Stream-ordered temporaries (assembly scratch, work lists) come from the
 // device's default memory pool; keeping up to 8 GiB of freed pool memory
 // mapped spares every later assembly the cost of mapping and clearing it

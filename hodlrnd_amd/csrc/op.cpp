@@ -426,8 +426,7 @@ void assemble(hodlr_op &op) {
     DevBuf d_rank, d_flags, pool;
     d_rank.alloc(sizeof(int32_t) * (nb + 1), s);
     d_flags.alloc(sizeof(int32_t) * (nb + 1), s);
-    size_t free_b = 0, total_b = 0;
-    HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+    const size_t free_b = device_free_bytes();
     // ACA scratch per launch: at most 1/16 of free memory and 4 GiB, so
     // concurrent builds neither crowd out operators nor pay for clearing
     // freshly mapped memory on every chunk
@@ -915,9 +914,13 @@ void assemble(hodlr_op &op) {
     // ---- 4. allocate and fill ----
     const double t_fill0 = now_s();
     if (stage.buf) {  // the staged factors must leave room for the arenas
-        size_t fb = 0, tb = 0;
         HIP_CHECK(hipStreamSynchronize(s));
-        HIP_CHECK(hipMemGetInfo(&fb, &tb));
+        size_t fb = device_free_bytes();
+        if (tl_slab) {  // arenas that do not fit the slab come from the driver
+            size_t fr = 0, tot = 0;
+            HIP_CHECK(hipMemGetInfo(&fr, &tot));
+            fb = std::max(fb, fr);
+        }
         double need = 8.0 * (double)(op.n_np + op.n_up + op.n_vp + 2 * kArenaPad);
         if (op.prec_flags & HODLR_PREC_F32) need *= 1.5;
         need += 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk + (double)(1 << 30);

@@ -488,6 +488,8 @@ struct OpEntry {
 }  // namespace hodlr
 
 struct hodlr_server {
+    // device memory of the builds and engines (declared first: released last)
+    std::unique_ptr<hodlr::DevSlab> slab;
     hodlr_server_config cfg{};
     int device = 0;
     std::vector<std::unique_ptr<hodlr_ctx>> bctx;  // one device + stream per builder
@@ -585,6 +587,7 @@ void hodlr_server::fail_op(OpEntry &e, int code, const std::string &msg) {
 
 void hodlr_server::build_loop(int idx) {
     (void)hipSetDevice(device);
+    hodlr::tl_slab = slab.get();
     // concurrent builds share the host: 16 threads between them
     hodlr::set_assembly_threads(std::max(1, 16 / (int)bctx.size()));
     hodlr_ctx &bc = *bctx[idx];
@@ -691,6 +694,7 @@ void hodlr_server::complete_column(OpEntry &e, int c, std::vector<double> &xt) {
 
 void hodlr_server::sched_loop() {
     (void)hipSetDevice(device);
+    hodlr::tl_slab = slab.get();
     while (true) {
         std::vector<OpEntry *> active;
         std::vector<std::tuple<OpEntry *, int, std::shared_ptr<Request>, int>> admits;
@@ -858,6 +862,18 @@ int hodlr_server_create(int device, const hodlr_server_config *cfg, hodlr_server
         size_t fr = 0, tot = 0;
         HIP_CHECK(hipMemGetInfo(&fr, &tot));
         srv->st.budget_bytes = cfg->hbm_budget > 0 ? cfg->hbm_budget : 0.9 * (double)fr;
+        // one slab for operators, engines and build scratch: the budget plus
+        // room for concurrent builds (allocations that do not fit fall back
+        // to the driver)
+        const double room = std::max(16.0 * (1 << 30), 0.05 * (double)fr);
+        const double sb = std::min(0.95 * (double)fr, srv->st.budget_bytes + room);
+        if (sb >= (double)(1 << 30)) {
+            try {
+                srv->slab = std::make_unique<hodlr::DevSlab>((size_t)sb);
+            } catch (const hodlr::ArgError &) {  // no slab: every buffer comes from the driver
+                srv->slab.reset();
+            }
+        }
         for (int b = 0; b < cfg->builders; b++) srv->builders.emplace_back([srv, b] { srv->build_loop(b); });
         srv->scheduler = std::thread([srv] { srv->sched_loop(); });
     });

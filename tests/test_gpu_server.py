@@ -61,7 +61,7 @@ def check_resid(op, B, X, tol=1e-8):
 
 def test_server_solves_to_tolerance():
     say("case server_basic")
-    srv = H.Server(slots=8, tol=1e-8)
+    srv = H.Server(slots=8, tol=1e-8, hbm_budget=8e9)
     P = pts3(4000, 1)
     oid = srv.submit_build(P, kern(4000))
     B = rhs(4000, 3, 10)
@@ -80,13 +80,13 @@ def test_server_alone_equals_in_mix():
     say("case server_alone_vs_mix")
     P1, P2, P3 = pts3(5000, 2), pts3(3000, 3), pts3(7000, 4)
     B = rhs(5000, 4, 20)
-    alone = H.Server(slots=8)
+    alone = H.Server(slots=8, hbm_budget=8e9)
     o1 = alone.submit_build(P1, kern(5000))
     X_alone, i_alone = alone.wait(alone.submit_solve(o1, B), timeout=120)
     alone.close()
     assert i_alone["converged"] == 1
 
-    mix = H.Server(slots=8)
+    mix = H.Server(slots=8, hbm_budget=8e9)
     m1 = mix.submit_build(P1, kern(5000))
     m2 = mix.submit_build(P2, kern(3000))
     m3 = mix.submit_build(P3, kern(7000))
@@ -111,7 +111,7 @@ def test_server_alone_equals_in_mix():
 def test_server_admission_eviction_rebuild():
     say("case server_admission")
     P1, P2 = pts3(6000, 5), pts3(6000, 6)
-    probe = H.Server(slots=4)
+    probe = H.Server(slots=4, hbm_budget=8e9)
     o = probe.submit_build(P1, kern(6000))
     probe.wait(probe.submit_solve(o, rhs(6000, 1, 80)), timeout=120)
     one = probe.stats()["used_bytes"]
@@ -151,7 +151,7 @@ def test_server_rejections_and_zero_rhs():
     z, rq = np.zeros(10), C.c_int64()
     assert H.lib.hodlr_server_submit_solve(srv.h, 12345, 1, H._p(z), C.byref(rq)) == H.E_ARG
     srv.close()
-    srv = H.Server(slots=2)
+    srv = H.Server(slots=2, hbm_budget=8e9)
     o = srv.submit_build(pts3(2000, 8), kern(2000))
     B = np.zeros((2000, 2))
     B[:, 1] = Wl.splitmix_uniform(3, 2000)
