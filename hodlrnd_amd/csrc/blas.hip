@@ -292,7 +292,8 @@ __global__ __launch_bounds__(256) void bscale_kernel(const double *__restrict__ 
     MACRO(2)                 \
     MACRO(4)                 \
     MACRO(8)                 \
-    MACRO(16)
+    MACRO(16)                \
+    MACRO(32)
 
 hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const Chunk *chunks, int nchunk,
                         double *partial, const int32_t *pchunk, int nprob, int K, double *out, hipStream_t s) {

@@ -412,7 +412,8 @@ int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y) {
     if (!op || !x || !y || nrhs < 1) return HODLR_E_ARG;
     return guard(op->ctx, [&] {
         const int K = padded_rhs(nrhs);
-        if (K < 0 || (prec == HODLR_PREC_F64 && K > 16)) throw ArgError{HODLR_E_ARG, "too many right-hand sides"};
+        if (K < 0 || (prec == HODLR_PREC_F64 && K > 16 && op->n_skel > 0))
+            throw ArgError{HODLR_E_ARG, "too many right-hand sides"};
         if (prec != HODLR_PREC_F64 && prec != HODLR_PREC_F32) throw ArgError{HODLR_E_ARG, "bad precision"};
         set_device(op->ctx);
         const int64_t n = op->N;
@@ -443,7 +444,8 @@ int hodlr_matvec_device(hodlr_op *op, int prec, int nrhs, const void *x, void *y
     if (!op || !x || !y) return HODLR_E_ARG;
     return guard(op->ctx, [&] {
         if (padded_rhs(nrhs) != nrhs) throw ArgError{HODLR_E_ARG, "nrhs must be 1, 2, 4, 8, 16 or 32"};
-        if (prec == HODLR_PREC_F64 && nrhs > 16) throw ArgError{HODLR_E_ARG, "FP64 supports at most 16 RHS"};
+        if (prec == HODLR_PREC_F64 && nrhs > 16 && op->n_skel > 0)
+            throw ArgError{HODLR_E_ARG, "FP64 skeleton operators support at most 16 RHS"};
         if (prec != HODLR_PREC_F64 && prec != HODLR_PREC_F32) throw ArgError{HODLR_E_ARG, "bad precision"};
         set_device(op->ctx);
         op->matvec_dev(prec, nrhs, x, y, stream ? static_cast<hipStream_t>(stream) : op->ctx->stream);

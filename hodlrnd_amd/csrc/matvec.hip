@@ -579,39 +579,51 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
                            int variant, hipEvent_t *ev, hipStream_t s) {
     constexpr bool kMfma = K >= 4;
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (npi) {
-        bool done = false;
-        if constexpr (kMfma)
-            if (variant & kMatvecProjMfma) {
-                proj_mfma_kernel<T, K><<<dim3((unsigned)(4 * npi)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
-                done = true;
+    if constexpr (sizeof(T) == 8 && K > 16) {
+        // FP64 with 32 RHS: matrix-core forms only (the vector-ALU forms'
+        // accumulators and reduction buffers do not fit)
+        if (npi) proj_mfma_kernel<T, K><<<dim3((unsigned)(4 * npi)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+        if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 256, 0, s>>>(pr, part, t, tidx);
+        if (ev) (void)hipEventRecord(ev[1], s);
+        if (npa) panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+        if (ev) (void)hipEventRecord(ev[2], s);
+        (void)variant;
+        return hipGetLastError();
+    } else {
+        if (npi) {
+            bool done = false;
+            if constexpr (kMfma)
+                if (variant & kMatvecProjMfma) {
+                    proj_mfma_kernel<T, K><<<dim3((unsigned)(4 * npi)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+                    done = true;
+                }
+            if constexpr (K <= 2)
+                if (!done && (variant & kMatvecProjPack)) {
+                    proj_pack_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+                    done = true;
+                }
+            if (!done) proj_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
+        }
+        if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 256, 0, s>>>(pr, part, t, tidx);
+        if (ev) (void)hipEventRecord(ev[1], s);
+        if (npa) {
+            bool done = false;
+            if constexpr (kMfma)
+                if (variant & kMatvecPanelMfma) {
+                    panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                    done = true;
+                }
+            if constexpr (K <= 8)
+                if (!done && (variant & kMatvecPanelVec)) {
+                    panel_vec_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                    done = true;
+                }
+            if (!done) {
+                if (variant & kMatvecPanelFlat)
+                    panel_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                else
+                    panel_seg_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
             }
-        if constexpr (K <= 2)
-            if (!done && (variant & kMatvecProjPack)) {
-                proj_pack_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
-                done = true;
-            }
-        if (!done) proj_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
-    }
-    if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 256, 0, s>>>(pr, part, t, tidx);
-    if (ev) (void)hipEventRecord(ev[1], s);
-    if (npa) {
-        bool done = false;
-        if constexpr (kMfma)
-            if (variant & kMatvecPanelMfma) {
-                panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
-                done = true;
-            }
-        if constexpr (K <= 8)
-            if (!done && (variant & kMatvecPanelVec)) {
-                panel_vec_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
-                done = true;
-            }
-        if (!done) {
-            if (variant & kMatvecPanelFlat)
-                panel_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
-            else
-                panel_seg_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
         }
     }
     if (ev) (void)hipEventRecord(ev[2], s);
@@ -634,11 +646,9 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
         HODLR_MV(4)
         HODLR_MV(8)
         HODLR_MV(16)
-    case 32:  // the panel kernel's reduction buffer is 4*64*K*sizeof(T) bytes of LDS
-        if constexpr (sizeof(T) == 4)
-            return launch_matvec_k<T, 32>(pi, npi, pr, npr, pa, npa, segs, VP, NP, UP, x, t,
-                                          part, tidx, y, variant, ev, s);
-        return hipErrorInvalidValue;
+    case 32:  // FP64: matrix-core forms only
+        return launch_matvec_k<T, 32>(pi, npi, pr, npr, pa, npa, segs, VP, NP, UP, x, t, part, tidx, y, variant,
+                                      ev, s);
     default:
         return hipErrorInvalidValue;
     }

@@ -199,7 +199,8 @@ void hodlr_op_destroy(hodlr_op *op);
 
 /* HDD_matrix.hpp:43-47 -> HDD_tree.hpp:86-107: y = A x for nrhs columns
  * (column-major n x nrhs), host buffers in point order. prec = HODLR_PREC_F64
- * or HODLR_PREC_F32. */
+ * or HODLR_PREC_F32.  nrhs <= 32 (at most 16 in FP64 on skeleton
+ * operators); FP64 with 17-32 columns uses the matrix-core kernel forms. */
 int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y);
 /* Same on device buffers in tree order, stream-ordered on `stream` (a
  * hipStream_t, NULL for the default stream).  F64: x,y are double n x nrhs

@@ -246,6 +246,11 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
             e1 = relerr(out[3][:, :4], y1)
             assert e1 < (1e-14 if prec == H.PREC_F64 else 1e-5), (prec, k, e1)
             say(" prec", prec, "K", k, "mfma vs valu", relerr(out[3], out[0]))
+    # FP64 with 32 RHS (matrix-core forms only) against the 4-RHS columns
+    y64 = op.matvec(X, prec=H.PREC_F64)
+    e32 = relerr(y64[:, :4], y1)
+    say(" FP64 K 32 vs 1-RHS columns", e32)
+    assert e32 < 1e-14, e32
     for prec in (H.PREC_F64, H.PREC_F32):  # flattened walk at 1 and 2 RHS
         for k in (1, 2):
             op.set_variant(prec, k, 0)

@@ -159,3 +159,18 @@ def test_server_rejections_and_zero_rhs():
     assert inf["converged"] == 1
     assert not np.any(X[:, 0])
     srv.close()
+
+
+def test_server_32_slots():
+    """32 columns per engine: the FP32 and FP64 32-RHS products (matrix-core
+    forms) serve one request of 32 right-hand sides in one pass."""
+    say("case server_32_slots")
+    srv = H.Server(slots=32, tol=1e-8, hbm_budget=8e9)
+    P = pts3(5000, 11)
+    oid = srv.submit_build(P, kern(5000))
+    B = rhs(5000, 32, 110)
+    X, info = srv.wait(srv.submit_solve(oid, B), timeout=120)
+    say(info)
+    assert info["rc"] == 0 and info["converged"] == 1 and info["max_resid"] < 1e-8
+    check_resid(direct_op(P), B, X)
+    srv.close()
