@@ -427,17 +427,14 @@ void assemble(hodlr_op &op) {
     d_rank.alloc(sizeof(int32_t) * (nb + 1), s);
     d_flags.alloc(sizeof(int32_t) * (nb + 1), s);
     const size_t free_b = device_free_bytes();
-    // ACA scratch per launch.  Under an admission hook (concurrent builds
-    // in a server) at most 1/16 of free memory and 4 GiB, so builds neither
-    // crowd out operators nor pay for clearing freshly mapped memory on
-    // every chunk.  A lone assembly takes up to 1/8 of free memory and
-    // 32 GiB: a chunk of large blocks (m + n ~ 10^4..10^5, rank capacity
-    // 128) then holds enough of them to fill the chip, where 4 GiB gave
-    // launches of 16-512 workgroups.
-    const int64_t kGiB = (int64_t)1 << 30;
+    // ACA scratch per launch: at most 1/16 of free memory and 4 GiB, so
+    // concurrent builds neither crowd out operators nor pay for clearing
+    // freshly mapped memory on every chunk.  (Up to 32 GiB per launch cut
+    // the ACA kernels' time at N = 10^6 from 1.09 s to 0.69 s, but the
+    // assembly took longer: 1.7 s instead of 1.45 s for the rank pass, once
+    // 7.1 s, allocating the larger pool in a process that followed another.)
     const int64_t budget = std::max<int64_t>(
-        op.admit ? std::min<int64_t>((int64_t)(free_b / 16), 4 * kGiB) / (int64_t)sizeof(double)
-                 : std::min<int64_t>((int64_t)(free_b / 8), 32 * kGiB) / (int64_t)sizeof(double),
+        std::min<int64_t>((int64_t)(free_b / 16 / sizeof(double)), ((int64_t)4 << 30) / (int64_t)sizeof(double)),
         (int64_t)1 << 24);
     std::vector<int32_t> cap(nb);
     std::vector<int64_t> todo(nb);
