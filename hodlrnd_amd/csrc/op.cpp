@@ -801,6 +801,23 @@ void assemble(hodlr_op &op) {
         };
         std::stable_sort(proj.begin(), proj.end(),
                          [&](const ProjItem &a, const ProjItem &b) { return size_class(a) > size_class(b); });
+        // Workgroup i runs on XCD i % 8.  Inside each class, deal the
+        // items so that every XCD gets one contiguous run of them: nearby
+        // column nodes then share an XCD's L2, where their scattered
+        // writes to the same t lines (row nodes near them) combine.
+        constexpr int kXcd = 8;
+        std::vector<ProjItem> dealt(proj.size());
+        size_t c0 = 0;
+        while (c0 < proj.size()) {
+            size_t c1 = c0;
+            while (c1 < proj.size() && size_class(proj[c1]) == size_class(proj[c0])) c1++;
+            size_t next = c0;  // next item of the class in arena order
+            for (int x = 0; x < kXcd; x++)
+                for (size_t p = c0; p < c1; p++)
+                    if ((int)(p % kXcd) == x) dealt[p] = proj[next++];
+            c0 = c1;
+        }
+        proj.swap(dealt);
     }
     if (verbose()) {  // projection work by tile height and width
         const int rcut[4] = {64, 256, 2048, 1 << 30};
