@@ -69,6 +69,7 @@ void from_tree(const hodlr_op *op, int k, int K, const double *src, double *dst)
 
 namespace hodlr {
 thread_local DevSlab *tl_slab = nullptr;
+thread_local bool tl_slab_synced = false;
 
 struct DevSlab::Impl {
     std::mutex mu;

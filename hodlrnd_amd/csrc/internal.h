@@ -58,6 +58,10 @@ struct DevSlab {
 // the driver); a buffer that does not fit in it falls back to the driver
 extern thread_local DevSlab *tl_slab;
 
+// set while a thread releases buffers after it has synchronised the device
+// itself (one wait for a whole operator or engine instead of one per buffer)
+extern thread_local bool tl_slab_synced;
+
 // free device memory for sizing decisions: the slab's largest free run on a
 // slab thread, else the driver's free memory
 size_t device_free_bytes();
@@ -118,7 +122,7 @@ struct DevBuf {
     }
     void release() {
         if (p && slab) {
-            (void)(fs ? hipStreamSynchronize(fs) : hipDeviceSynchronize());
+            if (!tl_slab_synced) (void)(fs ? hipStreamSynchronize(fs) : hipDeviceSynchronize());
             slab->free(p);
         } else if (p) {
             (void)(fs ? hipFreeAsync(p, fs) : hipFree(p));

@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 
 namespace hodlr {
 
@@ -488,8 +489,47 @@ void assemble(hodlr_op &op) {
             }
         }
     }
+    bool first_pass = true;
     while (!todo.empty()) {
-        run_aca_pass(op, todo, cap, 0, pool, budget, d_rank, d_flags, nullptr, s, stage, &stageable);
+        std::vector<int64_t> big, rest;
+        if (first_pass)
+            for (int64_t b : todo) (aca_class(op, b) == 1024 ? big : rest).push_back(b);
+        first_pass = false;
+        if (!big.empty() && !rest.empty()) {
+            // The largest blocks (workgroup-1024 class) put few workgroups on
+            // the chip per launch; a second host thread drives them on a
+            // second stream while this one runs the other classes.
+            HIP_CHECK(hipStreamSynchronize(s));  // rank, flag and staging buffers ready
+            hipStream_t s2 = nullptr;
+            HIP_CHECK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+            DevSlab *slab = tl_slab;
+            const int threads = tl_threads;
+            std::exception_ptr err2;
+            std::thread th([&] {
+                tl_slab = slab;
+                tl_threads = threads;
+                DevBuf pool2;
+                try {
+                    run_aca_pass(op, big, cap, 0, pool2, budget, d_rank, d_flags, nullptr, s2, stage, &stageable);
+                    (void)hipStreamSynchronize(s2);
+                } catch (...) {
+                    err2 = std::current_exception();
+                }
+            });
+            std::exception_ptr err1;
+            try {
+                run_aca_pass(op, rest, cap, 0, pool, budget, d_rank, d_flags, nullptr, s, stage, &stageable);
+            } catch (...) {
+                err1 = std::current_exception();
+            }
+            th.join();
+            (void)hipStreamSynchronize(s2);
+            (void)hipStreamDestroy(s2);
+            if (err1) std::rethrow_exception(err1);
+            if (err2) std::rethrow_exception(err2);
+        } else {
+            run_aca_pass(op, todo, cap, 0, pool, budget, d_rank, d_flags, nullptr, s, stage, &stageable);
+        }
         std::vector<int32_t> rk(nb), fl(nb);
         if (nb) {
             HIP_CHECK(hipMemcpyAsync(rk.data(), d_rank.p, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, s));

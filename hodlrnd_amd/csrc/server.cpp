@@ -526,8 +526,13 @@ using hodlr::Request;
 // flight for it: refs == 0 or the scheduler has dropped it)
 void hodlr_server::free_op(OpEntry &e) {
     (void)hipSetDevice(device);
+    // one device-wide wait, then the slab takes the ranges back without a
+    // wait per buffer
+    (void)hipDeviceSynchronize();
+    hodlr::tl_slab_synced = true;
     e.engine.reset();
     if (e.op) hodlr_op_destroy(e.op);
+    hodlr::tl_slab_synced = false;
     if (e.tree) hodlr_tree_destroy(e.tree);
     e.op = nullptr;
     e.tree = nullptr;
