@@ -699,8 +699,8 @@ def main():
                          "known-size kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one "
-              "MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (independent problems "
+              "would shard by problem, one operator by row ownership); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import hodlrnd_amd as H
