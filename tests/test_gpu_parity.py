@@ -208,7 +208,7 @@ def test_cpp_api_reference_driver():
     assert mv < 1e-8 and sol < 1e-8
 
 
-@pytest.mark.parametrize("nd,n,nmax", [(3, 6000, 64), (2, 5000, 32)])
+@pytest.mark.parametrize("nd,n,nmax", [(3, 6000, 64), (2, 5000, 32), (2, 5000, 128)])
 def test_matvec_kernel_forms_agree(nd, n, nmax):
     """Vector-ALU and matrix-core (MFMA) forms of both matvec phases give the
     same product for 4..32 right-hand sides: FP64 to 1e-14, FP32 to 1e-6
