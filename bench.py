@@ -870,6 +870,15 @@ def main():
             "algorithmic_bytes_per_launch": bytes_,
             "avg_launch_ms": avg_ms,
         },
+        # every pass of the timed mixed solves (the FP32 projection and panel
+        # product take nearly the same time, so which one is dominant varies)
+        "matvec_kernels": {
+            kn: {"avg_launch_ms": tm / max(cn, 1), "launches": int(cn),
+                 "algorithmic_bytes_per_launch": by,
+                 "achieved_gbs": by / (tm / max(cn, 1) * 1e-3) * 1e-9 if tm else None,
+                 "frac": by / (tm / max(cn, 1) * 1e-3) * 1e-9 / HBM_PEAK_GBS if tm else None,
+                 "traffic": measured_traffic(kn, by)[0]}
+            for kn, tm, cn, by in cands},
         "cpu_baseline": cpu,
         "cpu_baseline_omp": cpu_omp,
     }
