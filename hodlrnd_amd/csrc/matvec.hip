@@ -60,7 +60,7 @@ template <typename T, int K>
 __global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ items, int64_t nitems,
                                                    const T *__restrict__ VP, const T *__restrict__ x,
                                                    T *__restrict__ t, T *__restrict__ part,
-                                                   const int64_t *__restrict__ tidx) {
+                                                   const int32_t *__restrict__ tidx) {
     constexpr int U = 4;  // rows in flight per wave per step
     __shared__ T red[kProjTile][K];
     // uniform wave index: the row loop is scalar, x_b[j] a scalar load
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ 
     for (int q = threadIdx.x; q < P.w * K; q += blockDim.x) {
         const int col = q / K, k = q % K;
         if (P.out >= 0) part[(P.out * kProjTile + col) * K + k] = red[col][k];
-        else t[tidx[P.tcol + col] * K + k] = red[col][k];
+        else t[(int64_t)tidx[P.tcol + col] * K + k] = red[col][k];
     }
 }
 
@@ -130,7 +130,7 @@ template <typename T, int K>
 __global__ __launch_bounds__(256) void proj_pack_kernel(const ProjItem *__restrict__ items, int64_t nitems,
                                                         const T *__restrict__ VP, const T *__restrict__ x,
                                                         T *__restrict__ t, T *__restrict__ part,
-                                                        const int64_t *__restrict__ tidx) {
+                                                        const int32_t *__restrict__ tidx) {
     constexpr int kU = 8;
     __shared__ T red[4 * kWave][4][K];
     const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void proj_pack_kernel(const ProjItem *__restri
 #pragma unroll
         for (int k = 0; k < K; k++) {
             if (P.out >= 0) part[(P.out * kProjTile + col) * K + k] = s[k];
-            else t[tidx[P.tcol + col] * K + k] = s[k];
+            else t[(int64_t)tidx[P.tcol + col] * K + k] = s[k];
         }
     }
 }
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void proj_pack_kernel(const ProjItem *__restri
 template <typename T, int K>
 __global__ __launch_bounds__(256) void proj_reduce_kernel(const ProjReduce *__restrict__ items,
                                                           const T *__restrict__ part, T *__restrict__ t,
-                                                          const int64_t *__restrict__ tidx) {
+                                                          const int32_t *__restrict__ tidx) {
     const ProjReduce R = items[blockIdx.x];
     const int col = threadIdx.x;
     if (col >= R.w) return;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void proj_reduce_kernel(const ProjReduce *__re
     for (int c = 0; c < R.nchunks; c++)
 #pragma unroll
         for (int k = 0; k < K; k++) acc[k] += part[((R.slot0 + c) * kProjTile + col) * K + k];
-    const int64_t ti = tidx[R.tcol + col];
+    const int64_t ti = tidx[R.tcol + col];  // widened before the multiply
 #pragma unroll
     for (int k = 0; k < K; k++) t[ti * K + k] = acc[k];
 }
@@ -525,7 +525,7 @@ template <typename T, int K>
 __global__ __launch_bounds__(256) void proj_mfma_kernel(const ProjItem *__restrict__ items, int64_t nitems,
                                                         const T *__restrict__ VP, const T *__restrict__ x,
                                                         T *__restrict__ t, T *__restrict__ part,
-                                                        const int64_t *__restrict__ tidx) {
+                                                        const int32_t *__restrict__ tidx) {
     using MF = Mfma<T>;
     constexpr int NK = (K + 15) / 16, kU = 8;
     const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(256) void proj_mfma_kernel(const ProjItem *__restri
             const int oc = c0 + MF::row(lane, q), j = n * 16 + i16;
             if (oc < P.w && j < K) {
                 if (P.out >= 0) part[(P.out * kProjTile + oc) * K + j] = acc[n][q];
-                else t[tidx[P.tcol + oc] * K + j] = acc[n][q];
+                else t[(int64_t)tidx[P.tcol + oc] * K + j] = acc[n][q];
             }
         }
 }
@@ -575,7 +575,7 @@ template <typename T, int K>
 hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                            const PanelItem *pa, int64_t npa,
                            const Segment *segs, const T *VP, const T *NP, const T *UP,
-                           const T *x, T *t, T *part, const int64_t *tidx, T *y,
+                           const T *x, T *t, T *part, const int32_t *tidx, T *y,
                            int variant, hipEvent_t *ev, hipStream_t s) {
     constexpr bool kMfma = K >= 4;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -634,7 +634,7 @@ template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                          const PanelItem *pa, int64_t npa,
                          const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
-                         T *t, T *part, const int64_t *tidx, T *y, int variant, hipEvent_t *ev,
+                         T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
                          hipStream_t s) {
 #define HODLR_MV(KK)                                                                                 \
     case KK:                                                                                         \
@@ -658,12 +658,12 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
 template hipError_t launch_matvec<double>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
                                           const PanelItem *, int64_t,
                                           const Segment *, const double *, const double *, const double *,
-                                          const double *, double *, double *, const int64_t *, double *,
+                                          const double *, double *, double *, const int32_t *, double *,
                                           int, hipEvent_t *, hipStream_t);
 template hipError_t launch_matvec<float>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
                                          const PanelItem *, int64_t,
                                          const Segment *, const float *, const float *, const float *,
-                                         const float *, float *, float *, const int64_t *, float *,
+                                         const float *, float *, float *, const int32_t *, float *,
                                          int, hipEvent_t *, hipStream_t);
 
 }  // namespace hodlr

@@ -758,7 +758,8 @@ void assemble(hodlr_op &op) {
     op.e_near = (double)op.n_np;
 
     // projection tiles over the V panels; t index of every V column
-    std::vector<int64_t> tidx(nvcol > 0 ? nvcol : 1, 0);
+    // t position of every V column (32-bit: read once per column per matvec)
+    std::vector<int32_t> tidx(nvcol > 0 ? nvcol : 1, 0);
     std::vector<DenseBlock> dense;
     std::vector<FillDest> dest(nb > 0 ? nb : 1);
     op.t_lowrank = 0;
@@ -787,7 +788,10 @@ void assemble(hodlr_op &op) {
             db.pad = 0;
             dense.push_back(db);
         } else if (!skel[b]) {
-            for (int k = 0; k < op.rank[b]; k++) tidx[vcol_base[c] + voff[b] + k] = t_base[a] + tcoff[b] + k;
+            if (t_base[a] + tcoff[b] + op.rank[b] > INT32_MAX)
+                throw ArgError{HODLR_E_ARG, "more than 2^31 low-rank coefficients per right-hand side"};
+            for (int k = 0; k < op.rank[b]; k++)
+                tidx[vcol_base[c] + voff[b] + k] = (int32_t)(t_base[a] + tcoff[b] + k);
             op.t_lowrank += op.rank[b];
         }
     }
@@ -1287,7 +1291,7 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
                                         panel.as<PanelItem>(), n_panel,
                                         segs.as<Segment>(), VP64.as<double>(), NP64.as<double>(),
                                         UP64.as<double>(), static_cast<const double *>(x), t64.as<double>(),
-                                        part64.as<double>(), tidx.as<int64_t>(), static_cast<double *>(y),
+                                        part64.as<double>(), tidx.as<int32_t>(), static_cast<double *>(y),
                                         variant_for(prec, nrhs), ev, s));
         if (n_skcore)
             HIP_CHECK(launch_skeleton<double>(nrhs, skel_plan(), static_cast<const double *>(x), skt1.as<double>(),
@@ -1299,7 +1303,7 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
                                        panel.as<PanelItem>(), n_panel,
                                        segs.as<Segment>(), VP32.as<float>(), NP32.as<float>(), UP32.as<float>(),
                                        static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
-                                       tidx.as<int64_t>(), static_cast<float *>(y), variant_for(prec, nrhs), ev,
+                                       tidx.as<int32_t>(), static_cast<float *>(y), variant_for(prec, nrhs), ev,
                                        s));
         if (n_skcore)
             HIP_CHECK(launch_skeleton<float>(nrhs, skel_plan(), static_cast<const float *>(x), skt1.as<double>(),

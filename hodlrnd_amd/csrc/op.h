@@ -32,7 +32,7 @@ template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                          const PanelItem *pa, int64_t npa,
                          const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
-                         T *t, T *part, const int64_t *tidx, T *y, int variant, hipEvent_t *ev,
+                         T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
                          hipStream_t s);
 
 // skeleton blocks' on-the-fly product, added into y (t1, t2: FP64 slot
