@@ -857,6 +857,10 @@ def main():
         "aca_rank_pass_s": st["rank_seconds"],
         "fill_s": st["fill_seconds"],
         "operator_gb": {"f64": st["bytes_f64"] * 1e-9, "f32": st["bytes_f32"] * 1e-9},
+        # the reference's print_matrix_statistics figures (HDD_matrix.hpp:69-85)
+        "operator_stats": {"n_flop_ref": st["n_flop_ref"], "max_rank": int(st["max_rank"]),
+                           "blocks": int(st["n_blocks"]), "dense_blocks": int(st["n_dense"]),
+                           "e_near": st["e_near"], "e_far": st["e_far"]},
         "roofline": {
             "bound": "hbm",
             "kernel": name,
