@@ -317,8 +317,18 @@ __global__ __launch_bounds__(256) void panel_vec_kernel(const PanelItem *__restr
                 const int qu = q + u * GG;
                 const bool ok = qu < len;
                 a[u] = ok ? *reinterpret_cast<const vec_t *>(M + (int64_t)qu * ld) : vec_t(T(0));
+                if constexpr (K % R == 0) {  // the column's K values as K/R 16-byte loads
 #pragma unroll
-                for (int k = 0; k < K; k++) vv[u][k] = ok ? v[(int64_t)qu * K + k] : T(0);
+                    for (int kk = 0; kk < K / R; kk++) {
+                        const vec_t w = ok ? *reinterpret_cast<const vec_t *>(v + (int64_t)qu * K + kk * R)
+                                           : vec_t(T(0));
+#pragma unroll
+                        for (int r = 0; r < R; r++) vv[u][kk * R + r] = w[r];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; k++) vv[u][k] = ok ? v[(int64_t)qu * K + k] : T(0);
+                }
             }
 #pragma unroll
             for (int u = 0; u < kU; u++)
