@@ -170,7 +170,7 @@ struct hodlr_op {
 
     // kernel form per (precision, RHS count): kMatvec* bits, indexed
     // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller
-    int mv_variant[2][6] = {{8, 8, 2, 2, 2, 3}, {8, 8, 2, 2, 2, 3}};
+    int mv_variant[2][6] = {{8, 8, 8, 2, 2, 3}, {8, 8, 8, 2, 2, 3}};
     int variant_for(int prec, int nrhs) const;
     // time every kernel form per phase on this operator and keep the fastest
     void tune_matvec(int prec_mask, int reps);
