@@ -59,15 +59,18 @@ def splitmix_uniform(seed, n):
     return Wl.splitmix_uniform(seed, n)
 
 
-def measured_traffic(kernel, alg_bytes, name="matvec_traffic_grid100_N1e6.json"):
+def measured_traffic(kernel, alg_bytes, name="matvec_traffic_grid100_N1e6.json", kernel_name=None):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE and WRITE_SIZE passes scaled by the copy-kernel factors),
     used only if it was measured on this exact operator (same algorithmic
-    bytes per launch); otherwise None."""
+    bytes per launch) and, when kernel_name is given, with the same kernel
+    form; otherwise None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         d = json.load(open(path))
         e = d["kernels"][kernel]
+        if kernel_name is not None and e["kernel"] != kernel_name:
+            return None, None
         if abs(e["algorithmic_bytes_per_launch"] - alg_bytes) <= 1e-3 * alg_bytes:
             return float(e["hbm_bytes_per_launch"]), "profiles/" + name
     except (OSError, KeyError, ValueError):
@@ -354,6 +357,10 @@ def bench_batch(args, H):
     name, tot_ms, c, bytes_ = max(cands, key=lambda t: t[1])
     avg_ms = tot_ms / max(c, 1)
     achieved = bytes_ / (avg_ms * 1e-3) * 1e-9
+    pt = name[-3:]
+    vk = variants.get("%s_k%d" % (pt, K), 0)
+    kname = (panel_kernel_name if "panel" in name else proj_kernel_name)(vk, K, "float" if pt == "f32" else "double")
+    traffic_b, traffic_b_src = measured_traffic(name, bytes_, "matvec_traffic_batch256_k8.json", kname)
     mv32 = (ms[2] + ms[3]) / max(cnt[1], 1)
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)
     cpu = None
@@ -407,10 +414,10 @@ def bench_batch(args, H):
         "kernel_forms": variants,
         "operator_gb": {"f64": st["bytes_f64"] * 1e-9, "f32": st["bytes_f32"] * 1e-9},
         "roofline": {
-            "bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "measured_copy_gbs": copy_gbs,
-            "traffic": measured_traffic(name, bytes_, "matvec_traffic_batch256_k8.json")[0],
-            "traffic_source": measured_traffic(name, bytes_, "matvec_traffic_batch256_k8.json")[1],
+            "bound": "hbm", "kernel": name, "kernel_form": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "measured_copy_gbs": copy_gbs,
+            "traffic": traffic_b,
+            "traffic_source": traffic_b_src,
             "algorithmic_bytes_per_launch": bytes_, "avg_launch_ms": avg_ms,
         },
         "mfma": mfma_counters(variants, K),
@@ -420,13 +427,29 @@ def bench_batch(args, H):
     return out
 
 
+def panel_kernel_name(v, K, t="float"):
+    """The panel-product kernel that kernel form v runs at K right-hand sides."""
+    if v & 2 and K >= 4:
+        return "%s<%s, %d>" % ("panel_mfma_w_kernel" if v & 32 else "panel_mfma_kernel", t, K)
+    if v & 8 and K <= 8:
+        return "panel_vec_kernel<%s, %d>" % (t, K)
+    return "%s<%s, %d>" % ("panel_kernel" if v & 4 else "panel_seg_kernel", t, K)
+
+
+def proj_kernel_name(v, K, t="float"):
+    """The projection kernel that kernel form v runs at K right-hand sides."""
+    if v & 1 and K >= 4:
+        return "proj_mfma_kernel<%s, %d>" % (t, K)
+    return "%s<%s, %d>" % ("proj_pack_kernel" if v & 16 and K <= 2 else "proj_kernel", t, K)
+
+
 def mfma_counters(variants, K):
     """MFMA utilisation of the FP32 K-RHS matrix-core kernels the tuner picked,
     from the rocprofv3 counter summary committed for this workload (None when
     no matrix-core form was picked or it was not measured)."""
     v = variants.get("f32_k%d" % K, 0)
     names = (["proj_mfma_kernel<float, %d>" % K] if v & 1 else []) + \
-            (["panel_mfma_kernel<float, %d>" % K] if v & 2 else [])
+            ([panel_kernel_name(v, K)] if v & 2 else [])
     path = os.path.join(ROOT, "profiles", "mfma_util_batch256_k8.json")
     if not names or not os.path.exists(path):
         return None

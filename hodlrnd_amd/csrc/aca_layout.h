@@ -22,6 +22,9 @@ constexpr int kMatvecPanelVec = 8;
 // row-packed vector-ALU projection (1-2 right-hand sides): narrow V-panel
 // tiles put several rows in one wave load
 constexpr int kMatvecProjPack = 16;
+// matrix-core panel product loading its operands with 16-byte loads through
+// a wave-private LDS stage (with kMatvecPanelMfma)
+constexpr int kMatvecPanelWide = 32;
 constexpr int64_t kArenaPad = 16;
 
 // one admissible block for the ACA kernel
@@ -131,6 +134,16 @@ struct PanelItem {
     int32_t r0;           // row offset of the tile inside the leaf
     int32_t nq4;          // sum over segments of ceil(len / 4)
     int32_t nqg;          // sum over segments of ceil(len / G), G = 64 / h
+    int64_t q0;           // first quad descriptor of the leaf
+};
+
+// one group of up to 4 consecutive columns of a leaf's segment list: column
+// c < ncol is arena element mat + c*ld + row (arena NP if kind == 0, else
+// UP) and multiplies vector element vec + c (x if kind != 1, else t)
+struct QuadDesc {
+    int64_t mat;
+    int32_t vec;
+    int32_t info;         // kind | ncol << 8
 };
 
 // one contiguous ld x len column-major matrix times a vector slice:

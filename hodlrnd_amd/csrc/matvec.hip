@@ -527,6 +527,109 @@ __global__ __launch_bounds__(256) void panel_mfma_kernel(const PanelItem *__rest
     }
 }
 
+
+// Matrix-core panel product with 16-byte operand loads.  In
+// panel_mfma_kernel every A and B fragment is its own 4-byte-per-lane load:
+// 32 memory instructions per 16 quads, and the kernel streams at ~3.4 TB/s
+// on any operator (cutting the vector-ALU form's instruction count the
+// same way made it 1.4x faster).  Here a wave stages a group of QG quads
+// at a time into its own LDS area and reads the MFMA fragments from there.
+// The LPQ = 64 / QG lanes of a quad load its descriptor, then, with one
+// 16-byte load per lane and instruction, column m's 16 tile rows (64 or
+// 128 contiguous bytes) for m = 0..3, and its 4 x K vector values (also
+// contiguous): 1 + 4 + K/4 loads per lane and group instead of 32.  Rows
+// past the tile are read (the next column or the arena's tail padding)
+// and land only in discarded accumulator rows.  (Issuing the next group's
+// loads before the current group's products measured slower.)
+template <typename T, int K>
+__global__ __launch_bounds__(256) void panel_mfma_w_kernel(const PanelItem *__restrict__ items,
+                                                           const QuadDesc *__restrict__ quads,
+                                                           const T *__restrict__ NP, const T *__restrict__ UP,
+                                                           const T *__restrict__ x, const T *__restrict__ t,
+                                                           T *__restrict__ y) {
+    using MF = Mfma<T>;
+    constexpr int NW = 4, NK = (K + 15) / 16;
+    constexpr int E = 16 / (int)sizeof(T);      // elements per 16-byte load
+    constexpr int LPQ = 16 / E;                 // lanes per quad: 16 rows of a column in one load
+    constexpr int QG = kWave / LPQ;             // quads per group
+    constexpr int VB = 4 * K / (E * LPQ);       // 16-byte loads of vector values per lane
+    constexpr int AQS = 64 + E, BQS = 4 * K + E;  // LDS quad strides (padded against bank conflicts)
+    static_assert(K >= 4 && (4 * K) % (E * LPQ) == 0 && K % E == 0, "K");
+    typedef T vec_t __attribute__((ext_vector_type(E), aligned(sizeof(T))));
+    typedef T lvec_t __attribute__((ext_vector_type(E)));
+    __shared__ __align__(16) T As[NW][QG * AQS];
+    __shared__ __align__(16) T Bs[NW][QG * BQS];
+    __shared__ T red[NW][16][K];
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int i16 = lane & 15, kq = lane >> 4;
+    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const int h = P.h, NT = (h + 15) >> 4;
+    const int wpt = NW / NT;
+    const int tt = wid / wpt, wsub = wid % wpt;
+    const int64_t ld = P.ld;
+    const int uq = lane / LPQ, part = lane % LPQ;  // this lane's quad of a group, and its share of it
+    if (tt < NT) {
+        typename MF::acc_t acc[NK];
+#pragma unroll
+        for (int n = 0; n < NK; n++) acc[n] = typename MF::acc_t{0, 0, 0, 0};
+        const int64_t qtot = P.nq4;
+        const QuadDesc *qd = quads + P.q0;
+        T *as = As[wid] + uq * AQS + part * E;
+        T *bs = Bs[wid] + uq * BQS + part * E;
+        for (int64_t g0 = (int64_t)wsub * QG; g0 < qtot; g0 += (int64_t)wpt * QG) {
+            const int64_t quad = g0 + uq;
+            const bool qok = quad < qtot;
+            const QuadDesc Q = qd[qok ? quad : 0];
+            const int kind = Q.info & 3, ncol = qok ? Q.info >> 8 : 0;
+            const T *Mc = (kind == 0 ? NP : UP) + Q.mat + P.r0 + tt * 16 + part * E;
+            const T *vc = (kind == 1 ? t : x) + (int64_t)Q.vec * K + part * E;
+            vec_t av[4], bv[VB];
+#pragma unroll
+            for (int m = 0; m < 4; m++) av[m] = m < ncol ? *reinterpret_cast<const vec_t *>(Mc + m * ld) : vec_t(T(0));
+#pragma unroll
+            for (int m = 0; m < VB; m++) {
+                const int e0 = (m * LPQ + part) * E;  // first of the quad's 4 x K values this load covers
+                bv[m] = e0 < ncol * K ? *reinterpret_cast<const vec_t *>(vc + m * LPQ * E) : vec_t(T(0));
+            }
+#pragma unroll
+            for (int m = 0; m < 4; m++) *reinterpret_cast<lvec_t *>(as + m * 16) = av[m];
+#pragma unroll
+            for (int m = 0; m < VB; m++) *reinterpret_cast<lvec_t *>(bs + m * LPQ * E) = bv[m];
+            // the wave's LDS operations run in order: a compiler fence is all
+            // the stage needs between its lanes' writes and reads
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < QG; u++) {
+                const T a = As[wid][u * AQS + kq * 16 + i16];
+#pragma unroll
+                for (int n = 0; n < NK; n++) {
+                    const int j = n * 16 + i16;
+                    const T b = j < K ? Bs[wid][u * BQS + kq * K + j] : T(0);
+                    acc[n] = MF::mma(a, b, acc[n]);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int n = 0; n < NK; n++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = n * 16 + i16;
+                if (j < K) red[wid][MF::row(lane, q)][j] = acc[n][q];
+            }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < h * K; e += blockDim.x) {
+        const int row = e / K, k = e % K;
+        const int w0 = (row >> 4) * wpt;
+        T sum = red[w0][row & 15][k];
+        for (int w = 1; w < wpt; w++) sum += red[w0 + w][row & 15][k];
+        y[(P.y0 + row) * K + k] = sum;
+    }
+}
+
 // V^T x on matrix cores.  A (tile, row chunk) item is split over 4
 // workgroups of 64 columns; wave w of quarter q owns the 16 columns
 // 64q + 16w .. + 15 over all rows of the chunk, so no cross-wave reduction
@@ -584,7 +687,7 @@ __global__ __launch_bounds__(256) void proj_mfma_kernel(const ProjItem *__restri
 template <typename T, int K>
 hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                            const PanelItem *pa, int64_t npa,
-                           const Segment *segs, const T *VP, const T *NP, const T *UP,
+                           const Segment *segs, const QuadDesc *quads, const T *VP, const T *NP, const T *UP,
                            const T *x, T *t, T *part, const int32_t *tidx, T *y,
                            int variant, hipEvent_t *ev, hipStream_t s) {
     constexpr bool kMfma = K >= 4;
@@ -595,9 +698,13 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
         if (npi) proj_mfma_kernel<T, K><<<dim3((unsigned)(4 * npi)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
         if (npr) proj_reduce_kernel<T, K><<<dim3((unsigned)npr), 256, 0, s>>>(pr, part, t, tidx);
         if (ev) (void)hipEventRecord(ev[1], s);
-        if (npa) panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+        if (npa) {
+            if (variant & kMatvecPanelWide)
+                panel_mfma_w_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, quads, NP, UP, x, t, y);
+            else
+                panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+        }
         if (ev) (void)hipEventRecord(ev[2], s);
-        (void)variant;
         return hipGetLastError();
     } else {
         if (npi) {
@@ -620,7 +727,10 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
             bool done = false;
             if constexpr (kMfma)
                 if (variant & kMatvecPanelMfma) {
-                    panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                    if (variant & kMatvecPanelWide)
+                        panel_mfma_w_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, quads, NP, UP, x, t, y);
+                    else
+                        panel_mfma_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
                     done = true;
                 }
             if constexpr (K <= 8)
@@ -643,12 +753,12 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                          const PanelItem *pa, int64_t npa,
-                         const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
-                         T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
+                         const Segment *segs, const QuadDesc *quads, const T *VP, const T *NP, const T *UP,
+                         const T *x, T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
                          hipStream_t s) {
 #define HODLR_MV(KK)                                                                                 \
     case KK:                                                                                         \
-        return launch_matvec_k<T, KK>(pi, npi, pr, npr, pa, npa, segs, VP, NP, UP, x, t,         \
+        return launch_matvec_k<T, KK>(pi, npi, pr, npr, pa, npa, segs, quads, VP, NP, UP, x, t,  \
                                       part, tidx, y, variant, ev, s);
     switch (nrhs) {
         HODLR_MV(1)
@@ -657,8 +767,8 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
         HODLR_MV(8)
         HODLR_MV(16)
     case 32:  // FP64: matrix-core forms only
-        return launch_matvec_k<T, 32>(pi, npi, pr, npr, pa, npa, segs, VP, NP, UP, x, t, part, tidx, y, variant,
-                                      ev, s);
+        return launch_matvec_k<T, 32>(pi, npi, pr, npr, pa, npa, segs, quads, VP, NP, UP, x, t, part, tidx, y,
+                                      variant, ev, s);
     default:
         return hipErrorInvalidValue;
     }
@@ -667,12 +777,14 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
 
 template hipError_t launch_matvec<double>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
                                           const PanelItem *, int64_t,
-                                          const Segment *, const double *, const double *, const double *,
+                                          const Segment *, const QuadDesc *, const double *, const double *,
+                                          const double *,
                                           const double *, double *, double *, const int32_t *, double *,
                                           int, hipEvent_t *, hipStream_t);
 template hipError_t launch_matvec<float>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
                                          const PanelItem *, int64_t,
-                                         const Segment *, const float *, const float *, const float *,
+                                         const Segment *, const QuadDesc *, const float *, const float *,
+                                         const float *,
                                          const float *, float *, float *, const int32_t *, float *,
                                          int, hipEvent_t *, hipStream_t);
 

@@ -663,6 +663,7 @@ void assemble(hodlr_op &op) {
     std::vector<NearLeaf> near;
     std::vector<int64_t> colsrc;
     std::vector<Segment> segs;
+    std::vector<QuadDesc> quads;
     std::vector<PanelItem> panel;
     op.n_np = 0;
     for (int p = 0; p < P; p++) {
@@ -734,6 +735,19 @@ void assemble(hodlr_op &op) {
                 if (cur.arena >= 0) segs.push_back(cur);
             }
             const int64_t nseg = (int64_t)segs.size() - seg0;
+            // the leaf's quads of columns (shared by its row tiles)
+            const int64_t q0 = (int64_t)quads.size();
+            for (int64_t q = seg0; q < seg0 + nseg; q++) {
+                const Segment &S = segs[q];
+                for (int32_t c = 0; c < S.len; c += 4) {
+                    QuadDesc qd;
+                    qd.mat = S.mat + (int64_t)c * nc;
+                    if (S.vec + c > INT32_MAX) throw ArgError{HODLR_E_ARG, "vector offset beyond 32 bits"};
+                    qd.vec = (int32_t)(S.vec + c);
+                    qd.info = S.arena | (std::min<int32_t>(4, S.len - c) << 8);
+                    quads.push_back(qd);
+                }
+            }
             for (int64_t r0 = 0; r0 < nc; r0 += 64) {
                 PanelItem pi;
                 pi.y0 = op.node_start[g] + r0;
@@ -751,6 +765,7 @@ void assemble(hodlr_op &op) {
                 if (n4 > INT32_MAX || ng > INT32_MAX) throw ArgError{HODLR_E_ARG, "leaf panel too wide"};
                 pi.nq4 = (int32_t)n4;
                 pi.nqg = (int32_t)ng;
+                pi.q0 = q0;
                 panel.push_back(pi);
             }
         }
@@ -971,7 +986,7 @@ void assemble(hodlr_op &op) {
     if (op.admit) {
         double bytes = 8.0 * (double)(op.n_np + op.n_up + op.n_vp);
         if (op.prec_flags & HODLR_PREC_F32) bytes *= 1.5;
-        bytes += 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk;
+        bytes += 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk + (double)sizeof(QuadDesc) * (double)quads.size();
         op.admit(bytes);
         mark("admission");
     }
@@ -1170,6 +1185,7 @@ void assemble(hodlr_op &op) {
     upload(op.red, red, s);
     upload(op.panel, panel, s);
     upload(op.segs, segs, s);
+    upload(op.quads, quads, s);
     upload(op.tidx, tidx, s);
     upload(op.skproj, skproj, s);
     upload(op.skred, skred, s);
@@ -1182,6 +1198,7 @@ void assemble(hodlr_op &op) {
     op.n_red = (int64_t)red.size();
     op.n_panel = (int64_t)panel.size();
     op.n_seg = (int64_t)segs.size();
+    op.n_quad = (int64_t)quads.size();
     {  // per-position diagonal (a batch may mix diagonals) for the FP32 path
         std::vector<double> dg(N > 0 ? N : 1, 0.0);
         for (int p = 0; p < P; p++)
@@ -1289,7 +1306,7 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
     if (prec == HODLR_PREC_F64) {
         HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
                                         panel.as<PanelItem>(), n_panel,
-                                        segs.as<Segment>(), VP64.as<double>(), NP64.as<double>(),
+                                        segs.as<Segment>(), quads.as<QuadDesc>(), VP64.as<double>(), NP64.as<double>(),
                                         UP64.as<double>(), static_cast<const double *>(x), t64.as<double>(),
                                         part64.as<double>(), tidx.as<int32_t>(), static_cast<double *>(y),
                                         variant_for(prec, nrhs), ev, s));
@@ -1301,7 +1318,8 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
         if (!(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
         HIP_CHECK(launch_matvec<float>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
                                        panel.as<PanelItem>(), n_panel,
-                                       segs.as<Segment>(), VP32.as<float>(), NP32.as<float>(), UP32.as<float>(),
+                                       segs.as<Segment>(), quads.as<QuadDesc>(), VP32.as<float>(), NP32.as<float>(),
+                                       UP32.as<float>(),
                                        static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
                                        tidx.as<int32_t>(), static_cast<float *>(y), variant_for(prec, nrhs), ev,
                                        s));
@@ -1366,6 +1384,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             if (k <= 2) cands.push_back(kMatvecProjPack | kMatvecPanelVec);
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
+            if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma | kMatvecPanelWide);
             for (int v : cands) {
                 mv_variant[pi][lg] = v;
                 hipEvent_t ev[4];
@@ -1383,7 +1402,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                 }
                 for (auto &e : ev) (void)hipEventDestroy(e);
                 const int bits[2] = {kMatvecProjMfma | kMatvecProjPack,
-                                     kMatvecPanelMfma | kMatvecPanelFlat | kMatvecPanelVec};
+                                     kMatvecPanelMfma | kMatvecPanelFlat | kMatvecPanelVec | kMatvecPanelWide};
                 for (int q = 0; q < 2; q++)
                     if (ph[q] < best[q]) {
                         best[q] = ph[q];
@@ -1397,7 +1416,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                              pi == 0 ? "f64" : "f32", k,
                              pick[0] & kMatvecProjMfma ? "mfma" : (pick[0] & kMatvecProjPack ? "pack" : "valu"),
                              best[0] / reps,
-                             pick[1] & kMatvecPanelMfma ? "mfma"
+                             pick[1] & kMatvecPanelMfma ? (pick[1] & kMatvecPanelWide ? "mfma-wide" : "mfma")
                                  : (pick[1] & kMatvecPanelVec ? "vec" : (pick[1] ? "flat" : "seg")),
                              best[1] / reps);
         }

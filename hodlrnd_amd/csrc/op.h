@@ -31,8 +31,8 @@ hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
                          const PanelItem *pa, int64_t npa,
-                         const Segment *segs, const T *VP, const T *NP, const T *UP, const T *x,
-                         T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
+                         const Segment *segs, const QuadDesc *quads, const T *VP, const T *NP, const T *UP,
+                         const T *x, T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
                          hipStream_t s);
 
 // skeleton blocks' on-the-fly product, added into y (t1, t2: FP64 slot
@@ -103,8 +103,8 @@ struct hodlr_op {
     hodlr::DevBuf NP64, UP64, VP64, NP32, UP32, VP32;
 
     // matvec work lists
-    int64_t n_proj = 0, n_red = 0, n_panel = 0, n_seg = 0;
-    hodlr::DevBuf proj, red, panel, segs, tidx;
+    int64_t n_proj = 0, n_red = 0, n_panel = 0, n_seg = 0, n_quad = 0;
+    hodlr::DevBuf proj, red, panel, segs, quads, tidx;
     double e_near = 0, e_far = 0, e_v = 0, e_u = 0, t_lowrank = 0;
 
     // per-precision work buffers (grown on demand)
@@ -169,8 +169,10 @@ struct hodlr_op {
     std::vector<std::unique_ptr<hodlr_tree>> owned_trees;  // trees of a loaded operator
 
     // kernel form per (precision, RHS count): kMatvec* bits, indexed
-    // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller
-    int mv_variant[2][6] = {{8, 8, 8, 2, 2, 3}, {8, 8, 8, 2, 2, 3}};
+    // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller.  The
+    // defaults are the forms fastest on the N = 1e6 grid operator (vector-ALU
+    // 16-byte panel up to 4 RHS, LDS-staged matrix-core panel at 8-16)
+    int mv_variant[2][6] = {{8, 8, 8, 34, 34, 3}, {8, 8, 8, 34, 34, 3}};
     int variant_for(int prec, int nrhs) const;
     // time every kernel form per phase on this operator and keep the fastest
     void tune_matvec(int prec_mask, int reps);

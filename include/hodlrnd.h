@@ -267,7 +267,9 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * segments), 8 = vector-ALU panel product reading 16 bytes per lane and
  * column (two FP64 / four FP32 rows; up to 8 RHS; MFMA takes precedence
  * when also set, and 8 over 4), 16 = vector-ALU projection packing several
- * rows of a narrow V-panel tile into one wave load (1-2 RHS).  tune times the forms of each pass on this operator for
+ * rows of a narrow V-panel tile into one wave load (1-2 RHS), 32 = with 2,
+ * the matrix-core panel product staging each group of columns through LDS
+ * with 16-byte loads (4-16 RHS).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
  * results agree up to rounding order.  get/set read or force the choice for
  * one (prec, nrhs). */
@@ -276,6 +278,7 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
 #define HODLR_MV_PANEL_FLAT 4
 #define HODLR_MV_PANEL_VEC 8
 #define HODLR_MV_PROJ_PACK 16
+#define HODLR_MV_PANEL_WIDE 32
 int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps);
 int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant);
 int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);

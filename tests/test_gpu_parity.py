@@ -237,20 +237,24 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
     for prec, ks, tol in ((H.PREC_F64, (4, 8, 16), 1e-14), (H.PREC_F32, (4, 8, 16, 32), 1e-6)):
         for k in ks:
             out = {}
-            for v in (0, H.MV_PROJ_MFMA, H.MV_PANEL_MFMA, H.MV_PROJ_MFMA | H.MV_PANEL_MFMA, H.MV_PANEL_FLAT):
+            forms = [0, H.MV_PROJ_MFMA, H.MV_PANEL_MFMA, H.MV_PROJ_MFMA | H.MV_PANEL_MFMA, H.MV_PANEL_FLAT]
+            forms.append(H.MV_PANEL_MFMA | H.MV_PANEL_WIDE)  # LDS-staged matrix-core panel
+            for v in forms:
                 op.set_variant(prec, k, v)
                 out[v] = op.matvec(X[:, :k], prec=prec)
-            for v in (1, 2, 3, 4):
+            for v in forms[1:]:
                 e = relerr(out[v], out[0])
                 assert e < tol, (prec, k, v, e)
             e1 = relerr(out[3][:, :4], y1)
             assert e1 < (1e-14 if prec == H.PREC_F64 else 1e-5), (prec, k, e1)
             say(" prec", prec, "K", k, "mfma vs valu", relerr(out[3], out[0]))
     # FP64 with 32 RHS (matrix-core forms only) against the 4-RHS columns
-    y64 = op.matvec(X, prec=H.PREC_F64)
-    e32 = relerr(y64[:, :4], y1)
-    say(" FP64 K 32 vs 1-RHS columns", e32)
-    assert e32 < 1e-14, e32
+    for v in (H.MV_PROJ_MFMA | H.MV_PANEL_MFMA, H.MV_PROJ_MFMA | H.MV_PANEL_MFMA | H.MV_PANEL_WIDE):
+        op.set_variant(H.PREC_F64, 32, v)
+        y64 = op.matvec(X, prec=H.PREC_F64)
+        e32 = relerr(y64[:, :4], y1)
+        say(" FP64 K 32 form", v, "vs 1-RHS columns", e32)
+        assert e32 < 1e-14, e32
     for prec in (H.PREC_F64, H.PREC_F32):  # flattened walk at 1 and 2 RHS
         for k in (1, 2):
             op.set_variant(prec, k, 0)
@@ -261,7 +265,7 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
     op.tune(reps=1)
     for prec, ks in ((H.PREC_F64, (1, 2, 4, 8, 16)), (H.PREC_F32, (1, 2, 4, 8, 16, 32))):
         for k in ks:
-            assert op.get_variant(prec, k) in range(32)
+            assert op.get_variant(prec, k) in range(64)
 
 
 @pytest.mark.parametrize("nd,n,nmax,eps,sys_size,cap", [
