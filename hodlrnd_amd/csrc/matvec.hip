@@ -528,106 +528,156 @@ __global__ __launch_bounds__(256) void panel_mfma_kernel(const PanelItem *__rest
 }
 
 
-// Matrix-core panel product with 16-byte operand loads.  In
-// panel_mfma_kernel every A and B fragment is its own 4-byte-per-lane load:
-// 32 memory instructions per 16 quads, and the kernel streams at ~3.4 TB/s
-// on any operator (cutting the vector-ALU form's instruction count the
-// same way made it 1.4x faster).  Here a wave stages a group of QG quads
-// at a time into its own LDS area and reads the MFMA fragments from there.
-// The LPQ = 64 / QG lanes of a quad load its descriptor, then, with one
-// 16-byte load per lane and instruction, column m's 16 tile rows (64 or
-// 128 contiguous bytes) for m = 0..3, and its 4 x K vector values (also
-// contiguous): 1 + 4 + K/4 loads per lane and group instead of 32.  Rows
-// past the tile are read (the next column or the arena's tail padding)
-// and land only in discarded accumulator rows.  (Issuing the next group's
-// loads before the current group's products measured slower.)
-template <typename T, int K>
-__global__ __launch_bounds__(256) void panel_mfma_w_kernel(const PanelItem *__restrict__ items,
-                                                           const QuadDesc *__restrict__ quads,
-                                                           const T *__restrict__ NP, const T *__restrict__ UP,
-                                                           const T *__restrict__ x, const T *__restrict__ t,
-                                                           T *__restrict__ y) {
+// Geometry of the LDS-staged panel product for waves of RT row tiles of 16.
+template <typename T, int K, int RT>
+struct WideGeo {
+    static constexpr int E = 16 / (int)sizeof(T);     // elements per 16-byte load
+    static constexpr int ROWS = 16 * RT;              // rows per wave
+    static constexpr int LPQ = ROWS / E;              // lanes per quad: a column's rows in one load
+    static constexpr int QG = kWave / LPQ;            // quads per group
+    static constexpr int PB = 4 * K / E;              // 16-byte pieces of a quad's vector values
+    static constexpr int VB = (PB + LPQ - 1) / LPQ;   // their loads per lane
+    static constexpr int AQS = 4 * ROWS + E, BQS = 4 * K + E;  // LDS quad strides (padded against bank conflicts)
+    static constexpr int WS = QG * (AQS + BQS);       // a wave's stage
+};
+
+// One leaf row tile of the LDS-staged panel product with RT row tiles of 16
+// per wave.  Wave w owns WSTR elements of LDS at stage + w WSTR: its stage
+// while it walks the quads, then its ROWS x K partial sums (no other wave
+// touches them before the workgroup barrier).
+template <typename T, int K, int RT, int WSTR>
+__device__ __forceinline__ void panel_w_tile(const PanelItem &P, const QuadDesc *__restrict__ quads,
+                                             const T *__restrict__ NP, const T *__restrict__ UP,
+                                             const T *__restrict__ x, const T *__restrict__ t, T *__restrict__ y,
+                                             T *stage, int wid, int lane) {
     using MF = Mfma<T>;
-    constexpr int NW = 4, NK = (K + 15) / 16;
-    constexpr int E = 16 / (int)sizeof(T);      // elements per 16-byte load
-    constexpr int LPQ = 16 / E;                 // lanes per quad: 16 rows of a column in one load
-    constexpr int QG = kWave / LPQ;             // quads per group
-    constexpr int VB = 4 * K / (E * LPQ);       // 16-byte loads of vector values per lane
-    constexpr int AQS = 64 + E, BQS = 4 * K + E;  // LDS quad strides (padded against bank conflicts)
-    static_assert(K >= 4 && (4 * K) % (E * LPQ) == 0 && K % E == 0, "K");
+    using G = WideGeo<T, K, RT>;
+    constexpr int NW = 4, NK = (K + 15) / 16, E = G::E, ROWS = G::ROWS, LPQ = G::LPQ, QG = G::QG;
+    constexpr int PB = G::PB, VB = G::VB, AQS = G::AQS, BQS = G::BQS;
+    static_assert(K >= 4 && K % E == 0 && G::WS <= WSTR && ROWS * K <= WSTR, "K");
     typedef T vec_t __attribute__((ext_vector_type(E), aligned(sizeof(T))));
     typedef T lvec_t __attribute__((ext_vector_type(E)));
-    __shared__ __align__(16) T As[NW][QG * AQS];
-    __shared__ __align__(16) T Bs[NW][QG * BQS];
-    __shared__ T red[NW][16][K];
-    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int i16 = lane & 15, kq = lane >> 4;
-    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
-    const int h = P.h, NT = (h + 15) >> 4;
+    const int h = P.h, NT = (h + ROWS - 1) / ROWS;
     const int wpt = NW / NT;
     const int tt = wid / wpt, wsub = wid % wpt;
     const int64_t ld = P.ld;
     const int uq = lane / LPQ, part = lane % LPQ;  // this lane's quad of a group, and its share of it
     if (tt < NT) {
-        typename MF::acc_t acc[NK];
+        typename MF::acc_t acc[RT][NK];
 #pragma unroll
-        for (int n = 0; n < NK; n++) acc[n] = typename MF::acc_t{0, 0, 0, 0};
+        for (int r = 0; r < RT; r++)
+#pragma unroll
+            for (int n = 0; n < NK; n++) acc[r][n] = typename MF::acc_t{0, 0, 0, 0};
         const int64_t qtot = P.nq4;
         const QuadDesc *qd = quads + P.q0;
-        T *as = As[wid] + uq * AQS + part * E;
-        T *bs = Bs[wid] + uq * BQS + part * E;
+        T *const As = stage + wid * WSTR;
+        T *const Bs = As + QG * AQS;
+        T *as = As + uq * AQS + part * E;
+        T *bs = Bs + uq * BQS;
         for (int64_t g0 = (int64_t)wsub * QG; g0 < qtot; g0 += (int64_t)wpt * QG) {
             const int64_t quad = g0 + uq;
             const bool qok = quad < qtot;
             const QuadDesc Q = qd[qok ? quad : 0];
             const int kind = Q.info & 3, ncol = qok ? Q.info >> 8 : 0;
-            const T *Mc = (kind == 0 ? NP : UP) + Q.mat + P.r0 + tt * 16 + part * E;
-            const T *vc = (kind == 1 ? t : x) + (int64_t)Q.vec * K + part * E;
+            const T *Mc = (kind == 0 ? NP : UP) + Q.mat + P.r0 + tt * ROWS + part * E;
+            const T *vc = (kind == 1 ? t : x) + (int64_t)Q.vec * K;
             vec_t av[4], bv[VB];
 #pragma unroll
             for (int m = 0; m < 4; m++) av[m] = m < ncol ? *reinterpret_cast<const vec_t *>(Mc + m * ld) : vec_t(T(0));
 #pragma unroll
             for (int m = 0; m < VB; m++) {
-                const int e0 = (m * LPQ + part) * E;  // first of the quad's 4 x K values this load covers
-                bv[m] = e0 < ncol * K ? *reinterpret_cast<const vec_t *>(vc + m * LPQ * E) : vec_t(T(0));
+                const int pc = m * LPQ + part;  // piece of the quad's 4 x K values
+                bv[m] = pc * E < ncol * K ? *reinterpret_cast<const vec_t *>(vc + pc * E) : vec_t(T(0));
             }
 #pragma unroll
-            for (int m = 0; m < 4; m++) *reinterpret_cast<lvec_t *>(as + m * 16) = av[m];
+            for (int m = 0; m < 4; m++) *reinterpret_cast<lvec_t *>(as + m * ROWS) = av[m];
 #pragma unroll
-            for (int m = 0; m < VB; m++) *reinterpret_cast<lvec_t *>(bs + m * LPQ * E) = bv[m];
+            for (int m = 0; m < VB; m++) {
+                const int pc = m * LPQ + part;
+                if (PB % LPQ == 0 || pc < PB) *reinterpret_cast<lvec_t *>(bs + pc * E) = bv[m];
+            }
             // the wave's LDS operations run in order: a compiler fence is all
             // the stage needs between its lanes' writes and reads
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int u = 0; u < QG; u++) {
-                const T a = As[wid][u * AQS + kq * 16 + i16];
+                T b[NK];
 #pragma unroll
                 for (int n = 0; n < NK; n++) {
                     const int j = n * 16 + i16;
-                    const T b = j < K ? Bs[wid][u * BQS + kq * K + j] : T(0);
-                    acc[n] = MF::mma(a, b, acc[n]);
+                    b[n] = j < K ? Bs[u * BQS + kq * K + j] : T(0);
+                }
+#pragma unroll
+                for (int r = 0; r < RT; r++) {
+                    const T a = As[u * AQS + kq * ROWS + r * 16 + i16];
+#pragma unroll
+                    for (int n = 0; n < NK; n++) acc[r][n] = MF::mma(a, b[n], acc[r][n]);
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
         }
 #pragma unroll
-        for (int n = 0; n < NK; n++)
+        for (int r = 0; r < RT; r++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int j = n * 16 + i16;
-                if (j < K) red[wid][MF::row(lane, q)][j] = acc[n][q];
-            }
+            for (int n = 0; n < NK; n++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int j = n * 16 + i16;
+                    if (j < K) As[(r * 16 + MF::row(lane, q)) * K + j] = acc[r][n][q];
+                }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < h * K; e += blockDim.x) {
         const int row = e / K, k = e % K;
-        const int w0 = (row >> 4) * wpt;
-        T sum = red[w0][row & 15][k];
-        for (int w = 1; w < wpt; w++) sum += red[w0 + w][row & 15][k];
+        const int w0 = (row / ROWS) * wpt, rk = (row % ROWS) * K + k;
+        T sum = stage[w0 * WSTR + rk];
+        for (int w = 1; w < wpt; w++) sum += stage[(w0 + w) * WSTR + rk];
         y[(P.y0 + row) * K + k] = sum;
     }
+}
+
+// Matrix-core panel product with 16-byte operand loads.  In
+// panel_mfma_kernel every A and B fragment is its own 4-byte-per-lane load:
+// 32 memory instructions per 16 quads, and the kernel streams at ~3.4 TB/s
+// on any operator (cutting the vector-ALU form's instruction count the
+// same way made it 1.4x faster).  Here a wave stages a group of quads at a
+// time into its own LDS area and reads the MFMA fragments from there.  The
+// LPQ lanes of a quad load its descriptor, then, with one 16-byte load per
+// lane and instruction, column m's rows of the wave (one contiguous run)
+// for m = 0..3, and its 4 x K vector values (also contiguous): 1 + 4 +
+// ceil(4K / (16 LPQ)) loads per lane and group instead of 32.  An FP32 wave
+// takes 32 rows (two MFMA row tiles: whole 128-byte column runs, and each
+// quad's vector values loaded once for both) when the rows it reads past
+// the leaf tile stay within the arena's 16-element tail padding, else 16;
+// an FP64 wave 16 (128 bytes).  Rows past the tile are read (the next
+// column or the padding) and land only in discarded accumulator rows.
+// A wave's partial sums replace its stage after its walk (the smaller LDS
+// footprint measured faster than a separate buffer).  (Issuing the next
+// group's loads before the current group's products measured slower.)
+template <typename T, int K>
+__global__ __launch_bounds__(256) void panel_mfma_w_kernel(const PanelItem *__restrict__ items,
+                                                           const QuadDesc *__restrict__ quads,
+                                                           const T *__restrict__ NP, const T *__restrict__ UP,
+                                                           const T *__restrict__ x, const T *__restrict__ t,
+                                                           T *__restrict__ y) {
+    constexpr bool kPair = sizeof(T) == 4;
+    constexpr int WS1 = WideGeo<T, K, 1>::WS, WS2 = WideGeo<T, K, 2>::WS;
+    constexpr int WS = kPair && WS2 > WS1 ? WS2 : WS1;  // per wave
+    static_assert(kArenaPad >= 15, "row overread");
+    __shared__ __align__(16) T stage[4 * WS];
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const int h = P.h;
+    if constexpr (kPair) {
+        if (h > 16 && (h + 31) / 32 * 32 - h <= 15) {
+            panel_w_tile<T, K, 2, WS>(P, quads, NP, UP, x, t, y, stage, wid, lane);
+            return;
+        }
+    }
+    panel_w_tile<T, K, 1, WS>(P, quads, NP, UP, x, t, y, stage, wid, lane);
 }
 
 // V^T x on matrix cores.  A (tile, row chunk) item is split over 4

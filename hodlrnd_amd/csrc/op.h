@@ -170,9 +170,9 @@ struct hodlr_op {
 
     // kernel form per (precision, RHS count): kMatvec* bits, indexed
     // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller.  The
-    // defaults are the forms fastest on the N = 1e6 grid operator (vector-ALU
-    // 16-byte panel up to 4 RHS, LDS-staged matrix-core panel at 8-16)
-    int mv_variant[2][6] = {{8, 8, 8, 34, 34, 3}, {8, 8, 8, 34, 34, 3}};
+    // defaults: the vector-ALU 16-byte panel up to 4 RHS, the LDS-staged
+    // matrix-core panel from 8 on (matrix-core projection too at 32)
+    int mv_variant[2][6] = {{8, 8, 8, 34, 34, 35}, {8, 8, 8, 34, 34, 35}};
     int variant_for(int prec, int nrhs) const;
     // time every kernel form per phase on this operator and keep the fastest
     void tune_matvec(int prec_mask, int reps);

@@ -269,7 +269,7 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * when also set, and 8 over 4), 16 = vector-ALU projection packing several
  * rows of a narrow V-panel tile into one wave load (1-2 RHS), 32 = with 2,
  * the matrix-core panel product staging each group of columns through LDS
- * with 16-byte loads (4-16 RHS).  tune times the forms of each pass on this operator for
+ * with 16-byte loads (4-32 RHS).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
  * results agree up to rounding order.  get/set read or force the choice for
  * one (prec, nrhs). */
