@@ -33,7 +33,8 @@ class HodlrError(RuntimeError):
         self.code = code
 
 
-MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT, MV_PANEL_VEC, MV_PROJ_PACK, MV_PANEL_WIDE = 1, 2, 4, 8, 16, 32
+MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT, MV_PANEL_VEC, MV_PROJ_PACK, MV_PANEL_WIDE, MV_PROJ_WAVE = \
+    1, 2, 4, 8, 16, 32, 64
 
 
 class KernelDesc(C.Structure):

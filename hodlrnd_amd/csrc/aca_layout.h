@@ -25,6 +25,10 @@ constexpr int kMatvecProjPack = 16;
 // matrix-core panel product loading its operands with 16-byte loads through
 // a wave-private LDS stage (with kMatvecPanelMfma)
 constexpr int kMatvecPanelWide = 32;
+// projection of short items (at most kProjWaveRows rows) one wave per item,
+// four per workgroup, with no LDS reduction (else every item is a workgroup)
+constexpr int kMatvecProjWave = 64;
+constexpr int kProjWaveRows = 64;
 constexpr int64_t kArenaPad = 16;
 
 // one admissible block for the ACA kernel

@@ -618,7 +618,7 @@ int hodlr_op_set_solver_graphs(hodlr_op *op, int enable) {
 }
 
 int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant) {
-    if (!op || variant < 0 || variant > 63) return HODLR_E_ARG;
+    if (!op || variant < 0 || variant > 127) return HODLR_E_ARG;
     const int lg = variant_slot(prec, nrhs);
     if (lg < 0) return HODLR_E_ARG;
     op->mv_variant[prec == HODLR_PREC_F64 ? 0 : 1][lg] = variant;

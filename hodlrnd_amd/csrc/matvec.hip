@@ -119,6 +119,65 @@ __global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ 
     }
 }
 
+// Projection of short items (at most kProjWaveRows rows): one wave per item,
+// four items per workgroup.  The wave walks all of its item's rows (lane l:
+// columns 4l..4l+3, U rows in flight) and writes its sums from registers --
+// no LDS and no workgroup barrier, which in proj_kernel (four waves over a
+// few rows, then four reduction rounds) cost more than the item's loads.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void proj_wave_kernel(const ProjItem *__restrict__ items, int64_t nitems,
+                                                        const T *__restrict__ VP, const T *__restrict__ x,
+                                                        T *__restrict__ t, T *__restrict__ part,
+                                                        const int32_t *__restrict__ tidx) {
+    constexpr int U = 4;
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int64_t it = (int64_t)blockIdx.x * 4 + wid;
+    if (it >= nitems) return;
+    const ProjItem P = items[it];
+    const int c = lane * 4;
+    if (c >= P.w) return;
+    T acc[4][K];
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[e][k] = T(0);
+    const T *base = VP + P.vp + c;
+    const T *xv = x + P.x0 * K;
+    const int64_t ld = P.ld;
+    int j = 0;
+    for (; j + U <= P.rows; j += U) {
+        Vec4<T> a[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) a[u] = load4(base + (int64_t)(j + u) * ld);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const T xj = xv[(int64_t)(j + u) * K + k];
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[e][k] = fma(a[u].v[e], xj, acc[e][k]);
+            }
+    }
+    for (; j < P.rows; j++) {
+        const Vec4<T> a = load4(base + (int64_t)j * ld);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const T xj = xv[(int64_t)j * K + k];
+#pragma unroll
+            for (int e = 0; e < 4; e++) acc[e][k] = fma(a.v[e], xj, acc[e][k]);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int col = c + e;
+        if (col < P.w) {
+            T *dst = P.out >= 0 ? part + (P.out * kProjTile + col) * K : t + (int64_t)tidx[P.tcol + col] * K;
+#pragma unroll
+            for (int k = 0; k < K; k++) dst[k] = acc[e][k];
+        }
+    }
+}
+
 // Row-packed projection for narrow tiles (1-2 RHS).  A tile of width tw has
 // Q = tw/4 column quads per row; a wave reads RW = 64/Q consecutive rows per
 // load (lane l: quad l % Q, row offset l / Q), so a 100-column tile keeps 50
@@ -735,7 +794,8 @@ __global__ __launch_bounds__(256) void proj_mfma_kernel(const ProjItem *__restri
 }
 
 template <typename T, int K>
-hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
+hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjItem *pw, int64_t npw_big,
+                           const ProjReduce *pr, int64_t npr,
                            const PanelItem *pa, int64_t npa,
                            const Segment *segs, const QuadDesc *quads, const T *VP, const T *NP, const T *UP,
                            const T *x, T *t, T *part, const int32_t *tidx, T *y,
@@ -764,6 +824,16 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
                     proj_mfma_kernel<T, K><<<dim3((unsigned)(4 * npi)), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
                     done = true;
                 }
+            if (!done && (variant & kMatvecProjWave)) {
+                // the same items, the long ones first: one workgroup each,
+                // then the short ones four per workgroup
+                if (npw_big)
+                    proj_kernel<T, K><<<dim3((unsigned)npw_big), 256, 0, s>>>(pw, npw_big, VP, x, t, part, tidx);
+                if (npi > npw_big)
+                    proj_wave_kernel<T, K><<<dim3((unsigned)((npi - npw_big + 3) / 4)), 256, 0, s>>>(
+                        pw + npw_big, npi - npw_big, VP, x, t, part, tidx);
+                done = true;
+            }
             if constexpr (K <= 2)
                 if (!done && (variant & kMatvecProjPack)) {
                     proj_pack_kernel<T, K><<<dim3((unsigned)npi), 256, 0, s>>>(pi, npi, VP, x, t, part, tidx);
@@ -801,14 +871,15 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjReduce *pr
 }
 
 template <typename T>
-hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
+hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjItem *pw, int64_t npw_big,
+                         const ProjReduce *pr, int64_t npr,
                          const PanelItem *pa, int64_t npa,
                          const Segment *segs, const QuadDesc *quads, const T *VP, const T *NP, const T *UP,
                          const T *x, T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
                          hipStream_t s) {
 #define HODLR_MV(KK)                                                                                 \
     case KK:                                                                                         \
-        return launch_matvec_k<T, KK>(pi, npi, pr, npr, pa, npa, segs, quads, VP, NP, UP, x, t,  \
+        return launch_matvec_k<T, KK>(pi, npi, pw, npw_big, pr, npr, pa, npa, segs, quads, VP, NP, UP, x, t,  \
                                       part, tidx, y, variant, ev, s);
     switch (nrhs) {
         HODLR_MV(1)
@@ -817,7 +888,8 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
         HODLR_MV(8)
         HODLR_MV(16)
     case 32:  // FP64: matrix-core forms only
-        return launch_matvec_k<T, 32>(pi, npi, pr, npr, pa, npa, segs, quads, VP, NP, UP, x, t, part, tidx, y,
+        return launch_matvec_k<T, 32>(pi, npi, pw, npw_big, pr, npr, pa, npa, segs, quads, VP, NP, UP, x, t, part,
+                                      tidx, y,
                                       variant, ev, s);
     default:
         return hipErrorInvalidValue;
@@ -825,13 +897,15 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjRe
 #undef HODLR_MV
 }
 
-template hipError_t launch_matvec<double>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
+template hipError_t launch_matvec<double>(int, const ProjItem *, int64_t, const ProjItem *, int64_t,
+                                          const ProjReduce *, int64_t,
                                           const PanelItem *, int64_t,
                                           const Segment *, const QuadDesc *, const double *, const double *,
                                           const double *,
                                           const double *, double *, double *, const int32_t *, double *,
                                           int, hipEvent_t *, hipStream_t);
-template hipError_t launch_matvec<float>(int, const ProjItem *, int64_t, const ProjReduce *, int64_t,
+template hipError_t launch_matvec<float>(int, const ProjItem *, int64_t, const ProjItem *, int64_t,
+                                         const ProjReduce *, int64_t,
                                          const PanelItem *, int64_t,
                                          const Segment *, const QuadDesc *, const float *, const float *,
                                          const float *,

@@ -847,8 +847,11 @@ void assemble(hodlr_op &op) {
     // largest items first, by power-of-two size class: the hardware's
     // round-robin over XCDs then balances bytes per XCD, and inside a class
     // the items keep arena order, so concurrently running workgroups read
-    // nearby memory
-    {
+    // nearby memory.  per_wg items share a workgroup (item p runs in
+    // workgroup p / per_wg).
+    std::vector<ProjItem> projw;
+    int64_t projw_big = 0;
+    auto order_items = [](std::vector<ProjItem> &proj, int per_wg) {
         auto size_class = [](const ProjItem &a) {
             int64_t v = (int64_t)a.rows * a.ld;
             int l = 0;
@@ -873,11 +876,21 @@ void assemble(hodlr_op &op) {
             size_t next = c0;  // next item of the class in arena order
             for (int x = 0; x < kXcd; x++)
                 for (size_t p = c0; p < c1; p++)
-                    if ((int)(p % kXcd) == x) dealt[p] = proj[next++];
+                    if ((int)((p / per_wg) % kXcd) == x) dealt[p] = proj[next++];
             c0 = c1;
         }
         proj.swap(dealt);
+    };
+    {
+        std::vector<ProjItem> big, small;
+        for (const ProjItem &it : proj) (it.rows > kProjWaveRows ? big : small).push_back(it);
+        order_items(big, 1);
+        order_items(small, 4);
+        projw_big = (int64_t)big.size();
+        projw = std::move(big);
+        projw.insert(projw.end(), small.begin(), small.end());
     }
+    order_items(proj, 1);
     if (verbose()) {  // projection work by tile height and width
         const int rcut[4] = {64, 256, 2048, 1 << 30};
         for (int full = 1; full >= 0; full--)
@@ -986,7 +999,8 @@ void assemble(hodlr_op &op) {
     if (op.admit) {
         double bytes = 8.0 * (double)(op.n_np + op.n_up + op.n_vp);
         if (op.prec_flags & HODLR_PREC_F32) bytes *= 1.5;
-        bytes += 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk + (double)sizeof(QuadDesc) * (double)quads.size();
+        bytes += 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk + (double)sizeof(QuadDesc) * (double)quads.size() +
+                 (double)sizeof(ProjItem) * (double)projw.size();
         op.admit(bytes);
         mark("admission");
     }
@@ -1182,6 +1196,8 @@ void assemble(hodlr_op &op) {
         HIP_CHECK(hipStreamSynchronize(s));
     }
     upload(op.proj, proj, s);
+    upload(op.projw, projw, s);
+    op.n_projw_big = projw_big;
     upload(op.red, red, s);
     upload(op.panel, panel, s);
     upload(op.segs, segs, s);
@@ -1304,7 +1320,8 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
         ev = &ev_pool[first];
     }
     if (prec == HODLR_PREC_F64) {
-        HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
+        HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, projw.as<ProjItem>(), n_projw_big,
+                                        red.as<ProjReduce>(), n_red,
                                         panel.as<PanelItem>(), n_panel,
                                         segs.as<Segment>(), quads.as<QuadDesc>(), VP64.as<double>(), NP64.as<double>(),
                                         UP64.as<double>(), static_cast<const double *>(x), t64.as<double>(),
@@ -1316,7 +1333,8 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
         if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     } else {
         if (!(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
-        HIP_CHECK(launch_matvec<float>(nrhs, proj.as<ProjItem>(), n_proj, red.as<ProjReduce>(), n_red,
+        HIP_CHECK(launch_matvec<float>(nrhs, proj.as<ProjItem>(), n_proj, projw.as<ProjItem>(), n_projw_big,
+                                       red.as<ProjReduce>(), n_red,
                                        panel.as<PanelItem>(), n_panel,
                                        segs.as<Segment>(), quads.as<QuadDesc>(), VP32.as<float>(), NP32.as<float>(),
                                        UP32.as<float>(),
@@ -1384,6 +1402,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             if (k <= 2) cands.push_back(kMatvecProjPack | kMatvecPanelVec);
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
+            if (k <= 16) cands.push_back(kMatvecProjWave | (k >= 4 ? kMatvecPanelMfma | kMatvecPanelWide : kMatvecPanelVec));
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma | kMatvecPanelWide);
             for (int v : cands) {
                 mv_variant[pi][lg] = v;
@@ -1401,7 +1420,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                     ph[1] += b;
                 }
                 for (auto &e : ev) (void)hipEventDestroy(e);
-                const int bits[2] = {kMatvecProjMfma | kMatvecProjPack,
+                const int bits[2] = {kMatvecProjMfma | kMatvecProjPack | kMatvecProjWave,
                                      kMatvecPanelMfma | kMatvecPanelFlat | kMatvecPanelVec | kMatvecPanelWide};
                 for (int q = 0; q < 2; q++)
                     if (ph[q] < best[q]) {
@@ -1414,7 +1433,8 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             if (verbose())
                 std::fprintf(stderr, "[hodlr] tune %s K=%d: proj %s %.3f ms, panel %s %.3f ms\n",
                              pi == 0 ? "f64" : "f32", k,
-                             pick[0] & kMatvecProjMfma ? "mfma" : (pick[0] & kMatvecProjPack ? "pack" : "valu"),
+                             pick[0] & kMatvecProjMfma ? "mfma"
+                                 : (pick[0] & kMatvecProjWave ? "wave" : (pick[0] & kMatvecProjPack ? "pack" : "valu")),
                              best[0] / reps,
                              pick[1] & kMatvecPanelMfma ? (pick[1] & kMatvecPanelWide ? "mfma-wide" : "mfma")
                                  : (pick[1] & kMatvecPanelVec ? "vec" : (pick[1] ? "flat" : "seg")),

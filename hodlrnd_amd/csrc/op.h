@@ -29,7 +29,8 @@ hipError_t launch_f32_result(const float *y, const double *x, const double *diag
 hipError_t launch_add_diag_f32(float *y, const float *x, const double *diag, int64_t n, int k, hipStream_t s);
 hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 template <typename T>
-hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjReduce *pr, int64_t npr,
+hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjItem *pw, int64_t npw_big,
+                         const ProjReduce *pr, int64_t npr,
                          const PanelItem *pa, int64_t npa,
                          const Segment *segs, const QuadDesc *quads, const T *VP, const T *NP, const T *UP,
                          const T *x, T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
@@ -105,6 +106,11 @@ struct hodlr_op {
     // matvec work lists
     int64_t n_proj = 0, n_red = 0, n_panel = 0, n_seg = 0, n_quad = 0;
     hodlr::DevBuf proj, red, panel, segs, quads, tidx;
+    // the projection items again for the one-wave-per-item form: the first
+    // n_projw_big have more than kProjWaveRows rows (one workgroup each),
+    // the rest are taken four per workgroup
+    int64_t n_projw_big = 0;
+    hodlr::DevBuf projw;
     double e_near = 0, e_far = 0, e_v = 0, e_u = 0, t_lowrank = 0;
 
     // per-precision work buffers (grown on demand)
@@ -171,8 +177,9 @@ struct hodlr_op {
     // kernel form per (precision, RHS count): kMatvec* bits, indexed
     // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller.  The
     // defaults: the vector-ALU 16-byte panel up to 4 RHS, the LDS-staged
-    // matrix-core panel from 8 on (matrix-core projection too at 32)
-    int mv_variant[2][6] = {{8, 8, 8, 34, 34, 35}, {8, 8, 8, 34, 34, 35}};
+    // matrix-core panel from 8 on; short projection items one wave each
+    // for FP64 at 8-16 RHS and FP32 at 16; matrix-core projection at 32
+    int mv_variant[2][6] = {{8, 8, 8, 98, 98, 35}, {8, 8, 8, 34, 98, 35}};
     int variant_for(int prec, int nrhs) const;
     // time every kernel form per phase on this operator and keep the fastest
     void tune_matvec(int prec_mask, int reps);

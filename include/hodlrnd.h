@@ -269,7 +269,9 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * when also set, and 8 over 4), 16 = vector-ALU projection packing several
  * rows of a narrow V-panel tile into one wave load (1-2 RHS), 32 = with 2,
  * the matrix-core panel product staging each group of columns through LDS
- * with 16-byte loads (4-32 RHS).  tune times the forms of each pass on this operator for
+ * with 16-byte loads (4-32 RHS), 64 = vector-ALU projection taking items of
+ * at most 64 rows one wave each, four per workgroup (1-16 RHS; MFMA takes
+ * precedence).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
  * results agree up to rounding order.  get/set read or force the choice for
  * one (prec, nrhs). */
@@ -279,6 +281,7 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
 #define HODLR_MV_PANEL_VEC 8
 #define HODLR_MV_PROJ_PACK 16
 #define HODLR_MV_PANEL_WIDE 32
+#define HODLR_MV_PROJ_WAVE 64
 int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps);
 int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant);
 int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);

@@ -234,10 +234,15 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
                 c = op.matvec(Xk, prec=prec)
                 op.set_variant(prec, k, 0)
                 assert relerr(c, a) < tol, (prec, k, relerr(c, a))
+            op.set_variant(prec, k, H.MV_PROJ_WAVE)  # short items one wave each
+            c = op.matvec(Xk, prec=prec)
+            op.set_variant(prec, k, 0)
+            assert relerr(c, a) < tol, (prec, k, relerr(c, a))
     for prec, ks, tol in ((H.PREC_F64, (4, 8, 16), 1e-14), (H.PREC_F32, (4, 8, 16, 32), 1e-6)):
         for k in ks:
             out = {}
-            forms = [0, H.MV_PROJ_MFMA, H.MV_PANEL_MFMA, H.MV_PROJ_MFMA | H.MV_PANEL_MFMA, H.MV_PANEL_FLAT]
+            forms = [0, H.MV_PROJ_MFMA, H.MV_PANEL_MFMA, H.MV_PROJ_MFMA | H.MV_PANEL_MFMA, H.MV_PANEL_FLAT,
+                     H.MV_PROJ_WAVE]
             forms.append(H.MV_PANEL_MFMA | H.MV_PANEL_WIDE)  # LDS-staged matrix-core panel
             for v in forms:
                 op.set_variant(prec, k, v)
@@ -265,7 +270,7 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
     op.tune(reps=1)
     for prec, ks in ((H.PREC_F64, (1, 2, 4, 8, 16)), (H.PREC_F32, (1, 2, 4, 8, 16, 32))):
         for k in ks:
-            assert op.get_variant(prec, k) in range(64)
+            assert op.get_variant(prec, k) in range(128)
 
 
 @pytest.mark.parametrize("nd,n,nmax,eps,sys_size,cap", [
