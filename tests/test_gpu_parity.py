@@ -273,6 +273,36 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
             assert op.get_variant(prec, k) in range(128)
 
 
+def test_projection_chunk_tail_one_wave():
+    """A V panel taller than one projection chunk (2048 rows) ends in a short
+    chunk whose sums go to a partial-sum slot; the one-wave-per-item
+    projection takes that chunk.  A symmetric 1-D grid of 2 x 2078 points
+    splits into level-1 nodes of 2078 rows (d' = 0 compresses the two
+    adjacent halves), i.e. chunks of 2048 and 30 rows.  Both projection forms
+    agree with each other and with the oracle."""
+    n = 2 * 2078
+    say("case chunk tail", n)
+    h = 2.0 / n
+    pts = np.linspace(-1 + h / 2, 1 - h / 2, n).reshape(n, 1)
+    A, ref, ok = build_pair(pts, O.K_INV, 1.0 / n, 1.0, None, 64, 0, 1e-10)
+    op = A.op
+    X = np.stack([O.splitmix_uniform(60 + c, n) for c in range(8)], axis=1)
+    y_ref = ref.matvec(X[:, 0])
+    for prec, tol in ((H.PREC_F64, 1e-14), (H.PREC_F32, 1e-6)):
+        for k in (1, 8):
+            Xk = X[:, :k] if k > 1 else X[:, 0]
+            out = {}
+            for v in (0, H.MV_PROJ_WAVE):
+                op.set_variant(prec, k, v)
+                out[v] = op.matvec(Xk, prec=prec)
+            e = relerr(out[H.MV_PROJ_WAVE], out[0])
+            say(" prec", prec, "K", k, "wave vs workgroup projection", e)
+            assert e < tol, (prec, k, e)
+            y0 = out[H.MV_PROJ_WAVE] if k == 1 else out[H.MV_PROJ_WAVE][:, 0]
+            e_ref = relerr(y0, y_ref)
+            assert e_ref < (1e-12 if prec == H.PREC_F64 else 1e-5), (prec, k, e_ref)
+
+
 @pytest.mark.parametrize("nd,n,nmax,eps,sys_size,cap", [
     (3, 6000, 64, 1e-8, None, -1),        # every compressed block staged
     (2, 5000, 32, 1e-10, None, 1 << 20),  # 1 MiB: most blocks recomputed by the fill pass
