@@ -1402,7 +1402,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             if (k <= 2) cands.push_back(kMatvecProjPack | kMatvecPanelVec);
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
-            if (k <= 16) cands.push_back(kMatvecProjWave | (k >= 4 ? kMatvecPanelMfma | kMatvecPanelWide : kMatvecPanelVec));
+            cands.push_back(kMatvecProjWave | (k >= 4 ? kMatvecPanelMfma | kMatvecPanelWide : kMatvecPanelVec));
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma | kMatvecPanelWide);
             for (int v : cands) {
                 mv_variant[pi][lg] = v;

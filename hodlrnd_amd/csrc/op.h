@@ -178,8 +178,9 @@ struct hodlr_op {
     // [prec f64/f32][log2 nrhs]; set by tune_matvec or the caller.  The
     // defaults: the vector-ALU 16-byte panel up to 4 RHS, the LDS-staged
     // matrix-core panel from 8 on; short projection items one wave each
-    // for FP64 at 8-16 RHS and FP32 at 16; matrix-core projection at 32
-    int mv_variant[2][6] = {{8, 8, 8, 98, 98, 35}, {8, 8, 8, 34, 98, 35}};
+    // for FP64 at 8-16 RHS and FP32 at 16-32; FP64 at 32 RHS has only the
+    // matrix-core projection
+    int mv_variant[2][6] = {{8, 8, 8, 98, 98, 35}, {8, 8, 8, 34, 98, 98}};
     int variant_for(int prec, int nrhs) const;
     // time every kernel form per phase on this operator and keep the fastest
     void tune_matvec(int prec_mask, int reps);

@@ -270,8 +270,8 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * rows of a narrow V-panel tile into one wave load (1-2 RHS), 32 = with 2,
  * the matrix-core panel product staging each group of columns through LDS
  * with 16-byte loads (4-32 RHS), 64 = vector-ALU projection taking items of
- * at most 64 rows one wave each, four per workgroup (1-16 RHS; MFMA takes
- * precedence).  tune times the forms of each pass on this operator for
+ * at most 64 rows one wave each, four per workgroup (1-32 RHS, FP64 to 16;
+ * MFMA takes precedence).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
  * results agree up to rounding order.  get/set read or force the choice for
  * one (prec, nrhs). */
