@@ -122,8 +122,10 @@ __global__ __launch_bounds__(256) void proj_kernel(const ProjItem *__restrict__ 
 // Projection of short items (at most kProjWaveRows rows): one wave per item,
 // four items per workgroup.  The wave walks all of its item's rows (lane l:
 // columns 4l..4l+3, U rows in flight) and writes its sums from registers --
-// no LDS and no workgroup barrier, which in proj_kernel (four waves over a
-// few rows, then four reduction rounds) cost more than the item's loads.
+// no LDS and no workgroup barrier.  In proj_kernel a short item gives each
+// of four waves a few rows and then runs four reduction rounds through LDS;
+// from 8 RHS on those rounds outweigh the item's loads, while at 1-4 RHS
+// the four-wave form, with more loads in flight per item, is faster.
 template <typename T, int K>
 __global__ __launch_bounds__(256) void proj_wave_kernel(const ProjItem *__restrict__ items, int64_t nitems,
                                                         const T *__restrict__ VP, const T *__restrict__ x,
@@ -585,7 +587,6 @@ __global__ __launch_bounds__(256) void panel_mfma_kernel(const PanelItem *__rest
         y[(P.y0 + row) * K + k] = sum;
     }
 }
-
 
 // Geometry of the LDS-staged panel product for waves of RT row tiles of 16.
 template <typename T, int K, int RT>
