@@ -235,7 +235,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
         int bi = INT_MAX;
         for (int j = tid; j < n; j += BS) {
             double v = entry(K, pi, B.ord[B.col0l + j]);
-            for (int q = 0; q < r; q++) v = fma(-U[(size_t)q * m + ri], V[(size_t)q * n + j], v);
+            for (int q = 0; q < r; q++) v = v - U[(size_t)q * m + ri] * V[(size_t)q * n + j];
             row[j] = v;
             if (!cused[j]) {
                 double a = fabs(v);
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
         int bi = INT_MAX;
         for (int i = tid; i < m; i += BS) {
             double v = entry(K, B.ord[B.row0l + i], pj);
-            for (int q = 0; q < r; q++) v = fma(-V[(size_t)q * n + cj], U[(size_t)q * m + i], v);
+            for (int q = 0; q < r; q++) v = v - V[(size_t)q * n + cj] * U[(size_t)q * m + i];
             col[i] = v;
             if (!rused[i]) {
                 double a = fabs(v);
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
 // block's point coordinates in LDS, and the used-index sets are 64-bit
 // masks, so an ACA step has no global-memory round trip and no barrier.
 // The control flow, pivot rules and every reduction (wave_sum trees, the
-// same fma order in the residuals) are those of aca_kernel; the norm
+// same product-then-difference residual updates) are those of aca_kernel; the norm
 // estimate adds the 2|<u_q,u_r>||<v_q,v_r>| terms one q at a time, in the
 // order of LowRank_matrix.hpp:335-338.
 namespace {
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
         for (int q = 0; q < CAP; q++)
             if (q < r) {
                 const double uq = __shfl(Ur[q], ri, kWave);
-                v = fma(-uq, Vr[q], v);
+                v = v - uq * Vr[q];
             }
         rowv = v;
         double bv = 0.0, ba = -1.0;
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
         for (int q = 0; q < CAP; q++)
             if (q < r) {
                 const double vq = __shfl(Vr[q], cj, kWave);
-                v = fma(-vq, Ur[q], v);
+                v = v - vq * Ur[q];
             }
         colv = v;
         double bv = 0.0, ba = -1.0;

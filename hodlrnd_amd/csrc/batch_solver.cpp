@@ -158,6 +158,7 @@ struct Inst {
     std::vector<double> sv, cs, sn;
     double normb = 1.0, beta = 0.0, resid = 0.0;
     bool active = false;
+    bool converged = false;
     int iterations = 0;
 };
 
@@ -165,9 +166,11 @@ struct Inst {
 
 // Batched GMRES: A applies the operator to n x K device vectors; gkey names
 // the operator precision A uses (the step graphs are cached per key), < 0:
-// no graphs.
+// no graphs.  Instance a runs at most budget[a] Arnoldi steps (GMRES.hpp:
+// 298-302 with a per-instance max_iter); one that reaches it is updated with
+// the columns built and reported unconverged.
 void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWork &bw, const double *b,
-                 double *x, bool x0_zero, const std::vector<double> &tol, int max_iter, hipStream_t s,
+                 double *x, bool x0_zero, const std::vector<double> &tol, const std::vector<int> &budget, hipStream_t s,
                  std::vector<int> &iters, std::vector<int> &conv, std::vector<double> &resid, int &matvecs,
                  int gkey, uint64_t gen) {
     const int m = bw.restart, K = bw.K, P = bw.P;
@@ -198,10 +201,13 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
         q.normb = std::sqrt(nb[i]) == 0.0 ? 1.0 : std::sqrt(nb[i]);
         q.beta = std::sqrt(bt[i]);
         q.resid = q.beta / q.normb;
-        q.active = !(q.resid <= tol[i]);  // GMRES.hpp:198-213
+        q.converged = q.resid <= tol[i];  // GMRES.hpp:198-213
+        q.active = !q.converged && budget[i] > 0;
         q.iterations = 0;
         nactive += q.active;
     }
+    int max_iter = 0;
+    for (int i = 0; i < NI; i++) max_iter = std::max(max_iter, budget[i]);
     double *cf = bw.coef.as<double>();  // [0, P*(m+1)*K) h1; then h2; then scratch
     const size_t cfs = (size_t)P * (m + 1) * K;
     std::vector<double> hc(cfs);
@@ -315,9 +321,10 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
                 app_rot(q.sv[i], q.sv[i + 1], q.cs[i], q.sn[i]);
                 q.resid = std::fabs(q.sv[i + 1]) / q.normb;
                 if (!std::isfinite(hn)) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
-                if (q.resid < tol[a]) {
+                if (q.resid < tol[a] || j >= budget[a]) {  // converged, or its budget ends here
                     ks[a] = i;
                     q.iterations = j;
+                    q.converged = q.resid < tol[a];
                     nconv++;
                     scale[a] = 0.0;
                 } else {
@@ -357,6 +364,7 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
             q.resid = q.beta / q.normb;
             if (q.resid < tol[a]) {
                 q.active = false;
+                q.converged = true;
                 q.iterations = j;
                 nactive--;
             }
@@ -366,8 +374,8 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
     conv.assign(NI, 0);
     resid.assign(NI, 0.0);
     for (int a = 0; a < NI; a++) {
-        iters[a] = in[a].active ? max_iter : in[a].iterations;
-        conv[a] = !in[a].active;
+        iters[a] = in[a].active ? std::min(j - 1, budget[a]) : in[a].iterations;
+        conv[a] = in[a].converged;
         resid[a] = in[a].resid;
     }
 }
@@ -404,7 +412,8 @@ void solve_batch(hodlr_op &op, int K, const double *b, double *x, const hodlr_gm
     int mv64 = 0, mv32 = 0;
     if (p.mode == HODLR_SOLVE_F64) {
         std::vector<double> tol(NI, p.tol);
-        gmres_batch(A64, bw, b, x, x0_zero, tol, max_iter, s, it, cv, rs, mv64, graphs ? 0 : -1, op.work_gen);
+        std::vector<int> budget(NI, max_iter);
+        gmres_batch(A64, bw, b, x, x0_zero, tol, budget, s, it, cv, rs, mv64, graphs ? 0 : -1, op.work_gen);
         for (int a = 0; a < NI; a++) {
             infos[a].iterations = it[a];
             infos[a].converged = cv[a];
@@ -436,21 +445,31 @@ void solve_batch(hodlr_op &op, int K, const double *b, double *x, const hodlr_gm
         bool res_f64 = x0_zero;
         const int max_outer = p.max_outer > 0 ? p.max_outer : 20;
         const double floor = p.inner_tol > 0 ? p.inner_tol : 1e-14;
-        std::vector<int> total(NI, 0);
+        // as the single-problem mixed solve: max_iter bounds each instance's
+        // inner iterations over all sweeps, and an instance whose sweep made
+        // no progress is not refined again
+        std::vector<int> total(NI, 0), budget(NI);
+        std::vector<char> stopped(NI, 0);
         int outer = 0;
+        auto done = [&](int a) { return (res_f64 && res[a] < p.tol) || stopped[a] || total[a] >= max_iter; };
         auto all_done = [&] {
             if (!res_f64) return false;
             for (int a = 0; a < NI; a++)
-                if (!(res[a] < p.tol)) return false;
+                if (!done(a)) return false;
             return true;
         };
         while (!all_done() && outer < max_outer) {
             std::vector<double> target(NI);
-            for (int a = 0; a < NI; a++)  // a finished instance gets an unreachable-to-miss target
-                target[a] = (res_f64 && res[a] < p.tol) ? 1e300 : std::min(0.5, std::max(floor, 0.3 * p.tol / std::max(res[a], 1e-300)));
+            for (int a = 0; a < NI; a++) {  // a finished instance gets an unreachable-to-miss target
+                target[a] = done(a) ? 1e300 : std::min(0.5, std::max(floor, 0.3 * p.tol / std::max(res[a], 1e-300)));
+                budget[a] = done(a) ? 0 : max_iter - total[a];
+            }
             HIP_CHECK(hipMemsetAsync(d, 0, sizeof(double) * nk, s));
-            gmres_batch(A32, bw, rr, d, true, target, max_iter, s, it, cv, rs, mv32, graphs ? 1 : -1, op.work_gen);
-            for (int a = 0; a < NI; a++) total[a] += it[a];
+            gmres_batch(A32, bw, rr, d, true, target, budget, s, it, cv, rs, mv32, graphs ? 1 : -1, op.work_gen);
+            for (int a = 0; a < NI; a++) {
+                total[a] += it[a];
+                if (budget[a] > 0 && it[a] == 0) stopped[a] = 1;  // no progress possible at this precision
+            }
             HIP_CHECK(launch_add(x, d, nk, s));
             A64(x, w64);
             mv64++;

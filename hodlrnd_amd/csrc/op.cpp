@@ -449,6 +449,15 @@ void assemble(hodlr_op &op) {
     if (!op.preset_rank.empty()) {  // ranks of a stored operator
         if ((int64_t)op.preset_rank.size() != nb || (int64_t)op.preset_flags.size() != nb)
             throw ArgError{HODLR_E_ARG, "stored block count does not match the rebuilt tree"};
+        // every stored rank and flag is checked against its block before it
+        // drives the layout: flags 0 or 1 (dense), 0 <= rank <= min(m, n),
+        // dense exactly when rank >= min(m, n) - 1 (LowRank_matrix.hpp:348)
+        for (int64_t b = 0; b < nb; b++) {
+            const int mn = std::min(op.b_m[b], op.b_n[b]);
+            const int32_t r = op.preset_rank[b], f = op.preset_flags[b];
+            if ((f != 0 && f != 1) || r < 0 || r > mn || (f == 1) != (r >= mn - 1))
+                throw ArgError{HODLR_E_ARG, "stored rank or flag out of range for block " + std::to_string(b)};
+        }
         op.rank = op.preset_rank;
         op.flags = op.preset_flags;
         todo.clear();

@@ -76,9 +76,15 @@ struct Writer {
 struct Reader {
     std::ifstream f;
     uint64_t h = 1469598103934665603ull;
-    explicit Reader(const char *path) : f(path, std::ios::binary) {
+    uint64_t size = 0;
+    explicit Reader(const char *path) : f(path, std::ios::binary | std::ios::ate) {
         if (!f) throw ArgError{HODLR_E_ARG, std::string("cannot open ") + path};
+        size = (uint64_t)f.tellg();
+        f.seekg(0);
     }
+    // bytes not read yet: bounds every count taken from the file before it
+    // sizes a host allocation
+    uint64_t left() { return size - (uint64_t)f.tellg(); }
     void raw(void *p, size_t nb) {
         f.read(static_cast<char *>(p), (std::streamsize)nb);
         if ((size_t)f.gcount() != nb) throw ArgError{HODLR_E_ARG, "operator file is truncated"};
@@ -191,8 +197,10 @@ hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags) {
         const int hw = r.get<int32_t>();
         (void)r.get<int32_t>();
         if (t->ndim < 1 || t->ndim > kMaxDim || t->n < 0 || t->n > INT32_MAX || t->nmax < 1 || family < 0 ||
-            family > 2)
+            family > 2 || (hw != 0 && hw != 1))
             throw ArgError{HODLR_E_ARG, "corrupt problem header"};
+        if (((uint64_t)t->n * (uint64_t)(t->ndim + hw) + 2 * (uint64_t)t->ndim) * sizeof(double) > r.left())
+            throw ArgError{HODLR_E_ARG, "operator file is truncated (problem points)"};
         t->lo.resize(t->ndim);
         t->hi.resize(t->ndim);
         r.raw(t->lo.data(), sizeof(double) * t->ndim);
@@ -215,7 +223,7 @@ hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags) {
         op->owned_trees.push_back(std::move(t));
     }
     const int64_t nb = r.get<int64_t>();
-    if (nb < 0 || nb > ((int64_t)1 << 40)) throw ArgError{HODLR_E_ARG, "corrupt block count"};
+    if (nb < 0 || (uint64_t)nb * 2 * sizeof(int32_t) > r.left()) throw ArgError{HODLR_E_ARG, "corrupt block count"};
     op->preset_rank.resize(nb);
     op->preset_flags.resize(nb);
     r.raw(op->preset_rank.data(), sizeof(int32_t) * nb);

@@ -92,6 +92,7 @@ struct Column {
     int i = 0;       // Krylov step inside the cycle
     double normb = 1.0, beta = 0.0, res = 0.0, target = 0.0, hn = 0.0;
     int iterations = 0, outer = 0, mv32 = 0, mv64 = 0;
+    int fail = 0;    // status code of a breakdown seen this tick (its request fails)
     std::vector<double> H, sv, cs, sn, y;
     double t_admit = 0;
 };
@@ -297,6 +298,10 @@ struct Engine {
         for (int c = 0; c < K; c++) {
             Column &C = cols[c];
             if ((C.phase != 1 && C.phase != 2) || !C.in_a) continue;
+            if (!std::isfinite(r2[c]) || !std::isfinite(b2[c])) {
+                C.fail = HODLR_E_BREAKDOWN;  // non-finite right-hand side or residual
+                continue;
+            }
             if (C.phase == 1) C.normb = std::sqrt(b2[c]) == 0.0 ? 1.0 : std::sqrt(b2[c]);
             else C.outer++;
             C.beta = std::sqrt(r2[c]);
@@ -388,7 +393,12 @@ struct Engine {
             for (int k = 0; k <= i; k++) Hc(C, k, i) = h1[(size_t)k * K + c] + h2[(size_t)k * K + c];
             (void)nv;
             const double hn = std::sqrt(hn2[c]);
-            if (!std::isfinite(hn)) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
+            bool finite = std::isfinite(hn);
+            for (int k = 0; k <= i && finite; k++) finite = std::isfinite(Hc(C, k, i));
+            if (!finite) {  // this column's request fails; the other columns go on
+                C.fail = HODLR_E_BREAKDOWN;
+                continue;
+            }
             Hc(C, i + 1, i) = hn;
             C.hn = hn;
             for (int k = 0; k < i; k++) app_rot(Hc(C, k, i), Hc(C, k + 1, i), C.cs[k], C.sn[k]);
@@ -420,7 +430,7 @@ struct Engine {
             Column &C = cols[c];
             hdst[c] = nullptr;
             hscale[c] = 0.0;
-            if (C.phase != 3) continue;
+            if (C.phase != 3 || C.fail) continue;
             if (C.finish) {
                 nvu = std::max(nvu, C.i + 1);
             } else {
@@ -433,7 +443,7 @@ struct Engine {
             for (int v = 0; v < nvu; v++)
                 for (int c = 0; c < K; c++) {
                     const Column &C = cols[c];
-                    hcoef[(size_t)v * K + c] = (C.phase == 3 && C.finish && v <= C.i) ? C.y[v] : 0.0;
+                    hcoef[(size_t)v * K + c] = (C.phase == 3 && !C.fail && C.finish && v <= C.i) ? C.y[v] : 0.0;
                 }
         if (any_next) {
             HIP_CHECK(hipMemcpyAsync(tdst.p, hdst, sizeof(double *) * K, hipMemcpyHostToDevice, s));
@@ -446,7 +456,7 @@ struct Engine {
                                    K, s));
         }
         for (auto &C : cols) {
-            if (C.phase != 3) continue;
+            if (C.phase != 3 || C.fail) continue;
             if (C.finish) {
                 C.phase = 2;
                 C.finish = false;
@@ -454,6 +464,17 @@ struct Engine {
                 C.i++;
             }
         }
+    }
+
+    // zeroes every device column of slot c (stream-ordered): a failed
+    // column may leave non-finite values, and a later column in the slot
+    // meets them through masked (zero) coefficients, where 0 * NaN = NaN
+    // (a launch error is left to the next checked call of the tick, which
+    // fails every request on the device)
+    void clear_slot(int c) {
+        for (DevBuf *d : {&b, &x, &rr, &w64, &in, &w}) (void)launch_put_col(d->as<double>(), nullptr, n, K, c, s);
+        for (auto &v : V)
+            if (v.p) (void)launch_put_col(v.as<double>(), nullptr, n, K, c, s);
     }
 
     // solution column c in tree order
@@ -515,6 +536,8 @@ struct hodlr_server {
     void free_op(hodlr::OpEntry &e);
     void fail_op(hodlr::OpEntry &e, int code, const std::string &msg);
     void complete_column(hodlr::OpEntry &e, int c, std::vector<double> &xt);
+    void fail_request(hodlr::OpEntry &e, const std::shared_ptr<hodlr::Request> &r, int code, const std::string &msg);
+    void fail_columns(hodlr::OpEntry &e);
 };
 
 using hodlr::Column;
@@ -669,9 +692,49 @@ void hodlr_server::build_loop(int idx) {
     }
 }
 
+// one request fails (lock held): its status is set once and the operator's
+// reference dropped once; its columns still waiting for a slot are dropped
+// and its columns in slots are freed and zeroed.  Other requests, on this
+// engine or others, are untouched.
+void hodlr_server::fail_request(OpEntry &e, const std::shared_ptr<Request> &r, int code, const std::string &msg) {
+    if (r->status >= 0) {
+        r->status = code;
+        r->t_done = now();
+        e.refs--;
+        e.last_use = now();
+    }
+    for (auto it = e.waiting.begin(); it != e.waiting.end();)
+        it = it->first == r ? e.waiting.erase(it) : it + 1;
+    if (e.engine)
+        for (int c = 0; c < e.engine->K; c++) {
+            Column &C = e.engine->cols[c];
+            if (C.phase && C.req == r) {
+                e.engine->clear_slot(c);
+                C = Column();
+            }
+        }
+    last_error = msg;
+    cv_done.notify_all();
+    cv_mem.notify_all();
+}
+
+// requests of the columns that broke down in this tick's phases (lock held)
+void hodlr_server::fail_columns(OpEntry &e) {
+    std::vector<std::pair<std::shared_ptr<Request>, int>> bad;
+    for (auto &C : e.engine->cols)
+        if (C.phase && C.fail) bad.emplace_back(C.req, C.fail);
+    for (auto &b : bad)
+        fail_request(e, b.first, b.second,
+                     "request " + std::to_string(b.first->id) + ": non-finite residual or Arnoldi norm");
+}
+
 void hodlr_server::complete_column(OpEntry &e, int c, std::vector<double> &xt) {
     Column &C = e.engine->cols[c];
     Request &r = *C.req;
+    if (r.status < 0) {  // the request failed in another column
+        C = Column();
+        return;
+    }
     const auto &perm = e.tree->perm;
     for (int64_t q = 0; q < r.n; q++) r.x[(size_t)C.rc * r.n + perm[q]] = xt[q];
     hodlr_gmres_info &in = r.infos[C.rc];
@@ -761,7 +824,7 @@ void hodlr_server::sched_loop() {
                 e.engine->admit(std::get<1>(a), req, rc, bt, t);
                 std::unique_lock<std::mutex> lk(mu);
                 if (req->t_start < 0) req->t_start = t;
-                req->status = 1;
+                if (req->status >= 0) req->status = 1;
             }
             const std::vector<OpEntry *> &run = active;
             // phase A on every engine, then collect
@@ -776,9 +839,17 @@ void hodlr_server::sched_loop() {
                     complete_column(*run[q], c, xt);
                 }
             }
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                for (OpEntry *e : run) fail_columns(*e);
+            }
             // phase B (Arnoldi step) and C (next vectors / updates)
             for (OpEntry *e : run) e->engine->launch_b();
             for (OpEntry *e : run) e->engine->finish_b();
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                for (OpEntry *e : run) fail_columns(*e);
+            }
             for (OpEntry *e : run) e->engine->launch_c();
             std::unique_lock<std::mutex> lk(mu);
             st.ticks++;
@@ -799,22 +870,17 @@ void hodlr_server::sched_loop() {
             cv_done.notify_all();
             return;
         } catch (const hodlr::ArgError &ex) {
-            // a column broke down: fail the requests of the engines involved
+            // an unexpected argument error inside a tick: the requests in
+            // flight on the engines of this tick fail (each once); columns
+            // that have not started keep waiting
             std::unique_lock<std::mutex> lk(mu);
             for (OpEntry *e : active) e->in_tick = false;
-            last_error = ex.msg;
-            for (auto &kv : ops) {
-                OpEntry &e = *kv.second;
-                if (!e.engine) continue;
-                for (auto &C : e.engine->cols)
-                    if (C.phase && C.req->status >= 0) {
-                        C.req->status = ex.code;
-                        C.req->t_done = now();
-                        e.refs--;
-                        C = Column();
-                    }
+            for (OpEntry *e : active) {
+                std::vector<std::shared_ptr<Request>> inflight;
+                for (auto &C : e->engine->cols)
+                    if (C.phase) inflight.push_back(C.req);
+                for (auto &r : inflight) fail_request(*e, r, ex.code, ex.msg);
             }
-            cv_done.notify_all();
         }
     }
 }

@@ -508,7 +508,7 @@ static int aca_grow(const or_kernel_t *k, const double *pts,
         for (int j_ = 0; j_ < n; j_++) {                                      \
             double v_ = kernel_value(k, pts, rows[(ri)], cols[j_]);           \
             for (int q_ = 0; q_ < r; q_++)                                    \
-                v_ = fma(-U[(size_t)q_ * m + (ri)], V[(size_t)q_ * n + j_], v_); \
+                v_ = v_ - U[(size_t)q_ * m + (ri)] * V[(size_t)q_ * n + j_];  \
             row[j_] = v_;                                                     \
         }                                                                     \
     } while (0)
@@ -517,7 +517,7 @@ static int aca_grow(const or_kernel_t *k, const double *pts,
         for (int i_ = 0; i_ < m; i_++) {                                      \
             double v_ = kernel_value(k, pts, rows[i_], cols[(cj)]);           \
             for (int q_ = 0; q_ < r; q_++)                                    \
-                v_ = fma(-V[(size_t)q_ * n + (cj)], U[(size_t)q_ * m + i_], v_); \
+                v_ = v_ - V[(size_t)q_ * n + (cj)] * U[(size_t)q_ * m + i_];  \
             col[i_] = v_;                                                     \
         }                                                                     \
     } while (0)

@@ -174,3 +174,88 @@ def test_server_32_slots():
     assert info["rc"] == 0 and info["converged"] == 1 and info["max_resid"] < 1e-8
     check_resid(direct_op(P), B, X)
     srv.close()
+
+
+def test_server_nonfinite_rhs_fails_only_its_request():
+    """A right-hand side holding a NaN fails its own request with
+    HODLR_E_BREAKDOWN; a request sharing the engine with it completes with
+    the same result as alone, and the slots it used serve later requests
+    (cleared of its non-finite values) with the same results as alone."""
+    say("case server_nan_rhs")
+    P = pts3(4000, 12)
+    B_good = rhs(4000, 3, 120)
+    B_later = rhs(4000, 4, 130)
+    alone = H.Server(slots=8, hbm_budget=8e9)
+    o = alone.submit_build(P, kern(4000))
+    X_good, _ = alone.wait(alone.submit_solve(o, B_good), timeout=120)
+    X_later, _ = alone.wait(alone.submit_solve(o, B_later), timeout=120)
+    alone.close()
+
+    srv = H.Server(slots=8, hbm_budget=8e9)
+    o = srv.submit_build(P, kern(4000))
+    B_bad = rhs(4000, 6, 140)
+    B_bad[17, 2] = np.nan
+    r_bad = srv.submit_solve(o, B_bad)
+    r_good = srv.submit_solve(o, B_good)
+    _, i_bad = srv.wait(r_bad, timeout=120)
+    X1, i_good = srv.wait(r_good, timeout=120)
+    say("bad", i_bad, "good", i_good)
+    assert i_bad["status"] == H.E_BREAKDOWN and i_bad["rc"] == H.E_BREAKDOWN
+    assert i_good["rc"] == 0 and i_good["converged"] == 1
+    assert relerr(X1, X_good) < 1e-12
+    X2, i_later = srv.wait(srv.submit_solve(o, B_later), timeout=120)  # reuses the failed slots
+    assert i_later["rc"] == 0 and i_later["converged"] == 1
+    assert np.all(np.isfinite(X2)) and relerr(X2, X_later) < 1e-12
+    st = srv.stats()
+    srv.close()
+    assert st["requests_done"] == 2
+
+
+def test_server_poisson_stream_vs_oracle():
+    """A serving stream shaped like the 200 req/s workload, scaled for test
+    time: Poisson arrivals at 200 req/s, N log-uniform in [1000, 8000], 30%
+    of requests build a new operator, 70% reuse a resident one with 1-8 new
+    right-hand sides.  Every column is certified below 1e-8; the columns of
+    requests with N <= 5000 are compared with the FP64 oracle's GMRES
+    solution on its own operator (GMRES.hpp:160-303), within 100 tol."""
+    say("case server_poisson_stream")
+    import time
+    from oracle import pyoracle as O
+    count = 14
+    arrivals = Wl.poisson_arrivals(200.0, count, seed=9)
+    u = Wl.splitmix_uniform(77, 3 * count).reshape(count, 3)
+    srv = H.Server(slots=16, tol=1e-8, hbm_budget=16e9, builders=2)
+    ops, reqs = [], []
+    t0 = time.time()
+    for q in range(count):
+        while time.time() - t0 < arrivals[q]:
+            time.sleep(0.0005)
+        if not ops or u[q, 0] < -0.4:
+            n = Wl.log_uniform_int(u[q, 1], 1000, 8000)
+            P = pts3(n, 300 + q)
+            ops.append((srv.submit_build(P, kern(n)), P))
+            oid, P = ops[-1]
+            k = 1
+        else:
+            oid, P = ops[int((u[q, 1] + 1) / 2 * len(ops)) % len(ops)]
+            k = 1 + int((u[q, 2] + 1) / 2 * 8) % 8
+        B = rhs(len(P), k, 400 + 10 * q)
+        reqs.append((srv.submit_solve(oid, B), P, B))
+    checked = 0
+    for rid, P, B in reqs:
+        X, info = srv.wait(rid, timeout=120)
+        assert info["rc"] == 0 and info["converged"] == 1 and info["max_resid"] < 1e-8, info
+        n = len(P)
+        if n > 5000:
+            continue
+        T = O.Tree(P, -np.ones(3), np.ones(3), 64, 0)
+        ref = O.Operator(T, O.KernelDesc(O.K_INV, 3, 1.0 / n, 1.0), 1e-8, nthreads=16)
+        for c in range(B.shape[1]):
+            xr, ir = ref.gmres(B[:, c], tol=1e-10, nthreads=16)
+            e = relerr(X[:, c], xr)
+            assert e < 100 * 1e-8, (rid, c, e)
+            checked += 1
+    st = srv.stats()
+    srv.close()
+    say("stream: ops", len(ops), "requests", count, "columns vs oracle", checked, "stats", st)
+    assert len(ops) >= 3 and checked >= 5 and st["requests_done"] == count

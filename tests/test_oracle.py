@@ -247,6 +247,40 @@ def test_aca_cross_reproduces_pivots_and_error():
     assert s[a["rank"]] < 1e-8 * s[0] * 100
 
 
+def _fused(a, u, v):
+    """a - u*v rounded once (what an fma would give), via exact rationals."""
+    from fractions import Fraction
+    return float(Fraction(a) - Fraction(u) * Fraction(v))
+
+
+def test_aca_residual_rounds_product_then_difference():
+    """The reference's residual update `row - u[i](ri) * v[i]` (Eigen, g++
+    without -mfma: LowRank_matrix.hpp:130,240) rounds the product, then the
+    difference.  The second step's factors are checked bit for bit against
+    that two-rounding form computed here by numpy, and the check is shown to
+    separate it from a fused (one-rounding) update on this block."""
+    pts = O.splitmix_uniform(31, 3 * 120).reshape(120, 3)
+    pts[:60, 0] -= 4.0
+    k = O.KernelDesc(O.K_INV, 3, 1.0, 0.0)
+    rows, cols = np.arange(60), np.arange(60, 120)
+    a = O.aca(k, pts, rows, cols, 1e-8)
+    # entries ~0.25, far above the pivot floor: the first steps take no retry
+    assert a["rank"] >= 3 and not a["dense"]
+    K = O.dense_matrix(k, pts, rows, cols)
+    U, V = a["U"], a["V"]
+    ri0, cj0 = a["row_piv"][0], a["col_piv"][0]
+    assert ri0 == 0  # LowRank_matrix.hpp:96-97
+    assert np.array_equal(V[:, 0], K[ri0])
+    assert np.array_equal(U[:, 0], (1.0 / K[ri0, cj0]) * K[:, cj0])
+    ri1, cj1 = a["row_piv"][1], a["col_piv"][1]
+    row1 = K[ri1] - U[ri1, 0] * V[:, 0]          # numpy: product, then difference
+    assert np.array_equal(V[:, 1], row1)
+    col1 = K[:, cj1] - V[cj1, 0] * U[:, 0]
+    assert np.array_equal(U[:, 1], (1.0 / row1[cj1]) * col1)
+    fused = np.array([_fused(K[ri1, j], U[ri1, 0], V[j, 0]) for j in range(len(cols))])
+    assert np.count_nonzero(fused != row1) > 0
+
+
 def test_aca_dense_fallback_and_tiny_blocks():
     pts = O.splitmix_uniform(12, 30).reshape(10, 3)
     pts[:5, 0] -= 3.0
@@ -314,6 +348,37 @@ def test_stats_reference_formula():
     assert s["e_near"] == e_near
     r, d = op.block_ranks()
     assert s["n_blocks"] == len(r) and s["n_dense"] == int(d.sum())
+
+
+def test_stats_far_field_count_independent():
+    """The far-field part of n_FLOP (HDD_node.hpp:194-199: rank (m + n) over
+    every interaction-list pair with points on both sides, dense fallback at
+    min(m, n)) counted here by walking the tree's lists and compressing each
+    pair on its own, against the operator's statistics."""
+    pts, k = small_problem(n=1500, seed=23)
+    T = O.Tree(pts, -np.ones(3), np.ones(3), 48, 0)
+    eps = 1e-8
+    op = O.Operator(T, k, eps)
+    s = op.stats()
+    far, nblk, ndense, rmax = 0.0, 0, 0, 0
+    for l in range(1, T.nlevels):
+        for j in range(T.level_size(l)):
+            rows = T.points(l, j)
+            if len(rows) == 0:
+                continue
+            for b in T.interactions(l, j):
+                cols = T.points(l, b)
+                if len(cols) == 0:
+                    continue
+                a = O.aca(k, pts, rows, cols, eps, seed=O.lib.or_block_seed(l, j, int(b)))
+                assert a["rank"] == (min(len(rows), len(cols)) if a["dense"] else a["steps"])
+                far += a["rank"] * (len(rows) + len(cols))
+                nblk += 1
+                ndense += a["dense"]
+                rmax = max(rmax, a["rank"])
+    assert nblk > 100 and 0 < ndense < nblk
+    assert s["n_blocks"] == nblk and s["n_dense"] == ndense and s["max_rank"] == rmax
+    assert s["n_flop_ref"] - s["e_near"] == far
 
 
 # ---------------------------------------------------------------- GMRES and QR
