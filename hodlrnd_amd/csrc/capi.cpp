@@ -201,6 +201,7 @@ int hodlr_ctx_create(int device, hodlr_ctx **out) {
         HIP_CHECK(hipSetDevice(device));
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         keep_pool_memory(device);
+        noalloc_scratch_setup(device);  // this device's copy of the allocation-free matvec's array
     });
     if (rc != HODLR_OK) {
         delete ctx;

@@ -1328,17 +1328,41 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
         ev_pending.push_back({prec, first});
         ev = &ev_pool[first];
     }
+    MatvecWork w;
+    w.t = prec == HODLR_PREC_F64 ? t64.p : t32.p;
+    w.part = prec == HODLR_PREC_F64 ? part64.p : part32.p;
+    w.skt1 = skt1.as<double>();
+    w.skt2 = skt2.as<double>();
+    w.skpart = skpart.as<double>();
+    matvec_dev_on(prec, nrhs, x, y, s, w, ev, add_diag);
+}
+
+void hodlr_op::matvec_work_sizes(int prec, int nrhs, size_t sz[5]) const {
+    using namespace hodlr;
+    const size_t es = prec == HODLR_PREC_F64 ? sizeof(double) : sizeof(float);
+    sz[0] = es * (size_t)(n_t > 0 ? n_t : 1) * nrhs;
+    sz[1] = es * (size_t)(n_slots > 0 ? n_slots : 1) * kProjTile * nrhs;
+    sz[2] = sz[3] = sz[4] = 0;
+    if (n_skcore) {
+        sz[2] = sz[3] = sizeof(double) * (size_t)n_sk * nrhs;
+        sz[4] = sizeof(double) * (size_t)(n_skslots > 0 ? n_skslots : 1) * nrhs;
+    }
+}
+
+void hodlr_op::matvec_dev_on(int prec, int nrhs, const void *x, void *y, hipStream_t s, const hodlr::MatvecWork &w,
+                             hipEvent_t *ev, bool add_diag) {
+    using namespace hodlr;
     if (prec == HODLR_PREC_F64) {
         HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, projw.as<ProjItem>(), n_projw_big,
                                         red.as<ProjReduce>(), n_red,
                                         panel.as<PanelItem>(), n_panel,
                                         segs.as<Segment>(), quads.as<QuadDesc>(), VP64.as<double>(), NP64.as<double>(),
-                                        UP64.as<double>(), static_cast<const double *>(x), t64.as<double>(),
-                                        part64.as<double>(), tidx.as<int32_t>(), static_cast<double *>(y),
+                                        UP64.as<double>(), static_cast<const double *>(x), static_cast<double *>(w.t),
+                                        static_cast<double *>(w.part), tidx.as<int32_t>(), static_cast<double *>(y),
                                         variant_for(prec, nrhs), ev, s));
         if (n_skcore)
-            HIP_CHECK(launch_skeleton<double>(nrhs, skel_plan(), static_cast<const double *>(x), skt1.as<double>(),
-                                              skpart.as<double>(), skt2.as<double>(), static_cast<double *>(y), s));
+            HIP_CHECK(launch_skeleton<double>(nrhs, skel_plan(), static_cast<const double *>(x), w.skt1, w.skpart,
+                                              w.skt2, static_cast<double *>(y), s));
         if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     } else {
         if (!(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
@@ -1347,12 +1371,12 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
                                        panel.as<PanelItem>(), n_panel,
                                        segs.as<Segment>(), quads.as<QuadDesc>(), VP32.as<float>(), NP32.as<float>(),
                                        UP32.as<float>(),
-                                       static_cast<const float *>(x), t32.as<float>(), part32.as<float>(),
-                                       tidx.as<int32_t>(), static_cast<float *>(y), variant_for(prec, nrhs), ev,
-                                       s));
+                                       static_cast<const float *>(x), static_cast<float *>(w.t),
+                                       static_cast<float *>(w.part), tidx.as<int32_t>(), static_cast<float *>(y),
+                                       variant_for(prec, nrhs), ev, s));
         if (n_skcore)
-            HIP_CHECK(launch_skeleton<float>(nrhs, skel_plan(), static_cast<const float *>(x), skt1.as<double>(),
-                                             skpart.as<double>(), skt2.as<double>(), static_cast<float *>(y), s));
+            HIP_CHECK(launch_skeleton<float>(nrhs, skel_plan(), static_cast<const float *>(x), w.skt1, w.skpart,
+                                             w.skt2, static_cast<float *>(y), s));
         if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
         if (add_diag)
             HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x),

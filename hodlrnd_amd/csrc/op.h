@@ -67,6 +67,14 @@ hipError_t launch_add(double *a, const double *b, int64_t n, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, int64_t n, hipStream_t s);
 hipError_t launch_to_f64(const float *a, double *b, int64_t n, hipStream_t s);
 
+// device buffers of one matvec's intermediates: the projection's V^T x
+// values and partial-sum slots (operator precision), the skeleton phase's
+// slot values and partial sums (FP64)
+struct MatvecWork {
+    void *t = nullptr, *part = nullptr;
+    double *skt1 = nullptr, *skt2 = nullptr, *skpart = nullptr;
+};
+
 }  // namespace hodlr
 
 // The operator of one problem, or the block-diagonal operator of a batch of
@@ -162,6 +170,13 @@ struct hodlr_op {
     void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr,
                     bool add_diag = true);
     void ensure_work(int prec, int nrhs);
+    // the same product with its intermediates in caller-given device
+    // buffers (sizes from matvec_work_sizes): allocates nothing
+    void matvec_dev_on(int prec, int nrhs, const void *x, void *y, hipStream_t s, const hodlr::MatvecWork &w,
+                       hipEvent_t *ev = nullptr, bool add_diag = true);
+    // bytes of the intermediates of one (prec, nrhs) product, in the order
+    // of MatvecWork: t, part, skt1, skt2, skpart (zero where unused)
+    void matvec_work_sizes(int prec, int nrhs, size_t sz[5]) const;
 
     // called by assemble once the exact arena footprint (bytes, all
     // requested precisions) is known and before it is allocated; may block
@@ -193,6 +208,9 @@ void assemble(hodlr_op &op);
 void set_assembly_threads(int n);
 // keep freed stream-ordered memory of the device's default pool mapped
 void keep_pool_memory(int device);
+// device address of this device's scratch array of the allocation-free
+// matvec, looked up once per device (noalloc.hip; the device must be current)
+void *noalloc_scratch_setup(int device);
 // operator files (opio.cpp)
 void save_op(hodlr_op &op, const char *path);
 hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags);

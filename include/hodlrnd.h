@@ -208,6 +208,18 @@ int hodlr_matvec(hodlr_op *op, int prec, int nrhs, const double *x, double *y);
  * F32: float, same layout. */
 int hodlr_matvec_device(hodlr_op *op, int prec, int nrhs, const void *x, void *y,
                         void *stream);
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The product of hodlr_matvec_device -- same arguments, same
+ * kernels, bit-identical results -- with its intermediates (the V^T x
+ * values, partial sums and skeleton slot values) taken from one fixed
+ * 64 MiB device array per GPU instead of buffers the operator allocates;
+ * the call allocates nothing.  An operator and RHS count whose
+ * intermediates need more than 64 MiB are refused with HODLR_E_NOMEM and
+ * a message stating the size.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race (serialise them). */
+int hodlr_matvec_device_noalloc(hodlr_op *op, int prec, int nrhs, const void *x, void *y,
+                                void *stream);
 
 /* Rows of the dense operator applied to x, evaluated on the device in FP64
  * by direct summation (the dense self-check of main.cpp:42-57, on chosen
