@@ -50,20 +50,23 @@ def build(verbose=True):
 
 
 def build_examples(verbose=True):
-    """The reference-shaped C++ API example (host compiler + the C ABI)."""
+    """The reference-shaped C++ API examples (host compiler + the C ABI)."""
     out_dir = os.path.join(HERE, "examples", "build")
     os.makedirs(out_dir, exist_ok=True)
     root = os.path.dirname(HERE)
-    out = os.path.join(out_dir, "main_4d")
-    cmd = ["g++", "-std=c++17", "-O2", "-I" + os.path.join(root, "include"),
-           os.path.join(HERE, "examples", "main_4d.cpp"), "-o", out, "-L" + HERE, "-lhodlrnd",
-           "-Wl,-rpath,$ORIGIN/../.."]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("example build failed:\n" + r.stderr[-4000:])
-    if verbose:
-        print("built", out)
-    return out
+    outs = []
+    for name in ("main_4d", "user_kernels"):
+        out = os.path.join(out_dir, name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(root, "include"),
+               os.path.join(HERE, "examples", name + ".cpp"), "-o", out, "-L" + HERE, "-lhodlrnd",
+               "-Wl,-rpath,$ORIGIN/../.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("example build failed:\n" + r.stderr[-4000:])
+        if verbose:
+            print("built", out)
+        outs.append(out)
+    return outs
 
 
 if __name__ == "__main__":

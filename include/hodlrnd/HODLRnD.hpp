@@ -12,22 +12,36 @@
 //     operator*, solve, get_size, print_matrix_statistics  (HDD_matrix.hpp:20-85)
 //   iterSolver<HODLRdD_matrix<K>>(restart, max_iter, tol).GMRES(A, x, b)
 //                                                          (GMRES.hpp:44-50,160)
+//   iterSolver<Op> for any Op with Vec operator*(Vec)      (GMRES.hpp:31-303)
 // The knobs are compile-time constants as in the reference; define
 // HODLR_NDIM, HODLR_NMAX, HODLR_EPS_ACA or HODLR_DPRIME before including to
-// change them.  A user kernel class must also provide
+// change them.
+//
+// User kernel classes written for the reference compile unchanged: the
+// device cannot call their host getMatrixEntry, so the operator identifies
+// the entries instead.  It probes getMatrixEntry and accepts
+//   A_ij = diag (i == j),  w_j * f(|x_i - x_j|) (i != j),
+//   f one of log r, 1/r, exp(-r), w_j a constant or a per-column weight,
+// when every probed entry agrees to 1e-13 relative; the entries are then
+// evaluated on the GPU.  A kernel of another form fails with a message; such
+// a class can describe itself with
 //   hodlr_kernel_desc device_kernel() const
-// describing A_ij for evaluation on the GPU (the reference evaluates
-// getMatrixEntry on the host; the device has no access to host code).
+// which, when present, is used without probing.
 // Vectors are plain host arrays (the reference uses Eigen, which this API
-// does not require).  Failures throw hodlr::Error.
+// does not require; the Eigen vector names its user kernels use are
+// provided unless Eigen itself is included first).  Failures throw
+// hodlr::Error.
 #pragma once
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../hodlrnd.h"
@@ -103,7 +117,36 @@ public:
         for (size_t i = 0; i < d_.size(); i++) r.d_[i] -= o.d_[i];
         return r;
     }
+    Vec operator+(const Vec &o) const {
+        Vec r(*this);
+        for (size_t i = 0; i < d_.size(); i++) r.d_[i] += o.d_[i];
+        return r;
+    }
+    Vec operator*(double a) const {
+        Vec r(*this);
+        for (auto &v : r.d_) v *= a;
+        return r;
+    }
+    friend Vec operator*(double a, const Vec &v) { return v * a; }
+    Vec &operator-=(const Vec &o) {
+        for (size_t i = 0; i < d_.size(); i++) d_[i] -= o.d_[i];
+        return *this;
+    }
+    Vec &operator+=(const Vec &o) {
+        for (size_t i = 0; i < d_.size(); i++) d_[i] += o.d_[i];
+        return *this;
+    }
+    double squaredNorm() const { return dot(*this); }
 };
+
+// the Eigen vector names the reference's user kernels use
+// (user_kernel.hpp:17-27: VectorXd, Eigen::VectorXd)
+#if !defined(EIGEN_WORLD_VERSION) && !defined(HODLR_NO_EIGEN_NAMES)
+using VectorXd = Vec;
+namespace Eigen {
+using VectorXd = ::Vec;
+}
+#endif
 
 // ---- points (points_dt.hpp:9-39) ----
 struct ptsnD {
@@ -135,6 +178,9 @@ inline Vec uniform_nodes(double a, double b, int n) {
 namespace Vec_ops {
 inline double relative_error(const Vec &X_ori, const Vec &X_comp) { return (X_ori - X_comp).norm() / X_ori.norm(); }
 }  // namespace Vec_ops
+
+// myHeaders.hpp:122-127
+inline std::string getTimeStamp() { return "\n" + std::string(__DATE__) + "," + std::string(__TIME__) + "\n"; }
 
 // ---- test kernels (user_kernel.hpp:10-179): tensor grid, dim 0 outermost ----
 namespace hodlr_detail {
@@ -201,6 +247,9 @@ inline double quadruple_integral(const double *a, const double *b) {
     return result * cx[0] * cx[1] * cx[2] * cx[3];
 }
 }  // namespace hodlr_detail
+
+// integral_equation_4D/integral4d.hpp:18 (used by the reference's 4-D kernel)
+inline double quadruple_integral(double *a, double *b) { return hodlr_detail::quadruple_integral(a, b); }
 
 class kernel_2d_test {
     double c = -1.0 / (2 * PI);
@@ -281,8 +330,235 @@ public:
     Operator *get() { return kernel; }
 };
 
+// ---- entries of a user kernel on the device ----
+namespace hodlr_detail {
+template <class K, class = void>
+struct has_device_kernel : std::false_type {};
+template <class K>
+struct has_device_kernel<K, std::void_t<decltype(std::declval<K &>().device_kernel())>> : std::true_type {};
+
+inline double family_value(int family, double r) {
+    switch (family) {
+    case HODLR_KERNEL_LOG: return std::log(r);
+    case HODLR_KERNEL_INV_R: return 1.0 / r;
+    default: return std::exp(-r);
+    }
+}
+
+// The device evaluates A_ij = diag (i == j), scale * f(r_ij) * w_j (i != j)
+// for f in {log r, 1/r, exp(-r)} (hodlr_kernel_desc).  A reference-shaped
+// kernel class only has the host callback getMatrixEntry
+// (kernel_function.hpp:26,38,51), so its entries are matched to that form:
+// the diagonal from getMatrixEntry(i, i) (equal on every probed i), then for
+// each f a weight per column w_j = A(i_j, j) / f(r) from one partner row
+// i_j, checked on 512 further pairs (i, j) to 1e-13 relative.  Equal weights
+// become the scale; otherwise they are passed per point in `weights`.
+template <class K>
+hodlr_kernel_desc identify_kernel(K *ker, const std::vector<ptsnD> &P, std::vector<double> &weights) {
+    const int64_t n = (int64_t)P.size();
+    auto A = [&](int64_t i, int64_t j) { return (double)ker->getMatrixEntry((int)i, (int)j); };
+    auto r_of = [&](int64_t i, int64_t j) { return nd_points::euclidean_distance(P[(size_t)i], P[(size_t)j]); };
+    hodlr_kernel_desc kd{HODLR_KERNEL_INV_R, NDIM, 0.0, 0.0, nullptr};
+    weights.clear();
+    if (n == 0) return kd;
+    kd.diag = A(0, 0);
+    const int64_t step = n > 64 ? n / 64 : 1;
+    for (int64_t i = 0; i < n; i += step)
+        if (A(i, i) != kd.diag)
+            throw hodlr::Error(HODLR_E_ARG, "kernel has no single diagonal value (getMatrixEntry(i, i) varies); "
+                                            "describe it with device_kernel()");
+    if (n == 1) return kd;
+    // a deterministic stream of off-diagonal probe pairs
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto next = [&](int64_t m) {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return (int64_t)(st % (uint64_t)m);
+    };
+    const int fams[3] = {HODLR_KERNEL_INV_R, HODLR_KERNEL_LOG, HODLR_KERNEL_EXP};
+    for (int family : fams) {
+        // per-column weight from a partner row whose f(r) is finite and non-zero
+        auto weight = [&](int64_t j, double &w) {
+            for (int t = 0; t < 8; t++) {
+                int64_t i = t == 0 ? (j + n / 2) % n : next(n);
+                if (i == j) i = (j + 1) % n;
+                const double f = family_value(family, r_of(i, j));
+                if (std::isfinite(f) && f != 0.0) {
+                    w = A(i, j) / f;
+                    return std::isfinite(w);
+                }
+            }
+            return false;
+        };
+        std::vector<double> w;
+        bool ok = true, uniform = true;
+        double w0 = 0.0;
+        const int64_t nprobe = n < 64 ? n : 64;
+        for (int64_t q = 0; q < nprobe && ok; q++) {  // a sample of columns first
+            double wj;
+            ok = weight((q * step) % n, wj);
+            if (!ok) break;
+            if (q == 0) w0 = wj;
+            uniform = uniform && std::fabs(wj - w0) <= 1e-13 * std::fabs(w0);
+        }
+        if (!ok) continue;
+        if (!uniform) {
+            w.resize((size_t)n);
+            for (int64_t j = 0; j < n && ok; j++) ok = weight(j, w[(size_t)j]);
+            if (!ok) continue;
+        }
+        for (int q = 0; q < 512 && ok; q++) {
+            int64_t i = next(n), j = next(n);
+            if (i == j) continue;
+            const double f = family_value(family, r_of(i, j));
+            const double want = A(i, j), got = (uniform ? w0 : w[(size_t)j]) * f;
+            ok = std::isfinite(f) && std::fabs(got - want) <= 1e-13 * std::fabs(want) + 1e-300;
+        }
+        if (!ok) continue;
+        kd.family = family;
+        if (uniform) {
+            kd.scale = w0;
+        } else {
+            kd.scale = 1.0;
+            weights = std::move(w);
+            kd.weight = weights.data();
+        }
+        return kd;
+    }
+    throw hodlr::Error(HODLR_E_ARG, "kernel entries are not diag / w_j f(|x_i - x_j|) with f = log r, 1/r or "
+                                    "exp(-r); describe the kernel with device_kernel()");
+}
+}  // namespace hodlr_detail
+
+// ---- GMRES for any operator (GMRES.hpp:31-303) ----
+// Restarted GMRES(m) with modified Gram-Schmidt, real Givens rotations and
+// back substitution, on host vectors, for any Op with Vec operator*(const
+// Vec&) (the reference's iterSolver is generic the same way).  The HODLR
+// operator has its own specialization below that runs on the device.
 template <class Op>
-class iterSolver;
+class iterSolver {
+    int restart, max_iter, num_iter = 0;
+    double tol, resid = 0.0;
+    std::string fname;
+    bool ptofile = false;
+
+    static void gen_rot(double &dx, double &dy, double &cs, double &sn) {  // GMRES.hpp:83-110
+        if (dy == 0.0) {
+            cs = 1.0;
+            sn = 0.0;
+        } else if (std::fabs(dy) > std::fabs(dx)) {
+            const double t = dx / dy;
+            sn = 1.0 / std::sqrt(1.0 + t * t);
+            cs = t * sn;
+        } else {
+            const double t = dy / dx;
+            cs = 1.0 / std::sqrt(1.0 + t * t);
+            sn = t * cs;
+        }
+    }
+    static void app_rot(double &dx, double &dy, double cs, double sn) {  // GMRES.hpp:112-129
+        const double t = cs * dx + sn * dy;
+        dy = -sn * dx + cs * dy;
+        dx = t;
+    }
+
+public:
+    iterSolver(int restart_, int max_iter_, double tol_) : restart(restart_), max_iter(max_iter_), tol(tol_) {}
+    iterSolver(int max_iter_, double tol_) : restart(max_iter_), max_iter(max_iter_), tol(tol_) {}
+    void set_output_file(std::string f) {
+        fname = std::move(f);
+        ptofile = true;
+    }
+    double getResidual() { return resid; }
+    int getMaxIterations() { return num_iter; }
+    // x holds the initial guess on entry and the solution on exit; returns
+    // the iteration count (max_iter when tol was not reached)
+    int GMRES(Op *&A, Vec &x, Vec &b) {
+        FILE *f = ptofile ? std::fopen(fname.c_str(), "w") : nullptr;
+        if (f)
+            std::fprintf(f, "Restarted GMRES Parameters \nMaximum Iterations : %d\nTolerance :%g\nRestart Iteration at %d\n",
+                         max_iter, tol, restart);
+        double mv_time = 0.0;
+        int mv_count = 0;
+        auto apply = [&](const Vec &v) {
+            const auto t0 = std::chrono::steady_clock::now();
+            Vec y = (*A) * v;
+            mv_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            mv_count++;
+            return y;
+        };
+        auto finish = [&](const char *why) {
+            if (!f) return;
+            std::fprintf(f, "%s\nResidual : %g\nMat-Vec Time : %g\n", why, resid, mv_time / mv_count);
+            std::fclose(f);
+            f = nullptr;
+        };
+        const int m = restart;
+        std::vector<double> H((size_t)(m + 1) * m, 0.0), s(m + 1), cs(m + 1), sn(m + 1);
+        auto h = [&](int r, int c) -> double & { return H[(size_t)c * (m + 1) + r]; };
+        double normb = b.norm();
+        Vec r = b - apply(x);
+        double beta = r.norm();
+        if (normb == 0.0) normb = 1.0;
+        resid = beta / normb;
+        if (resid <= tol) {
+            num_iter = 0;
+            finish("Exited before any Progress");
+            return 0;
+        }
+        std::vector<Vec> v(m + 1);
+        auto update = [&](int k) {  // GMRES.hpp:132-146: x += V y, H y = s by back substitution
+            std::vector<double> y(s.begin(), s.begin() + k + 1);
+            for (int i = k; i >= 0; i--) {
+                y[i] /= h(i, i);
+                for (int j = i - 1; j >= 0; j--) y[j] -= h(j, i) * y[i];
+            }
+            for (int j = 0; j <= k; j++) x += v[j] * y[j];
+        };
+        int j = 1;
+        while (j <= max_iter) {
+            v[0] = r * (1.0 / beta);
+            std::fill(s.begin(), s.end(), 0.0);
+            s[0] = beta;
+            int i = 0;
+            for (; i < m && j <= max_iter; i++, j++) {
+                if (f) std::fprintf(f, "Iter[%d] : %.8g\n", j, resid);
+                Vec w = apply(v[i]);
+                for (int k = 0; k <= i; k++) {  // modified Gram-Schmidt
+                    h(k, i) = w.dot(v[k]);
+                    w -= v[k] * h(k, i);
+                }
+                h(i + 1, i) = w.norm();
+                // a zero norm (exact solution in the space) gives a zero vector
+                v[i + 1] = h(i + 1, i) != 0.0 ? w * (1.0 / h(i + 1, i)) : Vec(w.size());
+                for (int k = 0; k < i; k++) app_rot(h(k, i), h(k + 1, i), cs[k], sn[k]);
+                gen_rot(h(i, i), h(i + 1, i), cs[i], sn[i]);
+                app_rot(h(i, i), h(i + 1, i), cs[i], sn[i]);
+                app_rot(s[i], s[i + 1], cs[i], sn[i]);
+                resid = std::fabs(s[i + 1]) / normb;
+                if (resid < tol) {
+                    update(i);
+                    num_iter = j;
+                    finish("Reached Solution before Max_Iterations");
+                    return j;
+                }
+            }
+            update(i - 1);  // the columns of this cycle (m of them unless max_iter ended it)
+            r = b - apply(x);
+            beta = r.norm();
+            resid = beta / normb;
+            if (resid < tol) {
+                num_iter = j;
+                finish("Reached Solution before Max_Iterations");
+                return j;
+            }
+        }
+        if (f) std::fclose(f);
+        num_iter = max_iter;
+        return max_iter;
+    }
+};
 
 // ---- the operator (HDD_matrix.hpp:10-89) ----
 template <class Kernel>
@@ -293,6 +569,9 @@ class HODLRdD_matrix {
     hodlr_op *op_ = nullptr;
     kernel_function<Kernel> *kernel_func;
     Kernel *kernel_;
+    std::vector<ptsnD> *points_;
+    std::vector<double> lo_, hi_, weights_;
+    double matvec_seconds_ = 0.0;
 
     void check(int rc, const char *what) {
         if (rc != HODLR_OK && rc != HODLR_E_NOCONV)
@@ -305,7 +584,10 @@ public:
         std::printf("HMATRIX START\n");
         kernel_ = userkernel;
         kernel_func = new kernel_function<Kernel>(userkernel);
+        points_ = gPoints;
         N_ = gPoints->size();
+        lo_.assign(x1.data(), x1.data() + NDIM);
+        hi_.assign(x2.data(), x2.data() + NDIM);
         std::vector<double> pts(N_ * NDIM);
         for (size_t i = 0; i < N_; i++)
             for (int d = 0; d < NDIM; d++) pts[i * NDIM + d] = (*gPoints)[i].x[d];
@@ -326,8 +608,17 @@ public:
         delete kernel_func;
     }
     size_t get_size() { return N_; }
+    // the kernel's description for the device: its own device_kernel()
+    // when it has one, else identified from getMatrixEntry
+    hodlr_kernel_desc device_description() {
+        if constexpr (hodlr_detail::has_device_kernel<Kernel>::value) {
+            return kernel_->device_kernel();
+        } else {
+            return hodlr_detail::identify_kernel(kernel_, *points_, weights_);
+        }
+    }
     void Assemble_matrix_operators() {
-        hodlr_kernel_desc kd = kernel_->device_kernel();
+        hodlr_kernel_desc kd = device_description();
         check(hodlr_op_assemble_skeleton(ctx_, 1, &tree_, &kd, eps_ACA, SYS_SIZE, HODLR_PREC_F64 | HODLR_PREC_F32,
                                          &op_),
               "op_assemble");
@@ -335,24 +626,74 @@ public:
     }
     Vec operator*(const Vec &x) {
         Vec b(N_);
+        const auto t0 = std::chrono::steady_clock::now();
         check(hodlr_matvec(op_, HODLR_PREC_F64, 1, x.data(), b.data()), "matvec");
+        matvec_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         return b;
     }
     // GMRES(100), max_iter = N, tol = eps_ACA, x0 = b (HDD_matrix.hpp:51-62)
     Vec solve(Vec &b);
-    void print_matrix_statistics() {
+    // HDD_matrix.hpp:63-67 -> HDD_tree.hpp:117-132, HDD_node.hpp:100-113:
+    // every node's point count, box, neighbours and interaction list
+    void print_matrix_details() {
+        std::printf("============HODLRnD=================\n");
+        hodlr_tree_info ti;
+        hodlr_tree_get_info(tree_, &ti);
+        std::vector<int64_t> loff(ti.nlevels + 1), start(ti.nnodes), count(ti.nnodes), nptr(ti.nnodes + 1),
+            iptr(ti.nnodes + 1);
+        hodlr_tree_nodes(tree_, loff.data(), start.data(), count.data());
+        hodlr_tree_lists(tree_, 0, nptr.data(), nullptr);
+        hodlr_tree_lists(tree_, 1, iptr.data(), nullptr);
+        std::vector<int64_t> nidx(nptr[ti.nnodes] > 0 ? nptr[ti.nnodes] : 1),
+            iidx(iptr[ti.nnodes] > 0 ? iptr[ti.nnodes] : 1);
+        hodlr_tree_lists(tree_, 0, nptr.data(), nidx.data());
+        hodlr_tree_lists(tree_, 1, iptr.data(), iidx.data());
+        std::printf("Tree Depth:%d\n______________________________\n\n", ti.nlevels);
+        for (int l = 0; l < ti.nlevels; l++) {
+            std::printf("Level[%d]\n", l);
+            for (int64_t j = 0; j < loff[l + 1] - loff[l]; j++) {
+                const int64_t q = loff[l] + j;
+                std::printf("**************\nNode [%lld] , Nunmber particles = %lld\n", (long long)j,
+                            (long long)count[q]);
+                // the box: midpoint splits from the root box, dimension 0 in
+                // the most significant bit of each level's child index
+                std::printf("\nBounding box : ");
+                for (int d = 0; d < NDIM; d++) {
+                    double a = lo_[d], b = hi_[d];
+                    for (int t = l - 1; t >= 0; t--) {
+                        const double mid = 0.5 * (a + b);
+                        if ((j >> (t * NDIM + (NDIM - 1 - d))) & 1) a = mid;
+                        else b = mid;
+                    }
+                    std::printf(d + 1 < NDIM ? "[%g,%g]x" : "[%g,%g]", a, b);
+                }
+                std::printf("\nNeighbours -");
+                for (int64_t k = nptr[q]; k < nptr[q + 1]; k++) std::printf(" %lld", (long long)nidx[k]);
+                std::printf("\nInteraction List -");
+                for (int64_t k = iptr[q]; k < iptr[q + 1]; k++) std::printf(" %lld", (long long)iidx[k]);
+                std::printf("\n**************\n\n");
+            }
+            std::printf("______________________________\n");
+        }
+        std::printf("====================================\n");
+    }
+    void print_matrix_statistics() {  // HDD_matrix.hpp:69-85
         hodlr_op_stats s;
         hodlr_op_get_stats(op_, &s);
+        std::printf("%s\n", getTimeStamp().c_str());
         std::printf("Memory (in GB) : %g\n", 8e-9 * s.n_flop_ref);
         std::printf("Number of FLOP : %g\n", s.n_flop_ref);
         std::printf("Compression Ratio : %g\n", s.n_flop_ref / ((double)N_ * (double)N_));
         std::printf("Maximum rank across the Tree : %lld\n", (long long)s.max_rank);
         std::printf("Time to Initialize (in s) : %g\n", s.assemble_seconds);
+        // wall time of the last operator* (host vectors in and out, as the
+        // reference's timer around Tree::mat_vec, HDD_tree.hpp:95-99)
+        std::printf("Matrix-Vector time (in s) : %g\n", matvec_seconds_);
     }
     hodlr_op *handle() { return op_; }
 };
 
-// GMRES.hpp:30-53 for the HODLR operator: the solve runs on the device.
+// GMRES.hpp:30-303 for the HODLR operator: the solve runs on the device.
 template <class Kernel>
 class iterSolver<HODLRdD_matrix<Kernel>> {
     int restart, max_iter, num_iter = 0;
@@ -383,16 +724,26 @@ public:
         hodlr_gmres_info info{};
         int rc = hodlr_gmres(A->op_, b.data(), x.data(), &p, &info);
         A->check(rc, "gmres");
+        // the log's mean matvec time: the device product timed with HIP
+        // events (3 products on the operator's stream)
+        double mv_ms[4] = {0, 0, 0, 0};
+        if (ptofile) A->check(hodlr_matvec_timing(A->op_, HODLR_PREC_F64, 1, 3, mv_ms), "matvec_timing");
         resid = info.resid;
         num_iter = info.iterations;
-        if (ptofile) {  // GMRES.hpp:166-172,227-228
+        if (ptofile) {  // GMRES.hpp:166-172,206-209,227-228,260-263,287-290
             FILE *f = std::fopen(fname.c_str(), "w");
             if (f) {
                 std::fprintf(f, "Restarted GMRES Parameters \nMaximum Iterations : %d\nTolerance :%g\n"
                                 "Restart Iteration at %d\n",
                              max_iter, tol, restart);
-                for (int i = 0; i < info.hist_len; i++) std::fprintf(f, "Iter[%d] : %.8g\n", i + 1, hist[i]);
-                std::fprintf(f, "Residual : %g\n", info.resid);
+                for (int i = 0; i < info.hist_len && i < info.iterations; i++)
+                    std::fprintf(f, "Iter[%d] : %.8g\n", i + 1, hist[i]);
+                if (info.iterations == 0 && info.converged)
+                    std::fprintf(f, "Exited before any Progress\n");
+                else if (info.converged)
+                    std::fprintf(f, "Reached Solution before Max_Iterations\n");
+                if (info.converged || info.iterations == 0)
+                    std::fprintf(f, "Residual : %g\nMat-Vec Time : %g\n", info.resid, mv_ms[0] * 1e-3);
                 std::fclose(f);
             }
         }

@@ -814,9 +814,14 @@ static void build_block(or_op_t *op, or_block_t *b) {
         free(U);
         free(V);
     } else {
+        /* keep the r factor columns only (U, V are column-major, m x cap and
+         * n x cap, cap >= r: the first r columns are contiguous) */
         b->kind = BLK_LOWRANK;
-        b->L = U;
-        b->R = V;
+        const int r = info.rank;
+        double *L = (double *)realloc(U, sizeof(double) * (size_t)m * (size_t)r + 8);
+        double *R = (double *)realloc(V, sizeof(double) * (size_t)n * (size_t)r + 8);
+        b->L = L ? L : U;
+        b->R = R ? R : V;
     }
     free(rp);
     free(cp);

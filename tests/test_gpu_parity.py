@@ -208,6 +208,37 @@ def test_cpp_api_reference_driver():
     assert mv < 1e-8 and sol < 1e-8
 
 
+def test_cpp_api_reference_shaped_kernels(tmp_path):
+    """hodlrnd_amd/examples/user_kernels.cpp: kernel classes in the
+    reference's shape with no device description (their entries identified
+    from getMatrixEntry), through HODLRdD_matrix on the GPU: 16^3 grid
+    1/(4 pi r) and 4096 weighted second-kind points against the dense
+    product of their own getMatrixEntry (100 eps_ACA), the GMRES solution of
+    the weighted system, print_matrix_statistics with the matvec-time line
+    (HDD_matrix.hpp:84), print_matrix_details (HDD_tree.hpp:117-132) and the
+    gmres_out.txt log lines (GMRES.hpp:260-263); a Yukawa kernel is refused."""
+    import os
+    import re
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "hodlrnd_amd", "examples", "build", "user_kernels")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    say("case cpp_api_user_kernels")
+    r = subprocess.run([exe, "full", "16"], capture_output=True, text=True, timeout=240, cwd=str(tmp_path))
+    say(r.stdout[-1500:])
+    assert r.returncode == 0, r.stderr
+    o = r.stdout
+    g = float(re.search(r"green_3d matvec vs dense getMatrixEntry product (\S+)", o).group(1))
+    w = float(re.search(r"weighted_second_kind matvec vs dense getMatrixEntry product (\S+)", o).group(1))
+    sol = float(re.search(r"weighted_second_kind gmres solution error (\S+)", o).group(1))
+    assert g < 1e-8 and w < 1e-8 and sol < 1e-8
+    assert "yukawa_3d refused code=-1" in o
+    assert o.count("Matrix-Vector time (in s) : ") == 2
+    assert "Tree Depth:" in o and "Interaction List -" in o and "Bounding box : [-1,0]x[-1,0]x[-1,0]" in o
+    log = (tmp_path / "gmres_out.txt").read_text()
+    assert "Reached Solution before Max_Iterations" in log and "Mat-Vec Time :" in log
+
+
 @pytest.mark.parametrize("nd,n,nmax", [(3, 6000, 64), (2, 5000, 32), (2, 5000, 128)])
 def test_matvec_kernel_forms_agree(nd, n, nmax):
     """Vector-ALU and matrix-core (MFMA) forms of both matvec phases give the

@@ -199,3 +199,37 @@ def test_operator_file_rejects_bad_files(hctx, tmp_path):
     trunc.write_bytes(b"HODLROP1\x01\x00")
     assert H.lib.hodlr_op_load(hctx.h, str(trunc).encode(), 3, C.byref(h)) == H.E_ARG
     assert H.lib.hodlr_op_load(hctx.h, str(tmp_path / "missing.op").encode(), 3, C.byref(h)) == H.E_ARG
+
+
+def test_reference_shaped_user_kernels_identified(tmp_path):
+    """Kernel classes in the reference's shape (user_kernel.hpp:10-179: Eigen
+    vector names, Kernel_Fun, getMatrixEntry, no device description) compile
+    unchanged against include/hodlrnd/HODLRnD.hpp with g++; their entries are
+    identified from getMatrixEntry (a uniform 1/(4 pi r), per-column weights
+    w_j / r), a Yukawa kernel exp(-r)/r is refused with a message, and the
+    generic iterSolver<Op> (GMRES.hpp:31-303) solves a dense host operator
+    (restart 100 and 5) and exits at once on A = I with x0 = b."""
+    import math
+    import re
+    out = tmp_path / "user_kernels"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "hodlrnd_amd", "examples", "user_kernels.cpp"), "-o", str(out),
+                        "-L" + os.path.join(ROOT, "hodlrnd_amd"), "-lhodlrnd",
+                        "-Wl,-rpath," + os.path.join(ROOT, "hodlrnd_amd")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([str(out), "identify"], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    o = r.stdout
+    m = re.search(r"identified green_3d family=inv_r scale=(\S+) diag=(\S+) weights=0", o)
+    assert m and abs(float(m.group(1)) - 1 / (4 * math.pi)) < 1e-16 and float(m.group(2)) == 0.0
+    assert "identified weighted_second_kind family=inv_r scale=1 diag=1 weights=700" in o
+    assert re.search(r"refused yukawa_3d code=-1: kernel entries are not", o)
+    for rs in (100, 5):
+        g = re.search(r"generic gmres restart=%d iterations=(\d+) resid=(\S+) error=(\S+)" % rs, o)
+        assert g and 0 < int(g.group(1)) < 343 and float(g.group(2)) < 1e-12 and float(g.group(3)) < 1e-10
+    assert "generic gmres identity iterations=0" in o
+    log = (tmp_path / "gmres_generic_r5.txt").read_text()
+    assert log.startswith("Restarted GMRES Parameters") and "Iter[6]" in log
+    assert "Reached Solution before Max_Iterations" in log and "Mat-Vec Time :" in log
+    assert "Exited before any Progress" in (tmp_path / "gmres_generic_exact.txt").read_text()
