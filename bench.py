@@ -433,7 +433,7 @@ def bench_batch(args, H):
         },
         "mfma": mfma_counters(variants, K),
         "cpu_baseline": cpu,
-        "cpu_baseline_1t": cpu_1t,
+        "cpu_baseline_omp": cpu_omp,
     }
     return out
 
@@ -949,8 +949,8 @@ def main():
                          "skeleton: the grid100 solve with on-the-fly skeleton blocks")
     ap.add_argument("--sys-size", type=int, default=-1,
                     help="skeleton: blocks with more columns are skeletons (default 2 Nmax)")
-    ap.add_argument("--requests", type=int, default=1000, help="serve: requests in the stream")
-    ap.add_argument("--sub-requests", type=int, default=400,
+    ap.add_argument("--requests", type=int, default=100, help="serve: requests in the stream")
+    ap.add_argument("--sub-requests", type=int, default=100,
                     help="serve: requests of the second stream, offered at 0.8 x the served rate (0: none)")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second (0: all at once)")
     ap.add_argument("--slots", type=int, default=32, help="serve: solve columns per operator engine")
@@ -1165,7 +1165,7 @@ def main():
         del db, dx
         import subprocess
         legs = (("batched", ["--workload", "batch256", "--steps", "3", "--warmup", "1", "--cpu-budget", "10"]),
-                ("serving", ["--workload", "serve", "--requests", "1000", "--sub-requests", "400"]),
+                ("serving", ["--workload", "serve", "--requests", "100", "--sub-requests", "100"]),
                 ("config1", ["--workload", "cfg1", "--steps", "20", "--warmup", "2"]),
                 ("config3", ["--workload", "cfg3", "--steps", "10", "--warmup", "2"]),
                 ("skeleton", ["--workload", "skeleton", "--steps", "5", "--warmup", "1"]),

@@ -47,6 +47,9 @@ constexpr int64_t kChunkRows = 2048;
 // waits until it serves a quarter of the engine's solving columns, or until
 // the oldest waiting column has waited this many ticks (or nothing else runs)
 constexpr int kRefineWait = 4;
+// Krylov vectors reserved in an engine's admission (they are allocated as
+// the columns' cycles grow; the mixed solves here take 5-15 inner steps)
+constexpr int kBasisReserve = 8;
 
 void gen_rot(double dx, double dy, double &cs, double &sn) {  // GMRES.hpp:83-110
     if (dy == 0.0) {
@@ -119,11 +122,46 @@ struct Engine {
     double tol, floor;
     int max_outer;
 
+    // device bytes an engine of K slots takes at creation (its blocks and
+    // the operator's K-column matvec work buffers not allocated yet), plus
+    // room for `basis` Krylov vectors
+    static double estimate_bytes(const hodlr_op &o, int K, int basis) {
+        const double nk = (double)o.N * K;
+        double b = 8.0 * nk * 6 + 4.0 * nk * 2 + 8.0 * (double)std::max<int64_t>(o.N, 1) + 8.0 * nk * basis;
+        size_t sz[5];
+        if (o.cap_rhs32 < K && (o.prec_flags & HODLR_PREC_F32)) {
+            o.matvec_work_sizes(HODLR_PREC_F32, K, sz);
+            b += (double)(sz[0] + sz[1]);
+        }
+        if (o.cap_rhs64 < K) {
+            o.matvec_work_sizes(HODLR_PREC_F64, K, sz);
+            b += (double)(sz[0] + sz[1]);
+        }
+        return b;
+    }
+
     Engine(hodlr_op &o, int K_, int m_, double tol_, double floor_, int max_outer_)
         : op(o), K(K_), m(m_), n(o.N), tol(tol_), floor(floor_), max_outer(max_outer_) {
         // non-blocking: the engine's work is ordered on its own stream only,
         // never behind null-stream calls made by builders or callers
         HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            init();
+        } catch (...) {  // a failed allocation must not leak the stream or pinned memory
+            release_handles();
+            throw;
+        }
+    }
+    void release_handles() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+            s = nullptr;
+        }
+        if (hbuf) (void)hipHostFree(hbuf);
+        hbuf = nullptr;
+    }
+    void init() {
         std::vector<Chunk> ch;
         for (int64_t r0 = 0; r0 < n; r0 += kChunkRows) {
             Chunk c;
@@ -162,17 +200,13 @@ struct Engine {
         dptr.alloc(sizeof(pt));
         HIP_CHECK(hipMemcpy(dptr.p, pt, sizeof(pt), hipMemcpyHostToDevice));
         cols.resize(K);
-        bytes = 8.0 * nk * 6 + 4.0 * nk * 2 + 8.0 * std::max<int64_t>(n, 1);
+        // the blocks and the operator's K-column work buffers (t, partial
+        // sums), which exist while the operator is resident
+        bytes = estimate_bytes(op, K, 0);
         op.ensure_work(HODLR_PREC_F32, K);
         op.ensure_work(HODLR_PREC_F64, K);
     }
-    ~Engine() {
-        if (s) {
-            (void)hipStreamSynchronize(s);
-            (void)hipStreamDestroy(s);
-        }
-        if (hbuf) (void)hipHostFree(hbuf);
-    }
+    ~Engine() { release_handles(); }
 
     double *basis(int i) {
         if (!V[i].p) {
@@ -502,6 +536,7 @@ struct OpEntry {
     double last_use = 0;
     int refs = 0;             // requests queued or running on it
     bool in_tick = false;     // the scheduler is running its engine outside the lock
+    bool engine_deferred = false;  // its engine waits for memory that running requests will free
     std::deque<std::pair<std::shared_ptr<Request>, int>> waiting;  // columns without a slot
     std::string error;
 };
@@ -538,6 +573,8 @@ struct hodlr_server {
     void complete_column(hodlr::OpEntry &e, int c, std::vector<double> &xt);
     void fail_request(hodlr::OpEntry &e, const std::shared_ptr<hodlr::Request> &r, int code, const std::string &msg);
     void fail_columns(hodlr::OpEntry &e);
+    void fail_waiting(hodlr::OpEntry &e, int code, const std::string &msg);
+    void retry_deferred();
 };
 
 using hodlr::Column;
@@ -714,8 +751,23 @@ void hodlr_server::fail_request(OpEntry &e, const std::shared_ptr<Request> &r, i
             }
         }
     last_error = msg;
+    retry_deferred();
     cv_done.notify_all();
     cv_mem.notify_all();
+}
+
+// every request of an operator still waiting for a slot fails (lock held)
+void hodlr_server::fail_waiting(OpEntry &e, int code, const std::string &msg) {
+    std::vector<std::shared_ptr<Request>> rs;
+    for (auto &w : e.waiting)
+        if (std::find(rs.begin(), rs.end(), w.first) == rs.end()) rs.push_back(w.first);
+    for (auto &r : rs) fail_request(e, r, code, msg);
+}
+
+// a request ended: engines deferred for memory may fit now (lock held)
+void hodlr_server::retry_deferred() {
+    for (auto &kv : ops) kv.second->engine_deferred = false;
+    cv_work.notify_all();
 }
 
 // requests of the columns that broke down in this tick's phases (lock held)
@@ -754,6 +806,7 @@ void hodlr_server::complete_column(OpEntry &e, int c, std::vector<double> &xt) {
         st.requests_done++;
         e.refs--;
         e.last_use = now();
+        retry_deferred();
         cv_done.notify_all();
         cv_mem.notify_all();
     }
@@ -775,7 +828,7 @@ void hodlr_server::sched_loop() {
                 need_engine.clear();
                 for (auto &kv : ops) {
                     OpEntry &e = *kv.second;
-                    if (e.state == 2 && !e.waiting.empty() && !e.engine) need_engine.push_back(&e);
+                    if (e.state == 2 && !e.waiting.empty() && !e.engine && !e.engine_deferred) need_engine.push_back(&e);
                 }
                 if (!need_engine.empty()) break;  // engines first, then admission
                 for (auto &kv : ops) {
@@ -803,8 +856,39 @@ void hodlr_server::sched_loop() {
         }
         try {
             for (OpEntry *e : need_engine) {  // engines are created outside the lock
-                auto eng = std::make_unique<Engine>(*e->op, cfg.slots, cfg.restart, cfg.tol, cfg.inner_floor,
-                                                    cfg.max_outer);
+                {
+                    // an engine's memory is admitted like an operator's:
+                    // idle operators are evicted (least recently used
+                    // first); when running requests hold the rest, the
+                    // engine waits until one of them completes
+                    std::unique_lock<std::mutex> lk(mu);
+                    const double need = Engine::estimate_bytes(*e->op, cfg.slots, hodlr::kBasisReserve);
+                    while (used + need > st.budget_bytes && evict_one(e->id)) {
+                    }
+                    if (used + need > st.budget_bytes) {
+                        bool running = false;
+                        for (auto &kv : ops)
+                            if (kv.first != e->id && kv.second->engine && kv.second->engine->busy()) running = true;
+                        if (running) {
+                            e->engine_deferred = true;
+                            continue;
+                        }
+                        // nothing will release memory: try (the slab keeps
+                        // room beyond the budget)
+                    }
+                }
+                std::unique_ptr<Engine> eng;
+                try {
+                    eng = std::make_unique<Engine>(*e->op, cfg.slots, cfg.restart, cfg.tol, cfg.inner_floor,
+                                                   cfg.max_outer);
+                } catch (const hodlr::ArgError &ex) {
+                    // no device memory for this operator's engine: its
+                    // waiting requests fail (never retried in a loop)
+                    std::unique_lock<std::mutex> lk(mu);
+                    fail_waiting(*e, ex.code, "no device memory for the solve engine of operator " +
+                                                  std::to_string(e->id) + ": " + ex.msg);
+                    continue;
+                }
                 std::unique_lock<std::mutex> lk(mu);
                 e->engine_bytes = eng->bytes;
                 used += eng->bytes;
@@ -843,14 +927,27 @@ void hodlr_server::sched_loop() {
                 std::unique_lock<std::mutex> lk(mu);
                 for (OpEntry *e : run) fail_columns(*e);
             }
-            // phase B (Arnoldi step) and C (next vectors / updates)
-            for (OpEntry *e : run) e->engine->launch_b();
+            // phase B (Arnoldi step) and C (next vectors / updates).  The
+            // phases that allocate basis vectors fail only the requests of
+            // an engine whose allocation failed
+            auto guarded = [&](OpEntry *e, void (Engine::*phase)()) {
+                try {
+                    (e->engine.get()->*phase)();
+                } catch (const hodlr::ArgError &ex) {
+                    std::unique_lock<std::mutex> lk(mu);
+                    std::vector<std::shared_ptr<Request>> inflight;
+                    for (auto &C : e->engine->cols)
+                        if (C.phase) inflight.push_back(C.req);
+                    for (auto &r : inflight) fail_request(*e, r, ex.code, ex.msg);
+                }
+            };
+            for (OpEntry *e : run) guarded(e, &Engine::launch_b);
             for (OpEntry *e : run) e->engine->finish_b();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 for (OpEntry *e : run) fail_columns(*e);
             }
-            for (OpEntry *e : run) e->engine->launch_c();
+            for (OpEntry *e : run) guarded(e, &Engine::launch_c);
             std::unique_lock<std::mutex> lk(mu);
             st.ticks++;
             st.tick_seconds += now() - tick_t0;

@@ -12,6 +12,9 @@
 #define HODLR_NMAX 64
 #define HODLR_DPRIME 0
 #define HODLR_EPS_ACA 1e-10
+// every factor stored: the accuracy bars below are those of the stored form
+// (the skeleton form, SYS_SIZE = 2 Nmax, is covered by its own tests)
+#define HODLR_SYS_SIZE -1
 #include <cstring>
 
 #include "hodlrnd/HODLRnD.hpp"

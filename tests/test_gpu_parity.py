@@ -211,7 +211,8 @@ def test_cpp_api_reference_driver():
 def test_cpp_api_reference_shaped_kernels(tmp_path):
     """hodlrnd_amd/examples/user_kernels.cpp: kernel classes in the
     reference's shape with no device description (their entries identified
-    from getMatrixEntry), through HODLRdD_matrix on the GPU: 16^3 grid
+    from getMatrixEntry), through HODLRdD_matrix on the GPU (every factor
+    stored, HODLR_SYS_SIZE -1): 16^3 grid
     1/(4 pi r) and 4096 weighted second-kind points against the dense
     product of their own getMatrixEntry (100 eps_ACA), the GMRES solution of
     the weighted system, print_matrix_statistics with the matvec-time line
