@@ -231,4 +231,20 @@ struct Chunk {
     int32_t rows, prob;
 };
 
+// state of one GMRES solve (or one batch instance) kept on the device
+// between Arnoldi steps; the host reads it to learn when to stop
+struct GmresDevState {
+    double normb;    // ||b||, 1 when b = 0 (GMRES.hpp:192-193)
+    double tol;
+    double resid;    // relative residual estimate after the last step
+    double scale;    // scale of the next basis vector (0: none)
+    int32_t active;  // 1 while the solve goes on
+    int32_t conv;    // step of this cycle at which it ended; -1 running or out of budget at a
+                     // cycle start; -2 non-finite Arnoldi norm; -3 residual test passed at a cycle start
+    int32_t steps;   // Arnoldi steps taken (GMRES.hpp's j - 1)
+    int32_t budget;  // max_iter
+    int32_t iters;   // the iteration count reported once the solve has ended (GMRES.hpp:212,267,294,302)
+    int32_t converged;
+};
+
 }  // namespace hodlr

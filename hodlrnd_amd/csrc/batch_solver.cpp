@@ -11,13 +11,18 @@
 // of the batch).  Converged instances stop changing: their next basis
 // vectors are zero and their coefficients are dropped.
 //
-// The device work of Arnoldi step i -- the matvec, both CGS2 passes, the
-// norm of the new vector and the copies of the Hessenberg column to pinned
-// host memory -- depends only on i (basis vectors, work buffers and the
-// pinned staging are fixed per workspace), so it is captured once into a
-// hipGraph per (operator precision, i) and replayed as one launch followed
-// by one wait; the host then applies the Givens rotations.  A graph is
-// recaptured when the operator's work buffers or kernel forms change.
+// The Givens rotations, the residual tests and the back-solves run on the
+// device, one thread per instance (blas.hip); the host reads two counts per
+// step (instances still running, breakdowns).  The device work of Arnoldi
+// step i -- the matvec, both CGS2 passes, the norm of the new vector, the
+// rotations of every instance, the next basis vector and the copy of the
+// counts to pinned memory -- depends only on i (basis vectors, work buffers
+// and the pinned staging are fixed per workspace), so it is captured once
+// into a hipGraph per (operator precision, i) and replayed as one launch
+// followed by one wait.  A graph is recaptured when the operator's work
+// buffers or kernel forms change.  An instance that ends at a step keeps
+// its basis vectors until the end of the cycle, where every instance that
+// ran in it is updated by one product x += V y.
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -33,27 +38,6 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-void gen_rot(double dx, double dy, double &cs, double &sn) {
-    if (dy == 0.0) {
-        cs = 1.0;
-        sn = 0.0;
-    } else if (std::fabs(dy) > std::fabs(dx)) {
-        double temp = dx / dy;
-        sn = 1.0 / std::sqrt(1.0 + temp * temp);
-        cs = temp * sn;
-    } else {
-        double temp = dy / dx;
-        cs = 1.0 / std::sqrt(1.0 + temp * temp);
-        sn = temp * cs;
-    }
-}
-
-void app_rot(double &dx, double &dy, double cs, double sn) {
-    double temp = cs * dx + sn * dy;
-    dy = (-sn) * dx + cs * dy;
-    dx = temp;
-}
-
 constexpr int64_t kChunkRows = 2048;
 
 }  // namespace
@@ -65,7 +49,12 @@ struct BatchWork {
     int P = 0, nchunk = 0;
     std::vector<DevBuf> V;       // basis vectors, allocated on first use
     std::vector<double *> vptr;
-    DevBuf vtab, w, r, d, rr, w64, x32, y32, partial, coef, chunks, pchunk, tab1, tabw;
+    DevBuf vtab, w, r, d, rr, w64, x32, y32, partial, coef, chunks, pchunk, tab1, tabw, tabr;
+    // device GMRES state of every instance: GmresDevState, the Hessenberg
+    // matrices ((restart+1) x restart each), cs / sn / sv / y ((restart+1)
+    // each), next-vector scales, ||b||^2, ||r||^2, tolerances, budgets and
+    // the two step counts
+    DevBuf gst, gH, gvec, gscale, gnb2, gr2, gtol, gbud, gcnt;
     double *hbuf = nullptr;      // pinned staging
     size_t hcap = 0;
     // captured Arnoldi steps [operator precision: 0 FP64, 1 FP32][step] and
@@ -107,8 +96,20 @@ struct BatchWork {
         coef.alloc(sizeof(double) * (size_t)P * (restart + 1) * K * 3);
         tab1.alloc(sizeof(double *));
         tabw.alloc(sizeof(double *));
-        double *wp = w.as<double>();
+        tabr.alloc(sizeof(double *));
+        double *wp = w.as<double>(), *rp = r.as<double>();
         HIP_CHECK(hipMemcpy(tabw.p, &wp, sizeof(double *), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(tabr.p, &rp, sizeof(double *), hipMemcpyHostToDevice));
+        const size_t NI = (size_t)P * K, ld = (size_t)restart + 1;
+        gst.alloc(sizeof(GmresDevState) * NI);
+        gH.alloc(sizeof(double) * NI * ld * restart);
+        gvec.alloc(sizeof(double) * NI * ld * 4);
+        gscale.alloc(sizeof(double) * NI);
+        gnb2.alloc(sizeof(double) * NI);
+        gr2.alloc(sizeof(double) * NI);
+        gtol.alloc(sizeof(double) * NI);
+        gbud.alloc(sizeof(int32_t) * NI);
+        gcnt.alloc(sizeof(int32_t) * 4);
         hcap = (size_t)P * (restart + 1) * K * 3 + 16;
         HIP_CHECK(hipHostMalloc(&hbuf, sizeof(double) * hcap));
         for (int k = 0; k < 2; k++) {
@@ -153,15 +154,6 @@ void inst_dot(BatchWork &bw, const double *a, const double *b, std::vector<doubl
     out.assign(bw.hbuf + off, bw.hbuf + off + ni);
 }
 
-struct Inst {
-    std::vector<double> H;  // (m+1) x m column-major
-    std::vector<double> sv, cs, sn;
-    double normb = 1.0, beta = 0.0, resid = 0.0;
-    bool active = false;
-    bool converged = false;
-    int iterations = 0;
-};
-
 }  // namespace
 
 // Batched GMRES: A applies the operator to n x K device vectors; gkey names
@@ -174,19 +166,37 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
                  std::vector<int> &iters, std::vector<int> &conv, std::vector<double> &resid, int &matvecs,
                  int gkey, uint64_t gen) {
     const int m = bw.restart, K = bw.K, P = bw.P;
-    const int NI = P * K;
+    const int NI = P * K, ld = m + 1;
     const int64_t nk = bw.n * K;
-    std::vector<Inst> in(NI);
-    for (auto &q : in) {
-        q.H.assign((size_t)(m + 1) * m, 0.0);
-        q.sv.assign(m + 1, 0.0);
-        q.cs.assign(m + 1, 0.0);
-        q.sn.assign(m + 1, 0.0);
-    }
-    auto Hc = [&](Inst &q, int row, int c) -> double & { return q.H[(size_t)c * (m + 1) + row]; };
-    std::vector<double> nb, bt;
-    inst_dot(bw, b, b, nb, s);
     double *w = bw.w.as<double>(), *r = bw.r.as<double>();
+    double *cf = bw.coef.as<double>();  // [0, cfs) h1, [cfs, 2 cfs) h2, then |w|^2 or the update coefficients
+    const size_t cfs = (size_t)P * (m + 1) * K;
+    GmresDevState *st = bw.gst.as<GmresDevState>();
+    double *H = bw.gH.as<double>();
+    double *cs = bw.gvec.as<double>(), *sn = cs + (size_t)NI * ld, *sv = sn + (size_t)NI * ld,
+           *y = sv + (size_t)NI * ld;
+    double *scale = bw.gscale.as<double>();
+    int32_t *cnt = bw.gcnt.as<int32_t>();
+    int32_t *hcnt = reinterpret_cast<int32_t *>(bw.hbuf + bw.hcap - 4);  // pinned: running, breakdowns
+    const Chunk *ck = bw.chunks.as<Chunk>();
+    auto inst_dot_dev = [&](const double *const *tab, const double *v, double *out) {  // per-instance v . v
+        HIP_CHECK(launch_bmdot(tab, 1, v, ck, bw.nchunk, bw.partial.as<double>(), bw.pchunk.as<int32_t>(), P, K,
+                               out, s));
+    };
+    auto read_counts = [&] {
+        HIP_CHECK(hipMemcpyAsync(hcnt, cnt, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (hcnt[1]) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
+        return hcnt[0];
+    };
+    // inputs (the stream is idle: nothing queued reads them)
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::vector<int32_t> bud(budget.begin(), budget.end());
+    HIP_CHECK(hipMemcpy(bw.tab1.p, &b, sizeof(double *), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(bw.gtol.p, tol.data(), sizeof(double) * NI, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(bw.gbud.p, bud.data(), sizeof(int32_t) * NI, hipMemcpyHostToDevice));
+    inst_dot_dev(bw.tab1.as<const double *>(), b, bw.gnb2.as<double>());
+    HIP_CHECK(launch_bgmres_init(NI, bw.gnb2.as<double>(), bw.gtol.as<double>(), bw.gbud.as<int32_t>(), st, s));
     if (x0_zero) {
         HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * nk, hipMemcpyDeviceToDevice, s));
     } else {
@@ -194,85 +204,30 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
         matvecs++;
         HIP_CHECK(launch_sub(b, w, r, nk, s));
     }
-    inst_dot(bw, r, r, bt, s);
-    int nactive = 0;
-    for (int i = 0; i < NI; i++) {
-        Inst &q = in[i];
-        q.normb = std::sqrt(nb[i]) == 0.0 ? 1.0 : std::sqrt(nb[i]);
-        q.beta = std::sqrt(bt[i]);
-        q.resid = q.beta / q.normb;
-        q.converged = q.resid <= tol[i];  // GMRES.hpp:198-213
-        q.active = !q.converged && budget[i] > 0;
-        q.iterations = 0;
-        nactive += q.active;
-    }
-    int max_iter = 0;
-    for (int i = 0; i < NI; i++) max_iter = std::max(max_iter, budget[i]);
-    double *cf = bw.coef.as<double>();  // [0, P*(m+1)*K) h1; then h2; then scratch
-    const size_t cfs = (size_t)P * (m + 1) * K;
-    std::vector<double> hc(cfs);
-    // x += V_0..k y for the instances in `sel` (others get zero coefficients)
-    auto update = [&](const std::vector<int> &ks) {
-        std::fill(hc.begin(), hc.end(), 0.0);
-        int kmax = -1;
-        for (int i = 0; i < NI; i++) {
-            const int k = ks[i];
-            if (k < 0) continue;
-            Inst &q = in[i];
-            std::vector<double> y(q.sv.begin(), q.sv.begin() + k + 1);
-            for (int a = k; a >= 0; a--) {  // GMRES.hpp:136-141
-                y[a] /= Hc(q, a, a);
-                for (int c = a - 1; c >= 0; c--) y[c] -= Hc(q, c, a) * y[a];
-            }
-            const int p = i / K, c = i % K;
-            for (int a = 0; a <= k; a++) hc[((size_t)p * (m + 1) + a) * K + c] = y[a];
-            kmax = std::max(kmax, k);
-        }
-        if (kmax < 0) return;
-        // coefficients are laid out (p, v, c) with nv = kmax + 1
-        std::vector<double> packed((size_t)P * (kmax + 1) * K);
-        for (int p = 0; p < P; p++)
-            for (int a = 0; a <= kmax; a++)
-                for (int c = 0; c < K; c++)
-                    packed[((size_t)p * (kmax + 1) + a) * K + c] = hc[((size_t)p * (m + 1) + a) * K + c];
-        std::memcpy(bw.hbuf, packed.data(), sizeof(double) * packed.size());
-        HIP_CHECK(hipMemcpyAsync(cf + 2 * cfs, bw.hbuf, sizeof(double) * packed.size(), hipMemcpyHostToDevice, s));
-        HIP_CHECK(launch_bgemv(bw.table(), kmax + 1, cf + 2 * cfs, 1.0, x, bw.chunks.as<Chunk>(), bw.nchunk, K, s));
-        HIP_CHECK(hipStreamSynchronize(s));
-    };
-    std::vector<double> scale(NI);
-    int j = 1;
-    while (nactive > 0 && j <= max_iter) {
-        for (int i = 0; i < NI; i++) scale[i] = in[i].active ? 1.0 / in[i].beta : 0.0;
-        std::memcpy(bw.hbuf, scale.data(), sizeof(double) * NI);
-        HIP_CHECK(hipMemcpyAsync(cf + 2 * cfs, bw.hbuf, sizeof(double) * NI, hipMemcpyHostToDevice, s));
-        HIP_CHECK(launch_bscale(r, cf + 2 * cfs, bw.basis(0), bw.chunks.as<Chunk>(), bw.nchunk, K, s));
-        for (auto &q : in) {
-            std::fill(q.sv.begin(), q.sv.end(), 0.0);
-            q.sv[0] = q.beta;
-        }
+    inst_dot_dev(bw.tabr.as<const double *>(), r, bw.gr2.as<double>());
+    HIP_CHECK(launch_bgmres_cycle(NI, bw.gr2.as<double>(), ld, 1, sv, st, scale, cnt, s));
+    int nactive = read_counts();
+    while (nactive > 0) {
+        HIP_CHECK(launch_bscale(r, scale, bw.basis(0), ck, bw.nchunk, K, s));  // v_0 = r / beta
         int i = 0;
-        for (; i < m && j <= max_iter && nactive > 0; i++, j++) {
+        for (; i < m; i++) {
             const int nv = i + 1;
-            const size_t nh = (size_t)P * nv * K;
             bw.basis(i);
             bw.basis(i + 1);  // both allocated (and tabled) before any capture
-            // step i's device work: new vector, CGS2, its norm, and the copies
-            // of h1, h2 and |w|^2 to hbuf[0, nh), [nh, 2 nh), [2 cfs, 2 cfs + NI)
             auto step = [&] {
                 A(bw.vptr[i], w);
-                HIP_CHECK(launch_bmdot(bw.table(), nv, w, bw.chunks.as<Chunk>(), bw.nchunk, bw.partial.as<double>(),
+                HIP_CHECK(launch_bmdot(bw.table(), nv, w, ck, bw.nchunk, bw.partial.as<double>(),
                                        bw.pchunk.as<int32_t>(), P, K, cf, s));
-                HIP_CHECK(launch_bgemv(bw.table(), nv, cf, -1.0, w, bw.chunks.as<Chunk>(), bw.nchunk, K, s));
-                HIP_CHECK(launch_bmdot(bw.table(), nv, w, bw.chunks.as<Chunk>(), bw.nchunk, bw.partial.as<double>(),
+                HIP_CHECK(launch_bgemv(bw.table(), nv, cf, -1.0, w, ck, bw.nchunk, K, s));
+                HIP_CHECK(launch_bmdot(bw.table(), nv, w, ck, bw.nchunk, bw.partial.as<double>(),
                                        bw.pchunk.as<int32_t>(), P, K, cf + cfs, s));
-                HIP_CHECK(launch_bgemv(bw.table(), nv, cf + cfs, -1.0, w, bw.chunks.as<Chunk>(), bw.nchunk, K, s));
-                HIP_CHECK(launch_bmdot(bw.tabw.as<const double *>(), 1, w, bw.chunks.as<Chunk>(), bw.nchunk,
-                                       bw.partial.as<double>(), bw.pchunk.as<int32_t>(), P, K, cf + 2 * cfs, s));
-                HIP_CHECK(hipMemcpyAsync(bw.hbuf, cf, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
-                HIP_CHECK(hipMemcpyAsync(bw.hbuf + nh, cf + cfs, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
-                HIP_CHECK(hipMemcpyAsync(bw.hbuf + 2 * cfs, cf + 2 * cfs, sizeof(double) * NI, hipMemcpyDeviceToHost,
-                                         s));
+                HIP_CHECK(launch_bgemv(bw.table(), nv, cf + cfs, -1.0, w, ck, bw.nchunk, K, s));
+                HIP_CHECK(launch_bmdot(bw.tabw.as<const double *>(), 1, w, ck, bw.nchunk, bw.partial.as<double>(),
+                                       bw.pchunk.as<int32_t>(), P, K, cf + 2 * cfs, s));
+                HIP_CHECK(launch_bgmres_step(NI, K, cf, cf + cfs, cf + 2 * cfs, i, ld, H, cs, sn, sv, st, scale, cnt,
+                                             s));
+                HIP_CHECK(launch_bscale(w, scale, bw.vptr[i + 1], ck, bw.nchunk, K, s));
+                HIP_CHECK(hipMemcpyAsync(hcnt, cnt, sizeof(int32_t) * 2, hipMemcpyDeviceToHost, s));
             };
             if (gkey >= 0) {
                 hipGraphExec_t &ex = bw.gexec[gkey][i];
@@ -300,83 +255,33 @@ void gmres_batch(const std::function<void(const double *, double *)> &A, BatchWo
             }
             matvecs++;
             HIP_CHECK(hipStreamSynchronize(s));
-            std::vector<double> hn2(bw.hbuf + 2 * cfs, bw.hbuf + 2 * cfs + NI);
-            std::vector<double> h1(bw.hbuf, bw.hbuf + nh), h2(bw.hbuf + nh, bw.hbuf + 2 * nh);
-            std::vector<int> ks(NI, -1);
-            int nconv = 0;
-            for (int a = 0; a < NI; a++) {
-                Inst &q = in[a];
-                const int p = a / K, c = a % K;
-                if (!q.active) {
-                    scale[a] = 0.0;
-                    continue;
-                }
-                for (int k = 0; k <= i; k++)
-                    Hc(q, k, i) = h1[((size_t)p * nv + k) * K + c] + h2[((size_t)p * nv + k) * K + c];
-                const double hn = std::sqrt(hn2[a]);
-                Hc(q, i + 1, i) = hn;
-                for (int k = 0; k < i; k++) app_rot(Hc(q, k, i), Hc(q, k + 1, i), q.cs[k], q.sn[k]);
-                gen_rot(Hc(q, i, i), Hc(q, i + 1, i), q.cs[i], q.sn[i]);
-                app_rot(Hc(q, i, i), Hc(q, i + 1, i), q.cs[i], q.sn[i]);
-                app_rot(q.sv[i], q.sv[i + 1], q.cs[i], q.sn[i]);
-                q.resid = std::fabs(q.sv[i + 1]) / q.normb;
-                if (!std::isfinite(hn)) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
-                if (q.resid < tol[a] || j >= budget[a]) {  // converged, or its budget ends here
-                    ks[a] = i;
-                    q.iterations = j;
-                    q.converged = q.resid < tol[a];
-                    nconv++;
-                    scale[a] = 0.0;
-                } else {
-                    scale[a] = hn != 0.0 ? 1.0 / hn : 0.0;
-                }
-            }
-            std::memcpy(bw.hbuf, scale.data(), sizeof(double) * NI);
-            HIP_CHECK(hipMemcpyAsync(cf + 2 * cfs, bw.hbuf, sizeof(double) * NI, hipMemcpyHostToDevice, s));
-            HIP_CHECK(launch_bscale(w, cf + 2 * cfs, bw.basis(i + 1), bw.chunks.as<Chunk>(), bw.nchunk, K, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            if (nconv) {
-                update(ks);
-                for (int a = 0; a < NI; a++)
-                    if (ks[a] >= 0) {
-                        in[a].active = false;
-                        nactive--;
-                    }
-            }
+            if (hcnt[1]) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
+            nactive = hcnt[0];
+            if (nactive == 0) break;
         }
+        // end of the cycle (GMRES.hpp:271 uses m - 1 columns; an instance
+        // that ended earlier in it uses the columns it built)
+        const int kmax = i < m ? i : m - 1;
+        HIP_CHECK(launch_bgmres_backsolve(NI, K, kmax, ld, H, sv, st, y, cf + 2 * cfs, s));
+        HIP_CHECK(launch_bgemv(bw.table(), kmax + 1, cf + 2 * cfs, 1.0, x, ck, bw.nchunk, K, s));
         if (nactive == 0) break;
-        // end of a cycle (or of the iteration budget): update with the columns
-        // built in it (GMRES.hpp:271 uses m - 1; fewer exist when the budget
-        // ends mid-cycle)
-        std::vector<int> ks(NI, -1);
-        for (int a = 0; a < NI; a++)
-            if (in[a].active) ks[a] = i - 1;
-        update(ks);
         A(x, w);
         matvecs++;
         HIP_CHECK(launch_sub(b, w, r, nk, s));
-        std::vector<double> b2;
-        inst_dot(bw, r, r, b2, s);
-        for (int a = 0; a < NI; a++) {
-            Inst &q = in[a];
-            if (!q.active) continue;
-            q.beta = std::sqrt(b2[a]);
-            q.resid = q.beta / q.normb;
-            if (q.resid < tol[a]) {
-                q.active = false;
-                q.converged = true;
-                q.iterations = j;
-                nactive--;
-            }
-        }
+        inst_dot_dev(bw.tabr.as<const double *>(), r, bw.gr2.as<double>());
+        HIP_CHECK(launch_bgmres_cycle(NI, bw.gr2.as<double>(), ld, 0, sv, st, scale, cnt, s));
+        nactive = read_counts();
     }
+    std::vector<GmresDevState> out(NI);
+    HIP_CHECK(hipStreamSynchronize(s));
+    HIP_CHECK(hipMemcpy(out.data(), st, sizeof(GmresDevState) * NI, hipMemcpyDeviceToHost));
     iters.assign(NI, 0);
     conv.assign(NI, 0);
     resid.assign(NI, 0.0);
     for (int a = 0; a < NI; a++) {
-        iters[a] = in[a].active ? std::min(j - 1, budget[a]) : in[a].iterations;
-        conv[a] = in[a].converged;
-        resid[a] = in[a].resid;
+        iters[a] = out[a].iters;
+        conv[a] = out[a].converged;
+        resid[a] = out[a].resid;
     }
 }
 

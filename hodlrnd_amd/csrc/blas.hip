@@ -407,3 +407,334 @@ hipError_t launch_get_col(const double *src, double *dst, int64_t n, int K, int 
 }
 
 }  // namespace hodlr
+
+// ---- Givens rotations and the convergence test on the device ----
+// GMRES.hpp:83-146,244-268 for one solve (gmres_*_kernel) and for every
+// instance of a batch (bgmres_*_kernel, one thread per instance).  The
+// Hessenberg matrix, rotations and rotated right-hand side stay in device
+// memory; the host reads only each step's small state record.
+namespace hodlr {
+
+namespace {
+
+// GMRES.hpp:83-110 (real rotations)
+__device__ __forceinline__ void gen_rot_d(double dx, double dy, double &cs, double &sn) {
+    if (dy == 0.0) {
+        cs = 1.0;
+        sn = 0.0;
+    } else if (fabs(dy) > fabs(dx)) {
+        const double t = dx / dy;
+        sn = 1.0 / sqrt(1.0 + t * t);
+        cs = t * sn;
+    } else {
+        const double t = dy / dx;
+        cs = 1.0 / sqrt(1.0 + t * t);
+        sn = t * cs;
+    }
+}
+
+// GMRES.hpp:112-129
+__device__ __forceinline__ void app_rot_d(double &dx, double &dy, double cs, double sn) {
+    const double t = cs * dx + sn * dy;
+    dy = (-sn) * dx + cs * dy;
+    dx = t;
+}
+
+// one Arnoldi step's column: H(0..i, i) = h[k], H(i+1, i) = hn; rotations;
+// the rotated right-hand side; the residual test.  Returns the scale of the
+// next basis vector (0 when the solve ends here or hn is 0).
+__device__ double givens_step(double *H, double *cs, double *sn, double *sv, int ld, int i, double hn,
+                              GmresDevState &st) {
+    double *col = H + (int64_t)i * ld;
+    col[i + 1] = hn;
+    for (int k = 0; k < i; k++) app_rot_d(col[k], col[k + 1], cs[k], sn[k]);
+    gen_rot_d(col[i], col[i + 1], cs[i], sn[i]);
+    app_rot_d(col[i], col[i + 1], cs[i], sn[i]);
+    app_rot_d(sv[i], sv[i + 1], cs[i], sn[i]);
+    st.resid = fabs(sv[i + 1]) / st.normb;
+    st.steps++;
+    if (!(hn == hn) || hn == INFINITY) {  // non-finite Arnoldi norm: the solve breaks down here
+        st.active = 0;
+        st.conv = -2;
+        return 0.0;
+    }
+    if (st.resid < st.tol || st.steps >= st.budget) {  // GMRES.hpp:253-254, or the iteration budget
+        st.active = 0;
+        st.conv = i;
+        return 0.0;
+    }
+    return hn != 0.0 ? 1.0 / hn : 0.0;
+}
+
+// y = H(0..k, 0..k)^-1 sv(0..k) by back substitution (GMRES.hpp:132-146)
+__device__ void back_solve(const double *H, const double *sv, int ld, int k, double *y) {
+    for (int a = 0; a <= k; a++) y[a] = sv[a];
+    for (int a = k; a >= 0; a--) {
+        y[a] /= H[(int64_t)a * ld + a];
+        for (int q = a - 1; q >= 0; q--) y[q] -= H[(int64_t)a * ld + q] * y[a];
+    }
+}
+
+}  // namespace
+
+// one solve: h1 at hd[0..i], h2 at hd[128..128+i], |w|^2 at hd[256]
+__global__ void gmres_step_kernel(double *__restrict__ H, double *__restrict__ cs, double *__restrict__ sn,
+                                  double *__restrict__ sv, const double *__restrict__ hd, int i, int ld,
+                                  GmresDevState *__restrict__ stp) {
+    if (threadIdx.x != 0) return;
+    GmresDevState st = *stp;
+    if (!st.active) {  // ended at an earlier step: this one was launched ahead and changes nothing
+        stp->scale = 0.0;
+        return;
+    }
+    double *col = H + (int64_t)i * ld;
+    for (int k = 0; k <= i; k++) col[k] = hd[k] + hd[128 + k];
+    st.scale = givens_step(H, cs, sn, sv, ld, i, sqrt(hd[256]), st);
+    *stp = st;
+}
+
+// start of a cycle: beta = ||r|| (r2 = ||r||^2), the residual test
+// (GMRES.hpp:198 `<=` before the first cycle, :284 `<` at a restart), the
+// rotated right-hand side beta e_1 and the scale of v_0 = r / beta
+__global__ void gmres_cycle_kernel(const double *__restrict__ r2, double *__restrict__ sv, int ld, int first,
+                                   GmresDevState *__restrict__ stp) {
+    if (threadIdx.x != 0) return;
+    GmresDevState st = *stp;
+    const double beta = sqrt(*r2);
+    st.resid = beta / st.normb;
+    st.conv = -1;
+    const bool done = first ? st.resid <= st.tol : st.resid < st.tol;
+    st.active = (!done && st.steps < st.budget) ? 1 : 0;
+    if (done) st.conv = -3;
+    for (int a = 0; a < ld; a++) sv[a] = 0.0;
+    sv[0] = beta;
+    st.scale = beta != 0.0 ? 1.0 / beta : 0.0;
+    *stp = st;
+}
+
+// ||b|| (b2 = ||b||^2; a zero b counts as 1, GMRES.hpp:192-193) and the
+// fresh state of a solve
+__global__ void gmres_init_kernel(const double *__restrict__ b2, double tol, int budget,
+                                  GmresDevState *__restrict__ stp) {
+    if (threadIdx.x != 0) return;
+    GmresDevState st;
+    const double nb = sqrt(*b2);
+    st.normb = nb == 0.0 ? 1.0 : nb;
+    st.tol = tol;
+    st.resid = 0.0;
+    st.scale = 0.0;
+    st.active = 1;
+    st.conv = -1;
+    st.steps = 0;
+    st.budget = budget;
+    st.iters = 0;
+    st.converged = 0;
+    *stp = st;
+}
+
+// y for the update x += V y over columns 0..k
+__global__ void gmres_backsolve_kernel(const double *__restrict__ H, const double *__restrict__ sv, int ld, int k,
+                                       double *__restrict__ y) {
+    if (threadIdx.x == 0) back_solve(H, sv, ld, k, y);
+}
+
+// dst = src * (*alpha): the next basis vector scaled by a value the device computed
+__global__ void scale_by_kernel(const double *__restrict__ src, const double *__restrict__ alpha,
+                                double *__restrict__ dst, int64_t n) {
+    const double a = *alpha;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] * a;
+}
+
+hipError_t launch_gmres_init(const double *b2, double tol, int budget, GmresDevState *st, hipStream_t s) {
+    gmres_init_kernel<<<1, 64, 0, s>>>(b2, tol, budget, st);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_cycle(const double *r2, double *sv, int ld, int first, GmresDevState *st, hipStream_t s) {
+    gmres_cycle_kernel<<<1, 64, 0, s>>>(r2, sv, ld, first, st);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_step(double *H, double *cs, double *sn, double *sv, const double *hd, int i, int ld,
+                             GmresDevState *st, hipStream_t s) {
+    if (i + 1 >= ld) return hipErrorInvalidValue;
+    gmres_step_kernel<<<1, 64, 0, s>>>(H, cs, sn, sv, hd, i, ld, st);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_backsolve(const double *H, const double *sv, int ld, int k, double *y, hipStream_t s) {
+    gmres_backsolve_kernel<<<1, 64, 0, s>>>(H, sv, ld, k, y);
+    return hipGetLastError();
+}
+hipError_t launch_scale_by(const double *src, const double *alpha, double *dst, int64_t n, hipStream_t s) {
+    scale_by_kernel<<<dim3(grid_for(n)), 256, 0, s>>>(src, alpha, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace hodlr
+
+// ---- the same for every instance of a batch (instance a = p * K + c) ----
+namespace hodlr {
+
+namespace {
+// active instances of this workgroup -> out[0], breakdowns -> out[1]
+__device__ void block_count(int act, int bad, int32_t *out) {
+    __shared__ int sa[16], sb[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        act += __shfl_down(act, o, 64);
+        bad += __shfl_down(bad, o, 64);
+    }
+    if (lane == 0) {
+        sa[wid] = act;
+        sb[wid] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ta = 0, tb = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+            ta += sa[k];
+            tb += sb[k];
+        }
+        out[0] = ta;
+        out[1] = tb;
+    }
+}
+}  // namespace
+
+__global__ __launch_bounds__(1024) void bgmres_init_kernel(int NI, const double *__restrict__ b2,
+                                                           const double *__restrict__ tol,
+                                                           const int32_t *__restrict__ budget,
+                                                           GmresDevState *__restrict__ st) {
+    for (int a = threadIdx.x; a < NI; a += blockDim.x) {
+        GmresDevState s;
+        const double nb = sqrt(b2[a]);
+        s.normb = nb == 0.0 ? 1.0 : nb;
+        s.tol = tol[a];
+        s.resid = 0.0;
+        s.scale = 0.0;
+        s.active = 1;
+        s.conv = -1;
+        s.steps = 0;
+        s.budget = budget[a];
+        s.iters = 0;
+        s.converged = 0;
+        st[a] = s;
+    }
+}
+
+// cycle start of every instance still running: beta, the residual test
+// (`<=` on the first cycle, `<` at a restart), sv = beta e_1, the scale of
+// v_0; instances that ended earlier only reset their cycle marker
+__global__ __launch_bounds__(1024) void bgmres_cycle_kernel(int NI, const double *__restrict__ r2, int ld, int first,
+                                                            double *__restrict__ sv, GmresDevState *__restrict__ st,
+                                                            double *__restrict__ scale, int32_t *__restrict__ count) {
+    int act = 0;
+    for (int a = threadIdx.x; a < NI; a += blockDim.x) {
+        GmresDevState s = st[a];
+        s.conv = -1;
+        if (s.active) {
+            const double beta = sqrt(r2[a]);
+            s.resid = beta / s.normb;
+            const bool done = first ? s.resid <= s.tol : s.resid < s.tol;
+            if (done) {  // GMRES.hpp:198-213 / 284-295: j is one past the last step at a restart
+                s.active = 0;
+                s.conv = -3;
+                s.iters = first ? 0 : s.steps + 1;
+                s.converged = 1;
+            } else if (s.steps >= s.budget) {
+                s.active = 0;
+                s.iters = s.steps;
+                s.converged = 0;
+            }
+            double *sa = sv + (int64_t)a * ld;
+            for (int k = 0; k < ld; k++) sa[k] = 0.0;
+            sa[0] = beta;
+            s.scale = (s.active && beta != 0.0) ? 1.0 / beta : 0.0;
+        } else {
+            s.scale = 0.0;
+        }
+        scale[a] = s.scale;
+        st[a] = s;
+        act += s.active;
+    }
+    block_count(act, 0, count);
+}
+
+// Arnoldi step i of every running instance: h1, h2 laid out (p, v, c) with
+// nv = i + 1, |w|^2 per instance in n2
+__global__ __launch_bounds__(1024) void bgmres_step_kernel(int NI, int K, const double *__restrict__ h1,
+                                                           const double *__restrict__ h2,
+                                                           const double *__restrict__ n2, int i, int ld,
+                                                           double *__restrict__ H, double *__restrict__ cs,
+                                                           double *__restrict__ sn, double *__restrict__ sv,
+                                                           GmresDevState *__restrict__ st,
+                                                           double *__restrict__ scale, int32_t *__restrict__ count) {
+    const int nv = i + 1;
+    int act = 0, bad = 0;
+    for (int a = threadIdx.x; a < NI; a += blockDim.x) {
+        GmresDevState s = st[a];
+        if (!s.active) {
+            scale[a] = 0.0;
+            continue;
+        }
+        const int p = a / K, c = a % K;
+        double *Ha = H + (int64_t)a * ld * (ld - 1);
+        double *col = Ha + (int64_t)i * ld;
+        for (int k = 0; k < nv; k++) {
+            const int64_t e = ((int64_t)p * nv + k) * K + c;
+            col[k] = h1[e] + h2[e];
+        }
+        s.scale = givens_step(Ha, cs + (int64_t)a * ld, sn + (int64_t)a * ld, sv + (int64_t)a * ld, ld, i,
+                              sqrt(n2[a]), s);
+        if (!s.active) {
+            s.iters = s.steps;
+            s.converged = s.conv >= 0 && s.resid < s.tol;
+            bad += s.conv == -2;
+        }
+        scale[a] = s.scale;
+        st[a] = s;
+        act += s.active;
+    }
+    block_count(act, bad, count);
+}
+
+// end of a cycle whose last step had index kmax: y of every instance that
+// ran in it (to kmax, or to the step it ended at) into the update
+// coefficients laid out (p, v, c) with nv = kmax + 1; zero elsewhere
+__global__ void bgmres_backsolve_kernel(int NI, int K, int kmax, int ld, const double *__restrict__ H,
+                                        const double *__restrict__ sv, const GmresDevState *__restrict__ st,
+                                        double *__restrict__ y, double *__restrict__ coef) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= NI) return;
+    const GmresDevState s = st[a];
+    const int k = s.active ? kmax : s.conv;
+    double *ya = y + (int64_t)a * ld;
+    if (k >= 0) back_solve(H + (int64_t)a * ld * (ld - 1), sv + (int64_t)a * ld, ld, k, ya);
+    const int p = a / K, c = a % K, nv = kmax + 1;
+    for (int v = 0; v < nv; v++) coef[((int64_t)p * nv + v) * K + c] = v <= k ? ya[v] : 0.0;
+}
+
+hipError_t launch_bgmres_init(int NI, const double *b2, const double *tol, const int32_t *budget, GmresDevState *st,
+                              hipStream_t s) {
+    bgmres_init_kernel<<<1, 1024, 0, s>>>(NI, b2, tol, budget, st);
+    return hipGetLastError();
+}
+hipError_t launch_bgmres_cycle(int NI, const double *r2, int ld, int first, double *sv, GmresDevState *st,
+                               double *scale, int32_t *count, hipStream_t s) {
+    bgmres_cycle_kernel<<<1, 1024, 0, s>>>(NI, r2, ld, first, sv, st, scale, count);
+    return hipGetLastError();
+}
+hipError_t launch_bgmres_step(int NI, int K, const double *h1, const double *h2, const double *n2, int i, int ld,
+                              double *H, double *cs, double *sn, double *sv, GmresDevState *st, double *scale,
+                              int32_t *count, hipStream_t s) {
+    if (i + 1 >= ld) return hipErrorInvalidValue;
+    bgmres_step_kernel<<<1, 1024, 0, s>>>(NI, K, h1, h2, n2, i, ld, H, cs, sn, sv, st, scale, count);
+    return hipGetLastError();
+}
+hipError_t launch_bgmres_backsolve(int NI, int K, int kmax, int ld, const double *H, const double *sv,
+                                   const GmresDevState *st, double *y, double *coef, hipStream_t s) {
+    if (kmax < 0 || kmax >= ld - 1) return hipErrorInvalidValue;
+    bgmres_backsolve_kernel<<<dim3((unsigned)((NI + 255) / 256)), 256, 0, s>>>(NI, K, kmax, ld, H, sv, st, y, coef);
+    return hipGetLastError();
+}
+
+}  // namespace hodlr

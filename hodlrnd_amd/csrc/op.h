@@ -66,6 +66,22 @@ hipError_t launch_sub(const double *a, const double *b, double *c, int64_t n, hi
 hipError_t launch_add(double *a, const double *b, int64_t n, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, int64_t n, hipStream_t s);
 hipError_t launch_to_f64(const float *a, double *b, int64_t n, hipStream_t s);
+// Givens rotations and the residual test on the device (blas.hip)
+hipError_t launch_gmres_init(const double *b2, double tol, int budget, GmresDevState *st, hipStream_t s);
+hipError_t launch_gmres_cycle(const double *r2, double *sv, int ld, int first, GmresDevState *st, hipStream_t s);
+hipError_t launch_gmres_step(double *H, double *cs, double *sn, double *sv, const double *hd, int i, int ld,
+                             GmresDevState *st, hipStream_t s);
+hipError_t launch_gmres_backsolve(const double *H, const double *sv, int ld, int k, double *y, hipStream_t s);
+hipError_t launch_scale_by(const double *src, const double *alpha, double *dst, int64_t n, hipStream_t s);
+hipError_t launch_bgmres_init(int NI, const double *b2, const double *tol, const int32_t *budget, GmresDevState *st,
+                              hipStream_t s);
+hipError_t launch_bgmres_cycle(int NI, const double *r2, int ld, int first, double *sv, GmresDevState *st,
+                               double *scale, int32_t *count, hipStream_t s);
+hipError_t launch_bgmres_step(int NI, int K, const double *h1, const double *h2, const double *n2, int i, int ld,
+                              double *H, double *cs, double *sn, double *sv, GmresDevState *st, double *scale,
+                              int32_t *count, hipStream_t s);
+hipError_t launch_bgmres_backsolve(int NI, int K, int kmax, int ld, const double *H, const double *sv,
+                                   const GmresDevState *st, double *y, double *coef, hipStream_t s);
 
 // device buffers of one matvec's intermediates: the projection's V^T x
 // values and partial-sum slots (operator precision), the skeleton phase's

@@ -4,9 +4,10 @@
 // Control flow follows iterSolver::GMRES: residual check `<= tol` before the
 // first cycle, `< tol` after every Arnoldi step and after every restart,
 // x updated by the back-solve of the rotated Hessenberg system
-// (GMRES.hpp:132-146), real Givens rotations (GMRES.hpp:83-129).  Vectors
-// stay on the device in tree order; per iteration the host receives the new
-// Hessenberg column (i + 2 doubles) and applies the rotations.
+// (GMRES.hpp:132-146), real Givens rotations (GMRES.hpp:83-129).  Vectors,
+// the Hessenberg matrix, the rotations and the residual test all stay on
+// the device (blas.hip); per Arnoldi step the host reads a 56-byte state
+// record to learn whether the solve goes on.
 // A zero subdiagonal (lucky breakdown) leaves the next basis vector zero
 // instead of dividing by zero (the reference has no guard, GMRES.hpp:242).
 #include <chrono>
@@ -24,27 +25,6 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-void gen_rot(double dx, double dy, double &cs, double &sn) {
-    if (dy == 0.0) {
-        cs = 1.0;
-        sn = 0.0;
-    } else if (std::fabs(dy) > std::fabs(dx)) {
-        double temp = dx / dy;
-        sn = 1.0 / std::sqrt(1.0 + temp * temp);
-        cs = temp * sn;
-    } else {
-        double temp = dy / dx;
-        cs = 1.0 / std::sqrt(1.0 + temp * temp);
-        sn = temp * cs;
-    }
-}
-
-void app_rot(double &dx, double &dy, double cs, double sn) {
-    double temp = cs * dx + sn * dy;
-    dy = (-sn) * dx + cs * dy;
-    dx = temp;
-}
-
 }  // namespace
 
 Workspace::Workspace(int64_t n_, int restart_) : n(n_), restart(restart_) {
@@ -53,6 +33,8 @@ Workspace::Workspace(int64_t n_, int restart_) : n(n_), restart(restart_) {
     r.alloc(sizeof(double) * n);
     partial.alloc(sizeof(double) * (size_t)mdot_chunks(n) * 130);
     small.alloc(sizeof(double) * 512);
+    const size_t ld = (size_t)restart + 1;
+    gh.alloc(sizeof(double) * (ld * restart + 4 * ld) + sizeof(GmresDevState) + 64);
     HIP_CHECK(hipHostMalloc(&hbuf, sizeof(double) * 512));
 }
 
@@ -71,14 +53,19 @@ int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int 
                  bool x0_zero, hipStream_t s, GmresResult &res, std::vector<double> *hist) {
     const int64_t n = ws.n;
     const int m = ws.restart;
+    const int ld = m + 1;
     double *V = ws.V.as<double>();
     double *w = ws.w.as<double>();
     double *r = ws.r.as<double>();
-    double *hd = ws.small.as<double>();  // [0,128) h1, [128,256) h2
-    std::vector<double> H((size_t)(m + 1) * m, 0.0), sv(m + 1), cs(m + 1), sn(m + 1);
-    auto Hc = [&](int row, int c) -> double & { return H[(size_t)c * (m + 1) + row]; };
+    double *hd = ws.small.as<double>();  // [0,128) h1, [128,256) h2, [256] |w|^2, [300] |b|^2, [301] |r|^2
+    double *H = ws.gh.as<double>();
+    double *cs = H + (size_t)ld * m, *sn = cs + ld, *sv = sn + ld, *y = sv + ld;
+    GmresDevState *st = reinterpret_cast<GmresDevState *>(y + ld);
+    // pinned record of the state after each step
+    GmresDevState *slot = reinterpret_cast<GmresDevState *>(ws.hbuf + 400);
 
-    const double normb0 = ws.norm2(b, s);
+    HIP_CHECK(launch_mdot(b, n, 1, b, n, ws.partial.as<double>(), hd + 300, 0, s));
+    HIP_CHECK(launch_gmres_init(hd + 300, tol, max_iter, st, s));
     if (x0_zero) {
         HIP_CHECK(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     } else {
@@ -86,83 +73,81 @@ int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int 
         res.matvecs++;
         HIP_CHECK(launch_sub(b, w, r, n, s));
     }
-    double beta = ws.norm2(r, s);
-    double normb = normb0 == 0.0 ? 1.0 : normb0;
-    double resid = beta / normb;
-    res.resid = resid;
-    if (resid <= tol) {  // GMRES.hpp:198-213
+    auto cycle_start = [&](int first) {  // beta, the residual test, sv = beta e_1 (GMRES.hpp:186-213,276-291)
+        HIP_CHECK(launch_mdot(r, n, 1, r, n, ws.partial.as<double>(), hd + 301, 0, s));
+        HIP_CHECK(launch_gmres_cycle(hd + 301, sv, ld, first, st, s));
+        HIP_CHECK(hipMemcpyAsync(slot, st, sizeof(GmresDevState), hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        return slot[0];
+    };
+    GmresDevState cur = cycle_start(1);
+    res.resid = cur.resid;
+    if (!cur.active) {  // GMRES.hpp:198-213 (or no iteration budget at all)
         res.iterations = 0;
-        res.converged = true;
+        res.converged = cur.conv == -3;
         return 0;
     }
-    int j = 1;
-    auto update = [&](int k) {  // GMRES.hpp:132-146
-        std::vector<double> y(sv.begin(), sv.begin() + k + 1);
-        for (int i = k; i >= 0; i--) {
-            y[i] /= Hc(i, i);
-            for (int q = i - 1; q >= 0; q--) y[q] -= Hc(q, i) * y[i];
-        }
-        std::memcpy(ws.hbuf, y.data(), sizeof(double) * (k + 1));
-        HIP_CHECK(hipMemcpyAsync(hd, ws.hbuf, sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
-        HIP_CHECK(launch_gemv(V, n, k + 1, hd, 1.0, x, n, s));
+    auto launch_step = [&](int i) {
+        double *vi = V + (size_t)i * n;
+        A(vi, w);
+        res.matvecs++;
+        // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
+        HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd, 0, s));
+        HIP_CHECK(launch_gemv(V, n, i + 1, hd, -1.0, w, n, s));
+        HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd + 128, 0, s));
+        HIP_CHECK(launch_gemv(V, n, i + 1, hd + 128, -1.0, w, n, s));
+        HIP_CHECK(launch_mdot(w, n, 1, w, n, ws.partial.as<double>(), hd + 256, 0, s));
+        HIP_CHECK(launch_gmres_step(H, cs, sn, sv, hd, i, ld, st, s));
+        HIP_CHECK(launch_scale_by(w, &st->scale, V + (size_t)(i + 1) * n, n, s));
+        HIP_CHECK(hipMemcpyAsync(slot, st, sizeof(GmresDevState), hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
     };
-    while (j <= max_iter) {
-        HIP_CHECK(launch_scale(r, 1.0 / beta, V, n, s));
-        std::fill(sv.begin(), sv.end(), 0.0);
-        sv[0] = beta;
+    // (Launching step i + 1 before step i's record was read measured slower
+    // on small problems: the step launched after the end costs more than
+    // the host wait it hides.)
+    double resid = cur.resid;
+    while (true) {
+        HIP_CHECK(launch_scale_by(r, &st->scale, V, n, s));  // v_0 = r / beta
         int i = 0;
-        for (; i < m && j <= max_iter; i++, j++) {
+        while (true) {
+            launch_step(i);
+            cur = slot[0];
             if (hist) hist->push_back(resid);
-            double *vi = V + (size_t)i * n;
-            A(vi, w);
-            res.matvecs++;
-            // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
-            HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd, 0, s));
-            HIP_CHECK(launch_gemv(V, n, i + 1, hd, -1.0, w, n, s));
-            HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd + 128, 0, s));
-            HIP_CHECK(launch_gemv(V, n, i + 1, hd + 128, -1.0, w, n, s));
-            HIP_CHECK(launch_mdot(w, n, 1, w, n, ws.partial.as<double>(), hd + 256, 0, s));
-            HIP_CHECK(hipMemcpyAsync(ws.hbuf, hd, sizeof(double) * 257, hipMemcpyDeviceToHost, s));
-            HIP_CHECK(hipStreamSynchronize(s));
-            for (int k = 0; k <= i; k++) Hc(k, i) = ws.hbuf[k] + ws.hbuf[128 + k];
-            const double hn = std::sqrt(ws.hbuf[256]);
-            Hc(i + 1, i) = hn;
-            if (!std::isfinite(hn)) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
-            double *vn = V + (size_t)(i + 1) * n;
-            if (hn != 0.0) HIP_CHECK(launch_scale(w, 1.0 / hn, vn, n, s));
-            else HIP_CHECK(hipMemsetAsync(vn, 0, sizeof(double) * n, s));
-            for (int k = 0; k < i; k++) app_rot(Hc(k, i), Hc(k + 1, i), cs[k], sn[k]);
-            gen_rot(Hc(i, i), Hc(i + 1, i), cs[i], sn[i]);
-            app_rot(Hc(i, i), Hc(i + 1, i), cs[i], sn[i]);
-            app_rot(sv[i], sv[i + 1], cs[i], sn[i]);
-            resid = std::fabs(sv[i + 1]) / normb;
+            resid = cur.resid;
             res.resid = resid;
-            if (resid < tol) {
-                update(i);
-                res.iterations = j;
-                res.converged = true;
-                return j;
-            }
+            if (!cur.active || i + 1 == m) break;
+            i++;
         }
-        // GMRES.hpp:271 updates with m - 1 columns; when the iteration budget
-        // ends the cycle early only i columns exist
-        update(i - 1);
+        if (cur.conv == -2) throw ArgError{HODLR_E_BREAKDOWN, "non-finite Arnoldi norm"};
+        const int k = cur.active ? m - 1 : cur.conv;
+        HIP_CHECK(launch_gmres_backsolve(H, sv, ld, k, y, s));
+        HIP_CHECK(launch_gemv(V, n, k + 1, y, 1.0, x, n, s));
+        if (!cur.active && cur.resid < tol) {  // GMRES.hpp:253-268
+            HIP_CHECK(hipStreamSynchronize(s));
+            res.iterations = cur.steps;
+            res.converged = true;
+            return cur.steps;
+        }
+        // end of the cycle or of the iteration budget: restart from the true
+        // residual (GMRES.hpp:271-291); the reference's j has moved one past
+        // the last step here
         A(x, w);
         res.matvecs++;
         HIP_CHECK(launch_sub(b, w, r, n, s));
-        beta = ws.norm2(r, s);
-        resid = beta / normb;
+        cur = cycle_start(0);
+        resid = cur.resid;
         res.resid = resid;
-        if (resid < tol) {
-            res.iterations = j;
+        if (cur.conv == -3) {
+            res.iterations = cur.steps + 1;
             res.converged = true;
-            return j;
+            return cur.steps + 1;
+        }
+        if (!cur.active) {  // max_iter reached (GMRES.hpp:298-302)
+            res.iterations = max_iter;
+            res.converged = false;
+            return max_iter;
         }
     }
-    res.iterations = max_iter;
-    res.converged = false;
-    return max_iter;
 }
 
 struct SolveState {

@@ -15,6 +15,10 @@ struct Workspace {
     int64_t n;
     int restart;
     DevBuf V, w, r, partial, small;
+    // device GMRES state: Hessenberg matrix (restart+1) x restart, cs, sn,
+    // the rotated right-hand side and the back-solved y (restart+1 each),
+    // then the solve's GmresDevState
+    DevBuf gh;
     double *hbuf = nullptr;  // pinned host staging
     Workspace(int64_t n, int restart);
     ~Workspace();
