@@ -654,7 +654,7 @@ def solve_summary(r, steps):
     }
 
 
-def bench_cfg1(args, H):
+def bench_random3d(args, H):
     """NDIM 3, N = 10,000 points uniform iid in [-1,1]^3 (SplitMix64 seed 1),
     A = I + (1/N)/r, Nmax 64, eps_ACA 1e-10, d' 0, uniform RHS, GMRES(100) to
     1e-10: the GPU FP64 and mixed solves beside the FP64 CPU restatement run
@@ -679,7 +679,7 @@ def bench_cfg1(args, H):
     mv_err = float(np.linalg.norm(y - yd) / np.linalg.norm(yd))
     xs = r["f64"]["x"]
     res_dense = float(np.linalg.norm(b - op.dense_rows(np.arange(n), xs)) / np.linalg.norm(b))
-    log("cfg1: gpu fp64 %.2f ms/solve (%d it), matvec %.3f ms, vs dense: matvec %.3g, solution residual %.3g"
+    log("random3d: gpu fp64 %.2f ms/solve (%d it), matvec %.3f ms, vs dense: matvec %.3g, solution residual %.3g"
         % (r["f64"]["lat"].mean() * 1e3, r["f64"]["info"]["iterations"], mv[0], mv_err, res_dense))
     out = {
         "metric": "solves/s at 1e-10 residual, GPU FP64 vs the FP64 CPU path",
@@ -720,7 +720,7 @@ def bench_cfg1(args, H):
                         "iterations": ir["iterations"], "resid": ir["resid"],
                         "sample": "oracle FP64 restatement end to end on the whole problem, reference execution "
                                   "model (skeleton blocks for |cols| > 2*Nmax), %d thread(s)" % nt}
-            log("cfg1 %s: build %.2fs, matvec %.3fs, solve %.3fs (%d it)" % (key, t_asm, t_mv, t_solve,
+            log("random3d %s: build %.2fs, matvec %.3fs, solve %.3fs (%d it)" % (key, t_asm, t_mv, t_solve,
                                                                            ir["iterations"]))
         out["cpu_baseline"] = out["cpu_1t"]
     return out
@@ -739,7 +739,7 @@ def gauss_legendre_ref_order(m):
     return x[order], w[order]
 
 
-def bench_cfg3(args, H):
+def bench_gl4d(args, H):
     """4-D second-kind integral equation: the tensor product of 16
     Gauss-Legendre nodes per axis on [-1,1]^4 (N = 65,536), A_ij = delta_ij +
     w_j / |x_i - x_j| with w_j the product weight, Nmax 64, eps_ACA 1e-8,
@@ -773,7 +773,7 @@ def bench_cfg3(args, H):
              "roofline": kernel_roofline(op, H, r["mixed"]["ms"], r["mixed"]["cnt"])}
         d.update(solve_summary(r, args.steps))
         legs["dprime_%d" % dp] = d
-        log("cfg3 d'=%d: %d blocks, build %.2fs, mixed %.2f ms/solve, fp64 %.2f ms/solve, matvec f32 %.3f ms"
+        log("gl4d d'=%d: %d blocks, build %.2fs, mixed %.2f ms/solve, fp64 %.2f ms/solve, matvec f32 %.3f ms"
             % (dp, st["n_blocks"], t_build, r["mixed"]["lat"].mean() * 1e3, r["f64"]["lat"].mean() * 1e3,
                mv32[0]))
         del op, tree
@@ -942,7 +942,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=25.0, help="seconds of CPU baseline work per leg")
     ap.add_argument("--no-extra", action="store_true",
                     help="grid100: skip the batched and serving legs reported beside the headline")
-    ap.add_argument("--workload", choices=["grid100", "batch256", "serve", "skeleton", "cfg1", "cfg3"],
+    ap.add_argument("--workload", choices=["grid100", "batch256", "serve", "skeleton", "random3d", "gl4d"],
                     default="grid100",
                     help="grid100: one N=1e6 solve per step; batch256: 256 uneven problems x 8 RHS "
                          "solved as one batch per step; serve: a Poisson stream of solve requests; "
@@ -986,8 +986,8 @@ def main():
     if args.workload == "skeleton":
         print(json.dumps(bench_skeleton(args, H)))
         return
-    if args.workload in ("cfg1", "cfg3"):
-        print(json.dumps((bench_cfg1 if args.workload == "cfg1" else bench_cfg3)(args, H)))
+    if args.workload in ("random3d", "gl4d"):
+        print(json.dumps((bench_random3d if args.workload == "random3d" else bench_gl4d)(args, H)))
         return
 
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)
@@ -1166,8 +1166,8 @@ def main():
         import subprocess
         legs = (("batched", ["--workload", "batch256", "--steps", "3", "--warmup", "1", "--cpu-budget", "10"]),
                 ("serving", ["--workload", "serve", "--requests", "100", "--sub-requests", "100"]),
-                ("config1", ["--workload", "cfg1", "--steps", "20", "--warmup", "2"]),
-                ("config3", ["--workload", "cfg3", "--steps", "10", "--warmup", "2"]),
+                ("random3d_N1e4", ["--workload", "random3d", "--steps", "20", "--warmup", "2"]),
+                ("gauss_legendre_4d", ["--workload", "gl4d", "--steps", "10", "--warmup", "2"]),
                 ("skeleton", ["--workload", "skeleton", "--steps", "5", "--warmup", "1"]),
                 ("skeleton_beyond_hbm", ["--workload", "skeleton", "--n1d", "160", "--sys-size", "128",
                                          "--steps", "3", "--warmup", "1"]))
