@@ -22,7 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libhodlrnd.so")
 
 OK, E_ARG, E_DEVICE, E_NOMEM, E_RANK, E_NOCONV, E_BREAKDOWN, E_STATE = 0, -1, -2, -3, -4, -5, -6, -7
-KERNEL_LOG, KERNEL_INV_R, KERNEL_EXP = 0, 1, 2
+KERNEL_LOG, KERNEL_INV_R, KERNEL_EXP, KERNEL_CUSTOM = 0, 1, 2, 3
 PREC_F64, PREC_F32 = 1, 2
 SOLVE_F64, SOLVE_MIXED = 0, 1
 
@@ -109,6 +109,8 @@ SIGNATURES = {
     "hodlr_op_save": (C.c_int, [C.c_void_p, C.c_char_p]),
     "hodlr_op_load": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_void_p]),
     "hodlr_op_get_tree": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "hodlr_op_assemble_custom": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                           C.c_double, C.c_int, C.c_void_p]),
     "hodlr_vec_save": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p]),
     "hodlr_vec_load": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "hodlr_op_get_stats": (C.c_int, [C.c_void_p, C.c_void_p]),

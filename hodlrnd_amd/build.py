@@ -50,7 +50,8 @@ def build(verbose=True):
 
 
 def build_examples(verbose=True):
-    """The reference-shaped C++ API examples (host compiler + the C ABI)."""
+    """The reference-shaped C++ API examples: host compiler + the C ABI, and
+    device_kernels (hipcc) whose kernel classes' Kernel_Fun runs on the GPU."""
     out_dir = os.path.join(HERE, "examples", "build")
     os.makedirs(out_dir, exist_ok=True)
     root = os.path.dirname(HERE)
@@ -66,6 +67,19 @@ def build_examples(verbose=True):
         if verbose:
             print("built", out)
         outs.append(out)
+    # kernel classes compiled as host and device code: hipcc, with the same
+    # floating-point flags as the library
+    out = os.path.join(out_dir, "device_kernels")
+    cmd = [HIPCC, "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off", "-Xclang",
+           "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-I" + os.path.join(root, "include"),
+           os.path.join(HERE, "examples", "device_kernels.cpp"), "-o", out, "-L" + HERE, "-lhodlrnd",
+           "-Wl,-rpath,$ORIGIN/../.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("example build failed:\n" + r.stderr[-4000:])
+    if verbose:
+        print("built", out)
+    outs.append(out)
     return outs
 
 

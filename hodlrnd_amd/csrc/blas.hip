@@ -5,8 +5,8 @@
 // instead of the reference's i+1 dependent dot/axpy pairs (modified
 // Gram-Schmidt, GMRES.hpp:236-240).  Dot products reduce in a fixed order:
 // per-chunk workgroup sums, then one pass over the chunks.
-#include "aca_layout.h"
-#include "device_common.h"
+#include "../../include/hodlrnd/device/aca_layout.h"
+#include "../../include/hodlrnd/device/device_common.h"
 
 namespace hodlr {
 

@@ -4,6 +4,7 @@
 #include <new>
 
 #include <algorithm>
+#include <cmath>
 #include <iterator>
 #include <map>
 #include <mutex>
@@ -228,6 +229,17 @@ int hodlr_tree_build(hodlr_ctx *ctx, int ndim, int64_t n, const double *pts, con
         if (n < 0 || (n > 0 && !pts) || !lo || !hi) throw ArgError{HODLR_E_ARG, "null points or box"};
         if (nmax < 1) throw ArgError{HODLR_E_ARG, "nmax must be positive"};
         if (n > INT32_MAX) throw ArgError{HODLR_E_ARG, "at most 2^31-1 points"};
+        for (int k = 0; k < ndim; k++)
+            if (!(std::isfinite(lo[k]) && std::isfinite(hi[k]) && lo[k] < hi[k]))
+                throw ArgError{HODLR_E_ARG, "bounding box needs finite lo < hi in every dimension"};
+        // a point outside the box is never separated from the box's far
+        // corner by the midpoint splits, so the tree would deepen forever
+        for (int64_t i = 0; i < n; i++)
+            for (int k = 0; k < ndim; k++) {
+                const double x = pts[i * ndim + k];
+                if (!(x >= lo[k] && x <= hi[k]))
+                    throw ArgError{HODLR_E_ARG, "point " + std::to_string(i) + " lies outside the bounding box"};
+            }
         auto *t = new hodlr_tree;
         t->ctx = ctx;
         t->ndim = ndim;
@@ -320,6 +332,47 @@ int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
             throw;
         }
         for (auto &kd : op->kds) kd.weight = nullptr;  // caller arrays are not referenced after assembly
+        op->kd.weight = nullptr;
+        *out = op;
+    });
+}
+
+int hodlr_op_assemble_custom(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k, const void *kernel_obj,
+                             int64_t kernel_obj_bytes, const void *entries, double eps_aca, int prec_flags,
+                             hodlr_op **out) {
+    if (!ctx || !t || !k || !entries || !out || kernel_obj_bytes < 0 || (kernel_obj_bytes > 0 && !kernel_obj))
+        return HODLR_E_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        if (k->ndim != t->ndim) throw ArgError{HODLR_E_ARG, "kernel ndim differs from the tree's"};
+        if (k->family != HODLR_KERNEL_CUSTOM)
+            throw ArgError{HODLR_E_ARG, "hodlr_op_assemble_custom takes the HODLR_KERNEL_CUSTOM family"};
+        const auto *L = static_cast<const EntryLaunchers *>(entries);
+        if (L->version != kEntryAbiVersion || L->layout != entry_abi_layout())
+            throw ArgError{HODLR_E_ARG, "entry launcher table built against another layout of the entry kernels"};
+        if (!L->aca || !L->dense_fill || !L->near_fill || !L->dense_rows)
+            throw ArgError{HODLR_E_ARG, "entry launcher table has a null launcher"};
+        if (kernel_obj_bytes > (int64_t)1 << 20) throw ArgError{HODLR_E_ARG, "kernel object larger than 1 MiB"};
+        if (!(eps_aca > 0.0 && eps_aca < 1.0)) throw ArgError{HODLR_E_ARG, "eps_aca must be in (0, 1)"};
+        set_device(ctx);
+        auto *op = new hodlr_op;
+        op->ctx = ctx;
+        op->trees = {t};
+        op->kds = {*k};
+        op->eps = eps_aca;
+        op->prec_flags = prec_flags | HODLR_PREC_F64;
+        op->entries = L;
+        try {
+            if (kernel_obj_bytes > 0) {
+                op->kobj.alloc((size_t)kernel_obj_bytes);
+                HIP_CHECK(hipMemcpy(op->kobj.p, kernel_obj, (size_t)kernel_obj_bytes, hipMemcpyHostToDevice));
+            }
+            assemble(*op);
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        for (auto &kd : op->kds) kd.weight = nullptr;
         op->kd.weight = nullptr;
         *out = op;
     });
@@ -484,7 +537,7 @@ int hodlr_dense_rows(hodlr_op *op, int64_t nrows, const int64_t *rows, const dou
         upload(dx, xt, s);
         upload(dpb, op->pos_base, s);
         dy.alloc(sizeof(double) * std::max<int64_t>(nrows, 1));
-        HIP_CHECK(launch_dense_rows(drows.as<int64_t>(), dprob.as<int32_t>(), nrows, op->kdev.as<KernelDev>(),
+        HIP_CHECK(op->entries->dense_rows(drows.as<int64_t>(), dprob.as<int32_t>(), nrows, op->kdev.as<KernelDev>(),
                                     dpb.as<int64_t>(), dx.as<double>(), dy.as<double>(), s));
         HIP_CHECK(hipMemcpyAsync(y, dy.p, sizeof(double) * nrows, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
@@ -577,6 +630,9 @@ int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes) {
 int hodlr_op_save(hodlr_op *op, const char *path) {
     if (!op || !path) return HODLR_E_ARG;
     return guard(op->ctx, [&] {
+        for (const auto &kd : op->kds)
+            if (kd.family == HODLR_KERNEL_CUSTOM)
+                throw ArgError{HODLR_E_STATE, "an operator with a caller-compiled kernel cannot be saved"};
         set_device(op->ctx);
         save_op(*op, path);
     });

@@ -22,8 +22,8 @@
 //           Each leaf row is written once (no atomics).
 // With 4 or more right-hand sides both passes also have matrix-core forms
 // (further down); the operator picks one per pass.
-#include "aca_layout.h"
-#include "device_common.h"
+#include "../../include/hodlrnd/device/aca_layout.h"
+#include "../../include/hodlrnd/device/device_common.h"
 
 namespace hodlr {
 

@@ -279,7 +279,7 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
         });
         if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double), s);
         upload(d_blocks, chunk, s, true);
-        HIP_CHECK(launch_aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), op.kdev.as<KernelDev>(), op.eps,
+        HIP_CHECK(op.entries->aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), op.kdev.as<KernelDev>(), op.eps,
                              pool.as<double>(), d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
                              d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
                              op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(),
@@ -305,6 +305,7 @@ void assemble(hodlr_op &op) {
     const int P = (int)op.trees.size();
     op.tree = op.trees[0];
     op.kd = op.kds[0];
+    if (!op.entries) op.entries = &builtin_entry_launchers();
 
     // ---- global numbering of positions and nodes ----
     std::vector<int64_t> node_base(P + 1, 0);
@@ -353,6 +354,7 @@ void assemble(hodlr_op &op) {
             kd[p].X = op.X.as<double>() + xoff[p];
             kd[p].W = woff[p] >= 0 ? op.W.as<double>() + woff[p] : nullptr;
             kd[p].pos0 = op.pos_base[p];
+            kd[p].obj = op.kobj.p;
         }
         upload(op.kdev, kd, s);
         std::vector<int64_t> ls(N > 0 ? N : 1, 0);
@@ -1055,9 +1057,9 @@ void assemble(hodlr_op &op) {
         upload(d_colsrc, colsrc, s, true);
         upload(d_dense, dense, s, true);
         upload(d_dest, dest, s, true);
-        HIP_CHECK(launch_near_fill(d_near.as<NearLeaf>(), (int64_t)near.size(), d_colsrc.as<int64_t>(),
+        HIP_CHECK(op.entries->near_fill(d_near.as<NearLeaf>(), (int64_t)near.size(), d_colsrc.as<int64_t>(),
                                    op.kdev.as<KernelDev>(), op.NP64.as<double>(), s));
-        HIP_CHECK(launch_dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), op.kdev.as<KernelDev>(),
+        HIP_CHECK(op.entries->dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), op.kdev.as<KernelDev>(),
                                     op.leaf_start.as<int64_t>(), op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
         std::vector<int64_t> lowrank, staged;

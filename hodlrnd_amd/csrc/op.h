@@ -4,25 +4,17 @@
 #include <functional>
 #include <memory>
 
-#include "aca_layout.h"
-#include "device_common.h"
+#include "../../include/hodlrnd/device/aca_layout.h"
+#include "../../include/hodlrnd/device/device_common.h"
 #include "internal.h"
 
 namespace hodlr {
 
-hipError_t launch_aca(int bs, const AcaBlock *blocks, int64_t nblocks, const KernelDev *K, double tol,
-                      double *pool, int32_t *rank, int32_t *flags, int fill, const FillDest *dest,
-                      const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, double *VP,
-                      int32_t *SK, const AcaStage &S, hipStream_t s);
+// the entry kernels' launchers for the built-in kernel families
+const EntryLaunchers &builtin_entry_launchers();
 hipError_t launch_stage_copy(const StageCopy *items, int64_t n, const double *buf, const int64_t *leaf_start,
                              const int32_t *leaf_cnt, double *UP, double *VP, hipStream_t s);
-hipError_t launch_dense_fill(const DenseBlock *b, int64_t nb, const KernelDev *K,
-                             const int64_t *leaf_start, const int32_t *leaf_cnt, double *UP, hipStream_t s);
-hipError_t launch_near_fill(const NearLeaf *l, int64_t nl, const int64_t *colsrc, const KernelDev *K,
-                            double *NP, hipStream_t s);
 hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s);
-hipError_t launch_dense_rows(const int64_t *rows, const int32_t *rprob, int64_t nrows, const KernelDev *K,
-                             const int64_t *pos_base, const double *x, double *y, hipStream_t s);
 hipError_t launch_zero_self_diag(const NearLeaf *l, int64_t nl, float *NP, hipStream_t s);
 hipError_t launch_f32_result(const float *y, const double *x, const double *diag, double *out, int64_t n, int k,
                              hipStream_t s);
@@ -112,6 +104,10 @@ struct hodlr_op {
     hodlr::DevBuf X, W, leaf_start, leaf_cnt;
     hodlr::DevBuf ord;   // per problem and level: global positions in ascending point id per node
     hodlr::DevBuf kdev;  // per problem KernelDev
+    // entry kernels: the built-in families', or a caller's (custom family,
+    // whose kernel object is copied to kobj)
+    const hodlr::EntryLaunchers *entries = nullptr;
+    hodlr::DevBuf kobj;
     hodlr::DevBuf diag;  // per global position: A_ii (the FP32 copy stores a zero diagonal)
     bool uniform_diag = true;
 

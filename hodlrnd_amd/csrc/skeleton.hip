@@ -29,8 +29,8 @@
 // 1 ulp; one for the FP32 operator: ~1e-14); this path does not have to
 // round like the CPU restatement.  The phases are compute bound: about 15-20
 // FP64 vector instructions per entry, plus one FMA per entry and RHS.
-#include "aca_layout.h"
-#include "device_common.h"
+#include "../../include/hodlrnd/device/aca_layout.h"
+#include "../../include/hodlrnd/device/device_common.h"
 
 namespace hodlr {
 

@@ -16,6 +16,7 @@
 //                          the parent's neighbours in the parent's order.
 // The implementation is a level-synchronous stable counting sort over one
 // permutation array, so every node is a contiguous range of tree positions.
+#include <algorithm>
 #include <cmath>
 
 #include "internal.h"
@@ -86,6 +87,13 @@ void build_tree(hodlr_tree &t) {
     while (biggest > t.nmax) {
         const int64_t p0 = t.level_off[l], np = t.level_off[l + 1] - p0;
         const int64_t c0 = t.level_off[l + 1], ncnew = np * nch;
+        // more than nmax coincident points never split (HDD_tree.hpp:43-69
+        // would deepen without end): stop once a level would hold more than
+        // max(2^20, 64 n) nodes
+        if (ncnew > std::max<int64_t>(int64_t(1) << 20, 64 * t.n))
+            throw ArgError{HODLR_E_ARG, "tree level " + std::to_string(l + 1) + " would hold " +
+                                            std::to_string(ncnew) + " nodes: more than nmax points share every box "
+                                                                    "(coincident points?)"};
         t.level_off.push_back(c0 + ncnew);
         t.start.resize(c0 + ncnew);
         t.count.resize(c0 + ncnew);

@@ -38,7 +38,10 @@ enum {
 enum {
     HODLR_KERNEL_LOG = 0,  /* f(r) = log r     (user_kernel.hpp:46-49)   */
     HODLR_KERNEL_INV_R = 1,/* f(r) = 1 / r     (user_kernel.hpp:101-104) */
-    HODLR_KERNEL_EXP = 2   /* f(r) = exp(-r)   (user_kernel.hpp:167-170) */
+    HODLR_KERNEL_EXP = 2,  /* f(r) = exp(-r)   (user_kernel.hpp:167-170) */
+    HODLR_KERNEL_CUSTOM = 3 /* f(r) = the caller's Kernel_Fun (user_kernel.hpp:46,101,167),
+                               run on the device by caller-compiled kernels:
+                               hodlr_op_assemble_custom only */
 };
 
 /* operator precisions built by hodlr_op_assemble */
@@ -165,6 +168,21 @@ void hodlr_tree_destroy(hodlr_tree *t);
  * the dense near field.  The tree must outlive the operator. */
 int hodlr_op_assemble(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
                       double eps_aca, int prec_flags, hodlr_op **out);
+/* HDD_matrix.hpp:39-41 (Assemble_matrix_operators) for a user kernel with
+ * any Kernel_Fun (user_kernel.hpp:46-58,101-113,167-177):
+ *   A_ij = diag (i == j),  scale * f(|x_i - x_j|) [* weight_j] (i != j),
+ * f evaluated on the device by kernels the caller compiled with hipcc for its
+ * own f.  `entries` is that launcher table: hodlr::entry_launchers<Fn>() of
+ * include/hodlrnd/device/entry_kernels.hpp, which the C++ API instantiates
+ * for a kernel class whose Kernel_Fun is device code; a table built against
+ * another layout of the descriptors is refused.  kernel_obj_bytes bytes at
+ * kernel_obj (at most 1 MiB; the caller's kernel object) are copied to device
+ * memory and handed to f.  k->family must be HODLR_KERNEL_CUSTOM.  Every
+ * factor is stored (no skeleton form) and the operator cannot be saved;
+ * otherwise it is an operator like hodlr_op_assemble's. */
+int hodlr_op_assemble_custom(hodlr_ctx *ctx, hodlr_tree *t, const hodlr_kernel_desc *k,
+                             const void *kernel_obj, int64_t kernel_obj_bytes, const void *entries,
+                             double eps_aca, int prec_flags, hodlr_op **out);
 /* A batch of independent problems assembled together (no reference
  * counterpart; the reference assembles one operator at a time through
  * HDD_matrix.hpp:39-41).  The result is their block-diagonal operator: one

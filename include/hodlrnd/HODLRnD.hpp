@@ -21,12 +21,22 @@
 // device cannot call their host getMatrixEntry, so the operator identifies
 // the entries instead.  It probes getMatrixEntry and accepts
 //   A_ij = diag (i == j),  w_j * f(|x_i - x_j|) (i != j),
-//   f one of log r, 1/r, exp(-r), w_j a constant or a per-column weight,
-// when every probed entry agrees to 1e-13 relative; the entries are then
-// evaluated on the GPU.  A kernel of another form fails with a message; such
-// a class can describe itself with
-//   hodlr_kernel_desc device_kernel() const
-// which, when present, is used without probing.
+//   w_j a constant or a per-column weight,
+// when every probed entry agrees to 1e-13 relative, in this order:
+//   1. the class's own description, hodlr_kernel_desc device_kernel() const,
+//      when it has one (no probing);
+//   2. f one of the built-in log r, 1/r, exp(-r);
+//   3. f the class's own Kernel_Fun, run on the GPU: compiled with hipcc
+//      (-x hip -ffp-contract=off), the classes included between
+//        #include "hodlrnd/device_kernels_begin.hpp"
+//        #include "user_kernel.hpp"        (or the classes themselves)
+//        #include "hodlrnd/device_kernels_end.hpp"
+//      are host and device code, and the library's assembly runs entry
+//      kernels instantiated for their Kernel_Fun in the caller's program
+//      (hodlr_op_assemble_custom; every factor is stored, whatever SYS_SIZE).
+//      Kernel_Fun may read the object's own members (the object's bytes are
+//      copied to the device) but not follow its pointers or host globals.
+// A kernel of another form fails with a message.
 // Vectors are plain host arrays (the reference uses Eigen, which this API
 // does not require; the Eigen vector names its user kernels use are
 // provided unless Eigen itself is included first).  Failures throw
@@ -73,7 +83,8 @@ const double eps_ACA = HODLR_EPS_ACA;
 constexpr int SYS_SIZE = HODLR_SYS_SIZE;
 inline int numPoints = 10;
 inline size_t N = 1;
-const double PI = 4.0 * std::atan(1.0);
+// 4 atan(1) (myHeaders.hpp:30), constexpr so device code may use it
+constexpr double PI = 3.14159265358979323846;
 
 using dtype = double;
 using dtype_base = double;
@@ -346,28 +357,22 @@ inline double family_value(int family, double r) {
 }
 
 // The device evaluates A_ij = diag (i == j), scale * f(r_ij) * w_j (i != j)
-// for f in {log r, 1/r, exp(-r)} (hodlr_kernel_desc).  A reference-shaped
-// kernel class only has the host callback getMatrixEntry
-// (kernel_function.hpp:26,38,51), so its entries are matched to that form:
-// the diagonal from getMatrixEntry(i, i) (equal on every probed i), then for
-// each f a weight per column w_j = A(i_j, j) / f(r) from one partner row
-// i_j, checked on 512 further pairs (i, j) to 1e-13 relative.  Equal weights
-// become the scale; otherwise they are passed per point in `weights`.
-template <class K>
-hodlr_kernel_desc identify_kernel(K *ker, const std::vector<ptsnD> &P, std::vector<double> &weights) {
+// (hodlr_kernel_desc).  A reference-shaped kernel class only has the host
+// callback getMatrixEntry (kernel_function.hpp:26,38,51), so its entries are
+// matched to that form for a given f: the diagonal from getMatrixEntry(i, i)
+// (equal on every probed i), then a weight per column w_j = A(i_j, j) / f(r)
+// from one partner row i_j, checked on 512 further pairs (i, j) to 1e-13
+// relative.  Equal weights become the scale; otherwise they are passed per
+// point in `weights`.  Returns false when the entries are not of that form.
+template <class K, class F>
+bool match_form(K *ker, const std::vector<ptsnD> &P, F f, hodlr_kernel_desc &kd, std::vector<double> &weights) {
     const int64_t n = (int64_t)P.size();
     auto A = [&](int64_t i, int64_t j) { return (double)ker->getMatrixEntry((int)i, (int)j); };
     auto r_of = [&](int64_t i, int64_t j) { return nd_points::euclidean_distance(P[(size_t)i], P[(size_t)j]); };
-    hodlr_kernel_desc kd{HODLR_KERNEL_INV_R, NDIM, 0.0, 0.0, nullptr};
     weights.clear();
-    if (n == 0) return kd;
-    kd.diag = A(0, 0);
-    const int64_t step = n > 64 ? n / 64 : 1;
-    for (int64_t i = 0; i < n; i += step)
-        if (A(i, i) != kd.diag)
-            throw hodlr::Error(HODLR_E_ARG, "kernel has no single diagonal value (getMatrixEntry(i, i) varies); "
-                                            "describe it with device_kernel()");
-    if (n == 1) return kd;
+    kd.scale = 1.0;
+    kd.weight = nullptr;
+    if (n < 2) return true;
     // a deterministic stream of off-diagonal probe pairs
     uint64_t st = 0x9E3779B97F4A7C15ull;
     auto next = [&](int64_t m) {
@@ -376,59 +381,100 @@ hodlr_kernel_desc identify_kernel(K *ker, const std::vector<ptsnD> &P, std::vect
         st ^= st << 17;
         return (int64_t)(st % (uint64_t)m);
     };
+    const int64_t step = n > 64 ? n / 64 : 1;
+    // per-column weight from a partner row whose f(r) is finite and non-zero
+    auto weight = [&](int64_t j, double &w) {
+        for (int t = 0; t < 8; t++) {
+            int64_t i = t == 0 ? (j + n / 2) % n : next(n);
+            if (i == j) i = (j + 1) % n;
+            const double fv = f(r_of(i, j));
+            if (std::isfinite(fv) && fv != 0.0) {
+                w = A(i, j) / fv;
+                return std::isfinite(w);
+            }
+        }
+        return false;
+    };
+    std::vector<double> w;
+    bool uniform = true;
+    double w0 = 0.0;
+    const int64_t nprobe = n < 64 ? n : 64;
+    for (int64_t q = 0; q < nprobe; q++) {  // a sample of columns first
+        double wj;
+        if (!weight((q * step) % n, wj)) return false;
+        if (q == 0) w0 = wj;
+        uniform = uniform && std::fabs(wj - w0) <= 1e-13 * std::fabs(w0);
+    }
+    if (!uniform) {
+        w.resize((size_t)n);
+        for (int64_t j = 0; j < n; j++)
+            if (!weight(j, w[(size_t)j])) return false;
+    }
+    for (int q = 0; q < 512; q++) {
+        int64_t i = next(n), j = next(n);
+        if (i == j) continue;
+        const double fv = f(r_of(i, j));
+        const double want = A(i, j), got = (uniform ? w0 : w[(size_t)j]) * fv;
+        if (!(std::isfinite(fv) && std::fabs(got - want) <= 1e-13 * std::fabs(want) + 1e-300)) return false;
+    }
+    if (uniform) {
+        kd.scale = w0;
+    } else {
+        weights = std::move(w);
+        kd.weight = weights.data();
+    }
+    return true;
+}
+
+// the diagonal, equal on every probed i
+template <class K>
+double match_diag(K *ker, int64_t n) {
+    if (n == 0) return 0.0;
+    const double d = ker->getMatrixEntry(0, 0);
+    const int64_t step = n > 64 ? n / 64 : 1;
+    for (int64_t i = 0; i < n; i += step)
+        if (ker->getMatrixEntry((int)i, (int)i) != d)
+            throw hodlr::Error(HODLR_E_ARG, "kernel has no single diagonal value (getMatrixEntry(i, i) varies); "
+                                            "describe it with device_kernel()");
+    return d;
+}
+
+// f among the built-in families; throws when none matches
+template <class K>
+hodlr_kernel_desc identify_kernel(K *ker, const std::vector<ptsnD> &P, std::vector<double> &weights) {
+    hodlr_kernel_desc kd{HODLR_KERNEL_INV_R, NDIM, 0.0, 0.0, nullptr};
+    kd.diag = match_diag(ker, (int64_t)P.size());
+    if (P.size() < 2) return kd;
     const int fams[3] = {HODLR_KERNEL_INV_R, HODLR_KERNEL_LOG, HODLR_KERNEL_EXP};
     for (int family : fams) {
-        // per-column weight from a partner row whose f(r) is finite and non-zero
-        auto weight = [&](int64_t j, double &w) {
-            for (int t = 0; t < 8; t++) {
-                int64_t i = t == 0 ? (j + n / 2) % n : next(n);
-                if (i == j) i = (j + 1) % n;
-                const double f = family_value(family, r_of(i, j));
-                if (std::isfinite(f) && f != 0.0) {
-                    w = A(i, j) / f;
-                    return std::isfinite(w);
-                }
-            }
-            return false;
-        };
-        std::vector<double> w;
-        bool ok = true, uniform = true;
-        double w0 = 0.0;
-        const int64_t nprobe = n < 64 ? n : 64;
-        for (int64_t q = 0; q < nprobe && ok; q++) {  // a sample of columns first
-            double wj;
-            ok = weight((q * step) % n, wj);
-            if (!ok) break;
-            if (q == 0) w0 = wj;
-            uniform = uniform && std::fabs(wj - w0) <= 1e-13 * std::fabs(w0);
+        if (match_form(ker, P, [&](double r) { return family_value(family, r); }, kd, weights)) {
+            kd.family = family;
+            return kd;
         }
-        if (!ok) continue;
-        if (!uniform) {
-            w.resize((size_t)n);
-            for (int64_t j = 0; j < n && ok; j++) ok = weight(j, w[(size_t)j]);
-            if (!ok) continue;
-        }
-        for (int q = 0; q < 512 && ok; q++) {
-            int64_t i = next(n), j = next(n);
-            if (i == j) continue;
-            const double f = family_value(family, r_of(i, j));
-            const double want = A(i, j), got = (uniform ? w0 : w[(size_t)j]) * f;
-            ok = std::isfinite(f) && std::fabs(got - want) <= 1e-13 * std::fabs(want) + 1e-300;
-        }
-        if (!ok) continue;
-        kd.family = family;
-        if (uniform) {
-            kd.scale = w0;
-        } else {
-            kd.scale = 1.0;
-            weights = std::move(w);
-            kd.weight = weights.data();
-        }
-        return kd;
     }
     throw hodlr::Error(HODLR_E_ARG, "kernel entries are not diag / w_j f(|x_i - x_j|) with f = log r, 1/r or "
-                                    "exp(-r); describe the kernel with device_kernel()");
+                                    "exp(-r); describe the kernel with device_kernel(), or compile its class as "
+                                    "device code (hodlrnd/device_kernels_begin.hpp ... _end.hpp, hipcc)");
 }
+
+// f = the class's own Kernel_Fun (host side of the device form); throws
+// when the entries are not diag / w_j Kernel_Fun(r)
+template <class K>
+hodlr_kernel_desc identify_kernel_fun(K *ker, const std::vector<ptsnD> &P, std::vector<double> &weights) {
+    hodlr_kernel_desc kd{HODLR_KERNEL_CUSTOM, NDIM, 1.0, 0.0, nullptr};
+    kd.diag = match_diag(ker, (int64_t)P.size());
+    if (!match_form(ker, P, [&](double r) { return (double)ker->Kernel_Fun(r); }, kd, weights))
+        throw hodlr::Error(HODLR_E_ARG, "kernel entries are not diag / w_j Kernel_Fun(|x_i - x_j|)");
+    return kd;
+}
+
+// Specialized by hodlrnd/device_kernels_end.hpp for the classes whose
+// Kernel_Fun is device code: launchers() is then their entry-kernel table.
+template <class K, class = void>
+struct device_kernel_fun {
+    static constexpr bool value = false;
+    static const void *launchers() { return nullptr; }
+};
 }  // namespace hodlr_detail
 
 // ---- GMRES for any operator (GMRES.hpp:31-303) ----
@@ -609,19 +655,42 @@ public:
     }
     size_t get_size() { return N_; }
     // the kernel's description for the device: its own device_kernel()
-    // when it has one, else identified from getMatrixEntry
-    hodlr_kernel_desc device_description() {
+    // when it has one, else identified from getMatrixEntry against the
+    // built-in families, then (device-compiled classes) against its own
+    // Kernel_Fun, in which case *entries receives its entry-kernel table;
+    // per-point weights are held by this object until the next call
+    hodlr_kernel_desc device_description(const void **entries = nullptr) {
+        if (entries) *entries = nullptr;
         if constexpr (hodlr_detail::has_device_kernel<Kernel>::value) {
             return kernel_->device_kernel();
         } else {
+            if constexpr (hodlr_detail::device_kernel_fun<Kernel>::value) {
+                try {
+                    return hodlr_detail::identify_kernel(kernel_, *points_, weights_);
+                } catch (const hodlr::Error &) {
+                }
+                hodlr_kernel_desc kd = hodlr_detail::identify_kernel_fun(kernel_, *points_, weights_);
+                if (entries) *entries = hodlr_detail::device_kernel_fun<Kernel>::launchers();
+                return kd;
+            }
             return hodlr_detail::identify_kernel(kernel_, *points_, weights_);
         }
     }
     void Assemble_matrix_operators() {
-        hodlr_kernel_desc kd = device_description();
-        check(hodlr_op_assemble_skeleton(ctx_, 1, &tree_, &kd, eps_ACA, SYS_SIZE, HODLR_PREC_F64 | HODLR_PREC_F32,
-                                         &op_),
-              "op_assemble");
+        const void *entries = nullptr;
+        hodlr_kernel_desc kd = device_description(&entries);
+        if (kd.family == HODLR_KERNEL_CUSTOM) {
+            if (!entries)
+                throw hodlr::Error(HODLR_E_ARG, "custom kernel family without device-compiled Kernel_Fun");
+            check(hodlr_op_assemble_custom(ctx_, tree_, &kd, static_cast<const void *>(kernel_),
+                                           (int64_t)sizeof(Kernel), entries, eps_ACA,
+                                           HODLR_PREC_F64 | HODLR_PREC_F32, &op_),
+                  "op_assemble");
+        } else {
+            check(hodlr_op_assemble_skeleton(ctx_, 1, &tree_, &kd, eps_ACA, SYS_SIZE,
+                                             HODLR_PREC_F64 | HODLR_PREC_F32, &op_),
+                  "op_assemble");
+        }
         std::printf("Matrix operators formed...\n");
     }
     Vec operator*(const Vec &x) {

@@ -203,12 +203,17 @@ static inline double kernel_value(const or_kernel_t *k, const double *pts,
     switch (k->family) {
     case OR_K_LOG: f = log(r); break;
     case OR_K_INV: f = 1.0 / r; break;
+    case OR_K_CUSTOM: f = k->fn(r, k->params); break;
     default: f = exp(-r); break;
     }
     double v = k->scale * f;
     if (k->weight) v = v * k->weight[j];
     return v;
 }
+
+double or_f_yukawa(double r, const double *p) { return p[0] * exp(-r) / r; }
+double or_f_matern32(double r, const double *p) { return (1.0 + p[0] * r) * exp(-p[0] * r); }
+double or_f_gauss(double r, const double *p) { return exp(-r * r / p[0]); }
 
 double or_entry(const or_kernel_t *k, const double *pts, int64_t i, int64_t j) {
     return kernel_value(k, pts, i, j);

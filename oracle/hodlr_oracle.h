@@ -32,7 +32,10 @@ extern "C" {
 #define OR_MAXD 8
 
 /* kernel families: entry(i,j) = diag if i == j else scale * f(|x_i - x_j|) * w_j */
-enum { OR_K_LOG = 0, OR_K_INV = 1, OR_K_EXP = 2 };
+enum { OR_K_LOG = 0, OR_K_INV = 1, OR_K_EXP = 2, OR_K_CUSTOM = 3 };
+
+/* f(r) of a custom family (a user kernel's Kernel_Fun, user_kernel.hpp:46,101,167) */
+typedef double (*or_fn_t)(double r, const double *params);
 
 typedef struct {
     int32_t family;
@@ -40,7 +43,15 @@ typedef struct {
     double scale;
     double diag;
     const double *weight; /* nullable, indexed by point id */
+    or_fn_t fn;           /* OR_K_CUSTOM: f(r) = fn(r, params) */
+    const double *params;
 } or_kernel_t;
+
+/* Kernel_Fun of the device-compiled kernel classes the custom-family tests
+ * use, written as those classes write them */
+double or_f_yukawa(double r, const double *p);   /* p[0] exp(-r) / r */
+double or_f_matern32(double r, const double *p); /* (1 + p[0] r) exp(-p[0] r) */
+double or_f_gauss(double r, const double *p);    /* exp(-r r / p[0]) */
 
 /* ---- generators and quadrature ---- */
 void or_uniform_nodes(double a, double b, int n, double *out);

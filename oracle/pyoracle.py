@@ -12,12 +12,13 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
 
-K_LOG, K_INV, K_EXP = 0, 1, 2
+K_LOG, K_INV, K_EXP, K_CUSTOM = 0, 1, 2, 3
 
 
 class Kernel(C.Structure):
     _fields_ = [("family", C.c_int32), ("ndim", C.c_int32), ("scale", C.c_double),
-                ("diag", C.c_double), ("weight", C.c_void_p)]
+                ("diag", C.c_double), ("weight", C.c_void_p), ("fn", C.c_void_p),
+                ("params", C.c_void_p)]
 
 
 class AcaInfo(C.Structure):
@@ -139,16 +140,26 @@ def quadruple_integral(a, b, ngauss=32):
 
 
 class KernelDesc:
-    """entry(i, j) = diag if i == j else scale * f(|x_i - x_j|) * w_j"""
+    """entry(i, j) = diag if i == j else scale * f(|x_i - x_j|) * w_j;
+    family K_CUSTOM takes f by name: one of CUSTOM_F ("yukawa", "matern32",
+    "gauss") with its parameters"""
 
-    def __init__(self, family, ndim, scale, diag, weight=None):
+    def __init__(self, family, ndim, scale, diag, weight=None, fn=None, params=None):
         self.weight = None if weight is None else f64(weight)
+        self.params = None if params is None else f64(params)
+        fptr = None
+        if family == K_CUSTOM:
+            fptr = C.cast(getattr(lib, CUSTOM_F[fn]), C.c_void_p).value
         self.c = Kernel(family, ndim, scale, diag,
-                        None if self.weight is None else self.weight.ctypes.data)
+                        None if self.weight is None else self.weight.ctypes.data, fptr,
+                        None if self.params is None else self.params.ctypes.data)
 
     @property
     def ref(self):
         return C.byref(self.c)
+
+
+CUSTOM_F = {"yukawa": "or_f_yukawa", "matern32": "or_f_matern32", "gauss": "or_f_gauss"}
 
 
 def entry(k, pts, i, j):

@@ -233,3 +233,78 @@ def test_reference_shaped_user_kernels_identified(tmp_path):
     assert log.startswith("Restarted GMRES Parameters") and "Iter[6]" in log
     assert "Reached Solution before Max_Iterations" in log and "Mat-Vec Time :" in log
     assert "Exited before any Progress" in (tmp_path / "gmres_generic_exact.txt").read_text()
+
+
+def test_custom_entry_checks_its_launcher_table(hctx):
+    """hodlr_op_assemble_custom refuses another kernel family, a launcher
+    table of another layout or with a null launcher, before any device work."""
+    import ctypes as C
+
+    class Table(C.Structure):
+        _fields_ = [("version", C.c_uint32), ("layout", C.c_uint32)] + [(n, C.c_void_p) for n in
+                                                                         ("aca", "dense_fill", "near_fill",
+                                                                          "dense_rows")]
+
+    pts = O.splitmix_uniform(5, 300 * 3).reshape(300, 3)
+    T = H.Tree(pts, -np.ones(3), np.ones(3), 32, 0, hctx)
+    h = C.c_void_p()
+    obj = (C.c_double * 2)(1.0, 2.0)
+    bad = Table(0, 0, 1, 1, 1, 1)
+
+    def call(family, table, nbytes=16):
+        kd = H.KernelDesc(family, 3, 1.0, 1.0, None)
+        return H.lib.hodlr_op_assemble_custom(hctx.h, T.h, C.byref(kd), obj, nbytes, C.byref(table), 1e-8, 3,
+                                              C.byref(h))
+
+    assert call(H.KERNEL_INV_R, bad) == H.E_ARG
+    assert "HODLR_KERNEL_CUSTOM" in H.lib.hodlr_last_error(hctx.h).decode()
+    assert call(H.KERNEL_CUSTOM, bad) == H.E_ARG
+    assert "another layout" in H.lib.hodlr_last_error(hctx.h).decode()
+    assert call(H.KERNEL_CUSTOM, bad, nbytes=-1) == H.E_ARG
+    assert H.lib.hodlr_op_assemble_custom(hctx.h, T.h, None, obj, 16, C.byref(bad), 1e-8, 3, C.byref(h)) == H.E_ARG
+
+
+def test_device_kernel_classes_compile_with_hipcc(tmp_path):
+    """Kernel classes in the reference's shape between
+    hodlrnd/device_kernels_begin.hpp and _end.hpp compile unchanged as host
+    and device code (hipcc, -Werror), the library's entry kernels
+    instantiated for each Kernel_Fun; a class with a host-only Kernel_Fun
+    used in such a program is a compile error, not a silent host fallback."""
+    src = os.path.join(ROOT, "hodlrnd_amd", "examples", "device_kernels.cpp")
+    flags = ["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-O1", "-std=c++17", "-ffp-contract=off",
+             "-Wall", "-Werror", "-Wno-unused-command-line-argument", "-I" + os.path.join(ROOT, "include")]
+    r = subprocess.run(flags + ["-c", src, "-o", str(tmp_path / "dk.o")], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    bad = tmp_path / "host_only.cpp"
+    bad.write_text('#include "hodlrnd/device_kernels_begin.hpp"\n#include "hodlrnd/device_kernels_end.hpp"\n'
+                   "struct host_only {\n  std::vector<ptsnD> *gridPoints;\n"
+                   "  double Kernel_Fun(double x) { return std::exp(-x) / x; }\n"
+                   "  double getMatrixEntry(int i, int j) { return i == j ? 1.0 : 0.5; }\n};\n"
+                   "int main() { std::vector<ptsnD> *p = new std::vector<ptsnD>(4); host_only *k = new host_only;\n"
+                   "  k->gridPoints = p; Vec a(3), b(3); HODLRdD_matrix<host_only> M(k, p, a, b);\n"
+                   "  M.Assemble_matrix_operators(); }\n")
+    r = subprocess.run(flags + ["-c", str(bad), "-o", str(tmp_path / "bad.o")], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode != 0 and "Kernel_Fun" in r.stderr
+
+
+def test_tree_build_refuses_runaway_inputs(hctx):
+    """Points outside the bounding box, an empty or non-finite box, and more
+    than nmax coincident points would make the uniform-depth split loop
+    (HDD_tree.hpp:43-69) deepen without end: each is refused with E_ARG."""
+    pts = O.splitmix_uniform(9, 300 * 3).reshape(300, 3)
+    lo, hi = -np.ones(3), np.ones(3)
+    bad = pts.copy()
+    bad[17, 2] = 1.5
+    for p, a, b, msg in [(bad, lo, hi, "point 17 lies outside"),
+                         (pts, lo, lo, "finite lo < hi"),
+                         (pts, lo, np.array([1.0, np.inf, 1.0]), "finite lo < hi"),
+                         (np.vstack([pts, np.zeros((40, 3))]), lo, hi, "coincident points")]:
+        with pytest.raises(H.HodlrError) as e:
+            H.Tree(p, a, b, 32, 0, hctx)
+        assert e.value.code == H.E_ARG and msg in str(e.value), str(e.value)
+    nan = pts.copy()
+    nan[3, 0] = np.nan
+    with pytest.raises(H.HodlrError):
+        H.Tree(nan, lo, hi, 32, 0, hctx)

@@ -430,3 +430,33 @@ def test_oracle_gmres_on_hodlr():
     x, info = op.gmres(b, tol=1e-10)
     assert info["resid"] < 1e-10
     assert np.linalg.norm(op.matvec(x) - b) / np.linalg.norm(b) < 1e-9
+
+
+def test_custom_family_pinned():
+    """The oracle's custom family (f(r) = Kernel_Fun through a function
+    pointer) against what the mathematics fixes: Matern-3/2 with s = 0 is the
+    all-ones kernel, so every admissible block is rank 1 and the HODLR
+    product is exact; Yukawa entries equal c exp(-r) / r from numpy; a
+    Gaussian of width l2 = 1e12 is 1 to 1e-12, so its ranks are those of the
+    all-ones kernel."""
+    import math
+    n = 900
+    pts = O.splitmix_uniform(77, n * 3).reshape(n, 3)
+    lo, hi = -np.ones(3), np.ones(3)
+    T = O.Tree(pts, lo, hi, 32, 0)
+    k1 = O.KernelDesc(O.K_CUSTOM, 3, 1.0 / n, 2.0, fn="matern32", params=[0.0])
+    A = O.Operator(T, k1, 1e-10)
+    r, d = A.block_ranks()
+    assert len(r) > 100 and np.all(r[d == 0] == 1) and not d.any()
+    x = O.splitmix_uniform(78, n)
+    y = A.matvec(x)
+    assert np.allclose(y, 2.0 * x + (x.sum() - x) / n, rtol=0, atol=1e-13)
+    ky = O.KernelDesc(O.K_CUSTOM, 3, 0.25, 1.0, fn="yukawa", params=[1.0 / (4 * math.pi)])
+    for i, j in [(0, 1), (5, 700), (899, 3)]:
+        rr = math.sqrt(sum((pts[i, q] - pts[j, q]) ** 2 for q in range(3)))
+        want = 0.25 * (1.0 / (4 * math.pi) * math.exp(-rr) / rr)
+        assert O.entry(ky, pts, i, j) == want
+    assert O.entry(ky, pts, 4, 4) == 1.0
+    kg = O.KernelDesc(O.K_CUSTOM, 3, 1.0 / n, 1.0, fn="gauss", params=[1e12])
+    rg, dg = O.Operator(T, kg, 1e-10).block_ranks()
+    assert np.array_equal(rg, r) and np.array_equal(dg, d)
