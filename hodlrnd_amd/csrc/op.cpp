@@ -500,8 +500,18 @@ void assemble(hodlr_op &op) {
             }
         }
     }
+    // Retries (blocks whose rank outgrew the first budget of 128) are few
+    // and large: 4-D d' = 3 puts 240 blocks of 4096 x 4096 with ranks up
+    // to ~800 there, each a single-workgroup ACA of seconds.  One launch
+    // takes all of them when memory allows (a per-launch scratch of up to a
+    // quarter of free memory instead of 4 GiB), and the staging buffer grows
+    // so their factors are copied, not recomputed, by the fill step.
+    const int64_t retry_budget =
+        op.admit ? budget : std::max<int64_t>(budget, (int64_t)(free_b / 4 / sizeof(double)));
+    const int64_t stage_room = (int64_t)(free_b / (op.admit ? 16 : 4) / sizeof(double));
     bool first_pass = true;
     while (!todo.empty()) {
+        const int64_t pass_budget = first_pass ? budget : retry_budget;
         std::vector<int64_t> big, rest;
         if (first_pass)
             for (int64_t b : todo) (aca_class(op, b) == 1024 ? big : rest).push_back(b);
@@ -521,7 +531,8 @@ void assemble(hodlr_op &op) {
                 tl_threads = threads;
                 DevBuf pool2;
                 try {
-                    run_aca_pass(op, big, cap, 0, pool2, budget, d_rank, d_flags, nullptr, s2, stage, &stageable);
+                    run_aca_pass(op, big, cap, 0, pool2, pass_budget, d_rank, d_flags, nullptr, s2, stage,
+                                 &stageable);
                     (void)hipStreamSynchronize(s2);
                 } catch (...) {
                     err2 = std::current_exception();
@@ -529,7 +540,7 @@ void assemble(hodlr_op &op) {
             });
             std::exception_ptr err1;
             try {
-                run_aca_pass(op, rest, cap, 0, pool, budget, d_rank, d_flags, nullptr, s, stage, &stageable);
+                run_aca_pass(op, rest, cap, 0, pool, pass_budget, d_rank, d_flags, nullptr, s, stage, &stageable);
             } catch (...) {
                 err1 = std::current_exception();
             }
@@ -539,7 +550,7 @@ void assemble(hodlr_op &op) {
             if (err1) std::rethrow_exception(err1);
             if (err2) std::rethrow_exception(err2);
         } else {
-            run_aca_pass(op, todo, cap, 0, pool, budget, d_rank, d_flags, nullptr, s, stage, &stageable);
+            run_aca_pass(op, todo, cap, 0, pool, pass_budget, d_rank, d_flags, nullptr, s, stage, &stageable);
         }
         std::vector<int32_t> rk(nb), fl(nb);
         if (nb) {
@@ -554,8 +565,36 @@ void assemble(hodlr_op &op) {
             if (fl[b] & 2) {  // rank budget exhausted: retry with a larger cap
                 const int mn = std::min(op.b_m[b], op.b_n[b]);
                 if (cap[b] >= mn) throw ArgError{HODLR_E_RANK, "ACA rank budget exhausted"};
-                cap[b] = std::min<int64_t>(mn, (int64_t)cap[b] * 4);
+                cap[b] = std::min<int64_t>(mn, (int64_t)cap[b] * 8);
                 again.push_back(b);
+            }
+        }
+        if (stage.buf && !again.empty()) {
+            // room for the retried blocks' factors: a larger buffer holding
+            // what is staged so far, the cursor continuing from where it is
+            int64_t extra = 0;
+            for (int64_t b : again)
+                if (stageable[b]) extra += (int64_t)cap[b] * (op.b_m[b] + op.b_n[b]);
+            unsigned long long cur = 0;
+            HIP_CHECK(hipMemcpyAsync(&cur, stage.cursor, sizeof(cur), hipMemcpyDeviceToHost, s));
+            HIP_CHECK(hipStreamSynchronize(s));
+            const int64_t want = std::min<int64_t>((int64_t)cur + extra, stage_room);
+            if (want > stage.cap) {
+                DevBuf bigger;
+                try {
+                    bigger.alloc(sizeof(double) * (size_t)want, s);
+                } catch (const ArgError &) {  // no room: those blocks are recomputed by the fill pass
+                }
+                if (bigger.p) {
+                    const int64_t keep = std::min<int64_t>((int64_t)cur, stage.cap);
+                    if (keep > 0)
+                        HIP_CHECK(hipMemcpyAsync(bigger.p, stage.buf, sizeof(double) * (size_t)keep,
+                                                 hipMemcpyDeviceToDevice, s));
+                    HIP_CHECK(hipStreamSynchronize(s));
+                    stage_buf = std::move(bigger);
+                    stage.buf = stage_buf.as<double>();
+                    stage.cap = want;
+                }
             }
         }
         todo.swap(again);
