@@ -14,7 +14,7 @@ SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libhodlrnd.so")
 OBJ = os.path.join(HERE, "_build")
 
-SOURCES = ["tree.cpp", "op.cpp", "opio.cpp", "solver.cpp", "batch_solver.cpp", "server.cpp", "capi.cpp", "aca.hip",
+SOURCES = ["tree.cpp", "op.cpp", "opio.cpp", "vecio.cpp", "solver.cpp", "batch_solver.cpp", "server.cpp", "capi.cpp", "aca.hip",
            "matvec.hip", "blas.hip", "skeleton.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,7 +1,6 @@
-// opio.cpp -- operator files and cereal vector files.
+// opio.cpp -- operator files (the cereal vector files are in vecio.cpp).
 //
-// The reference keeps operators in memory only and stores vectors with
-// cereal's portable binary archive (myHeaders.hpp:146-170).  An operator
+// The reference keeps operators in memory only.  An operator
 // file lets a compressed operator outlive the process (a restarted server
 // reloads its resident operators without repeating ACA).  Layout, all
 // little-endian:
@@ -259,35 +258,3 @@ hodlr_op *load_op(hodlr_ctx *ctx, const char *path, int prec_flags) {
 }
 
 }  // namespace hodlr
-
-// ---- cereal portable-binary std::vector<double> (myHeaders.hpp:146-170):
-//      one byte 0x01 (little-endian archive), u64 count, the doubles ----
-extern "C" int hodlr_vec_save(const char *path, int64_t n, const double *v) {
-    if (!path || n < 0 || (n > 0 && !v)) return HODLR_E_ARG;
-    FILE *f = std::fopen(path, "wb");
-    if (!f) return HODLR_E_ARG;
-    const unsigned char flag = 1;
-    const uint64_t cnt = (uint64_t)n;
-    bool ok = std::fwrite(&flag, 1, 1, f) == 1 && std::fwrite(&cnt, 8, 1, f) == 1 &&
-              (n == 0 || std::fwrite(v, sizeof(double), (size_t)n, f) == (size_t)n);
-    ok = (std::fclose(f) == 0) && ok;
-    return ok ? HODLR_OK : HODLR_E_ARG;
-}
-
-extern "C" int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n) {
-    if (!path || !n || cap < 0) return HODLR_E_ARG;
-    FILE *f = std::fopen(path, "rb");
-    if (!f) return HODLR_E_ARG;
-    unsigned char flag = 0;
-    uint64_t cnt = 0;
-    int rc = HODLR_OK;
-    if (std::fread(&flag, 1, 1, f) != 1 || flag != 1 || std::fread(&cnt, 8, 1, f) != 1) {
-        rc = HODLR_E_ARG;
-    } else {
-        *n = (int64_t)cnt;
-        if (v && (int64_t)cnt > cap) rc = HODLR_E_ARG;
-        else if (v && cnt && std::fread(v, sizeof(double), (size_t)cnt, f) != (size_t)cnt) rc = HODLR_E_ARG;
-    }
-    std::fclose(f);
-    return rc;
-}

@@ -17,8 +17,11 @@
  *     :243-250 dereference unallocated factors);
  *   - a rank-0 skeleton block contributes nothing (LowRank_matrix.hpp:694-698
  *     leaves the QR pointer unset).
- * Residual updates in ACA use fused multiply-add, exactly as the GPU kernels
- * do, so both sides round identically.
+ * Residual updates in ACA round the product, then the difference (no fused
+ * multiply-add: the reference's Eigen row - u * v, LowRank_matrix.hpp:130,
+ * 240, compiles without -mfma, and this file is built with
+ * -ffp-contract=off); the GPU kernels round the same way, so both sides
+ * agree bit for bit.
  */
 #ifndef HODLR_ORACLE_H
 #define HODLR_ORACLE_H
