@@ -207,30 +207,57 @@ hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t
 namespace hodlr {
 
 // partial[(chunk * nv + v) * K + c] = sum over the chunk of V_v[row, c] * w[row, c]
+// The batched kernels below read n x K interleaved blocks (element (row, c)
+// at row * K + c) with lanes mapped to columns: thread t owns column t % K
+// and row offset t / K of each step of 256 / K rows, so one wave load reads
+// 64 consecutive elements (512 bytes).  A column's arithmetic never depends
+// on the other columns.
+
+// sum of x over the lanes of a wave that share lane % K (K a power of two,
+// at most 32), in a fixed butterfly order; every such lane gets the sum
+template <int K>
+__device__ __forceinline__ double col_sum(double x) {
+#pragma unroll
+    for (int m = 32; m >= K; m >>= 1) x += __shfl_xor(x, m);
+    return x;
+}
+
+// partial[(chunk * nv + v) * K + c] = sum over the chunk's rows of V_v[row, c] * w[row, c].
+// Basis vectors are taken G at a time, so w is read once per G of them.
 template <int K>
 __global__ __launch_bounds__(256) void bmdot_kernel(const double *const *__restrict__ V, int nv,
                                                     const double *__restrict__ w, const Chunk *__restrict__ chunks,
                                                     double *__restrict__ partial) {
-    __shared__ double red[4][K];
+    constexpr int RP = 256 / K, G = 8;
+    __shared__ double red[G][4][K];
     const Chunk C = chunks[blockIdx.x];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    for (int v = 0; v < nv; v++) {
-        const double *vv = V[v];
-        double acc[K];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, c = tid % K, ro = tid / K;
+    for (int v0 = 0; v0 < nv; v0 += G) {
+        const int gn = min(G, nv - v0);
+        const double *vp[G];
 #pragma unroll
-        for (int c = 0; c < K; c++) acc[c] = 0.0;
-        for (int r = tid; r < C.rows; r += 256) {
-            const int64_t base = (C.row0 + r) * K;
+        for (int g = 0; g < G; g++) vp[g] = V[v0 + min(g, gn - 1)];
+        double acc[G];
 #pragma unroll
-            for (int c = 0; c < K; c++) acc[c] = fma(vv[base + c], w[base + c], acc[c]);
+        for (int g = 0; g < G; g++) acc[g] = 0.0;
+        for (int r = ro; r < C.rows; r += RP) {
+            const int64_t e = (C.row0 + r) * K + c;
+            const double wv = w[e];
+#pragma unroll
+            for (int g = 0; g < G; g++)
+                if (g < gn) acc[g] = fma(vp[g][e], wv, acc[g]);
         }
 #pragma unroll
-        for (int c = 0; c < K; c++) {
-            const double sum = wave_sum(acc[c]);
-            if (lane == 0) red[wid][c] = sum;
+        for (int g = 0; g < G; g++) {
+            const double sw = col_sum<K>(acc[g]);
+            if (g < gn && lane < K) red[g][wid][lane] = sw;
         }
         __syncthreads();
-        if (tid < K) partial[((int64_t)blockIdx.x * nv + v) * K + tid] = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+        for (int q = tid; q < gn * K; q += 256) {
+            const int g = q / K, cc = q % K;
+            partial[((int64_t)blockIdx.x * nv + v0 + g) * K + cc] =
+                (red[g][0][cc] + red[g][1][cc]) + (red[g][2][cc] + red[g][3][cc]);
+        }
         __syncthreads();
     }
 }
@@ -249,26 +276,23 @@ __global__ __launch_bounds__(256) void bfinish_kernel(const double *__restrict__
 }
 
 // w[row, c] += sign * sum_v coef[(prob * nv + v) * K + c] * V_v[row, c]
+// (per element: w first, then the basis vectors in order).  The chunk's
+// nv x K coefficients sit in dynamic LDS.
 template <int K>
 __global__ __launch_bounds__(256) void bgemv_kernel(const double *const *__restrict__ V, int nv,
                                                     const double *__restrict__ coef, double sign,
                                                     double *__restrict__ w, const Chunk *__restrict__ chunks) {
-    __shared__ double cs[128 * K];
+    constexpr int RP = 256 / K;
+    extern __shared__ double cs[];
     const Chunk C = chunks[blockIdx.x];
     for (int e = threadIdx.x; e < nv * K; e += blockDim.x) cs[e] = sign * coef[(int64_t)C.prob * nv * K + e];
     __syncthreads();
-    for (int r = threadIdx.x; r < C.rows; r += 256) {
-        const int64_t base = (C.row0 + r) * K;
-        double acc[K];
-#pragma unroll
-        for (int c = 0; c < K; c++) acc[c] = w[base + c];
-        for (int v = 0; v < nv; v++) {
-            const double *vv = V[v];
-#pragma unroll
-            for (int c = 0; c < K; c++) acc[c] = fma(cs[v * K + c], vv[base + c], acc[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < K; c++) w[base + c] = acc[c];
+    const int c = threadIdx.x % K, ro = threadIdx.x / K;
+    for (int r = ro; r < C.rows; r += RP) {
+        const int64_t e = (C.row0 + r) * K + c;
+        double acc = w[e];
+        for (int v = 0; v < nv; v++) acc = fma(cs[v * K + c], V[v][e], acc);
+        w[e] = acc;
     }
 }
 
@@ -276,14 +300,13 @@ __global__ __launch_bounds__(256) void bgemv_kernel(const double *const *__restr
 template <int K>
 __global__ __launch_bounds__(256) void bscale_kernel(const double *__restrict__ src, const double *__restrict__ coef,
                                                      double *__restrict__ dst, const Chunk *__restrict__ chunks) {
+    constexpr int RP = 256 / K;
     const Chunk C = chunks[blockIdx.x];
-    double cf[K];
-#pragma unroll
-    for (int c = 0; c < K; c++) cf[c] = coef[(int64_t)C.prob * K + c];
-    for (int r = threadIdx.x; r < C.rows; r += 256) {
-        const int64_t base = (C.row0 + r) * K;
-#pragma unroll
-        for (int c = 0; c < K; c++) dst[base + c] = src[base + c] * cf[c];
+    const int c = threadIdx.x % K, ro = threadIdx.x / K;
+    const double cf = coef[(int64_t)C.prob * K + c];
+    for (int r = ro; r < C.rows; r += RP) {
+        const int64_t e = (C.row0 + r) * K + c;
+        dst[e] = src[e] * cf;
     }
 }
 
@@ -297,7 +320,7 @@ __global__ __launch_bounds__(256) void bscale_kernel(const double *__restrict__ 
 
 hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const Chunk *chunks, int nchunk,
                         double *partial, const int32_t *pchunk, int nprob, int K, double *out, hipStream_t s) {
-    if (nv > 128) return hipErrorInvalidValue;
+    if (nv < 0 || nv > 128) return hipErrorInvalidValue;
     switch (K) {
 #define CASE(KK) \
     case KK: bmdot_kernel<KK><<<dim3((unsigned)nchunk), 256, 0, s>>>(V, nv, w, chunks, partial); break;
@@ -312,10 +335,14 @@ hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const C
 
 hipError_t launch_bgemv(const double *const *V, int nv, const double *coef, double sign, double *w,
                         const Chunk *chunks, int nchunk, int K, hipStream_t s) {
-    if (nv > 128 || nv * K > 128 * K) return hipErrorInvalidValue;
+    if (nv < 0 || nv > 128) return hipErrorInvalidValue;
+    if (nv == 0) return hipSuccess;  // w unchanged
     switch (K) {
 #define CASE(KK) \
-    case KK: bgemv_kernel<KK><<<dim3((unsigned)nchunk), 256, 0, s>>>(V, nv, coef, sign, w, chunks); break;
+    case KK:                                                                                               \
+        bgemv_kernel<KK><<<dim3((unsigned)nchunk), 256, sizeof(double) * (size_t)nv * KK, s>>>(V, nv, coef, sign, w, \
+                                                                                               chunks);              \
+        break;
         HODLR_BATCH_K(CASE)
 #undef CASE
     default:
