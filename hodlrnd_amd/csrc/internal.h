@@ -139,15 +139,19 @@ struct DevBuf {
 // waited for, so v may go away); temp = true makes the buffer itself
 // stream-ordered.
 template <class T>
-void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s = nullptr, bool temp = false) {
-    b.alloc(v.size() * sizeof(T), temp ? s : nullptr);
-    if (v.empty()) return;
+void upload(DevBuf &b, const T *v, size_t n, hipStream_t s = nullptr, bool temp = false) {
+    b.alloc(n * sizeof(T), temp ? s : nullptr);
+    if (n == 0) return;
     if (s) {
-        HIP_CHECK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(b.p, v, n * sizeof(T), hipMemcpyHostToDevice, s));
         HIP_CHECK(hipStreamSynchronize(s));
     } else {
-        HIP_CHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(b.p, v, n * sizeof(T), hipMemcpyHostToDevice));
     }
+}
+template <class T>
+void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s = nullptr, bool temp = false) {
+    upload(b, v.data(), v.size(), s, temp);
 }
 
 }  // namespace hodlr

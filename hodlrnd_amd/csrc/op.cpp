@@ -26,6 +26,7 @@
 #include "op.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <thread>
@@ -33,6 +34,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <memory>
 
 namespace hodlr {
 
@@ -95,6 +97,25 @@ void parallel_for(int64_t n, F &&fn, int64_t grain = 1 << 16) {
         if (a < b) th.emplace_back([&fn, a, b] { fn(a, b); });
     }
     for (auto &x : th) x.join();
+}
+
+// A host table every entry of which is written before it is read: unlike a
+// std::vector it is not value-initialised on allocation (layout tables run
+// to GBs and are written by parallel passes).
+template <class T>
+struct Table {
+    std::unique_ptr<T[]> p;
+    size_t n = 0;
+    explicit Table(size_t count) : p(new T[count > 0 ? count : 1]), n(count) {}
+    T &operator[](size_t i) { return p[i]; }
+    const T &operator[](size_t i) const { return p[i]; }
+    T *data() { return p.get(); }
+    const T *data() const { return p.get(); }
+    size_t size() const { return n; }
+};
+template <class T>
+void upload(DevBuf &b, const Table<T> &v, hipStream_t s = nullptr, bool temp = false) {
+    upload(b, v.data(), v.size(), s, temp);
 }
 
 uint64_t block_seed(int level, int64_t row_node, int64_t col_node) {
@@ -359,8 +380,21 @@ void assemble(hodlr_op &op) {
         upload(op.kdev, kd, s);
         std::vector<int64_t> ls(N > 0 ? N : 1, 0);
         std::vector<int32_t> lc(N > 0 ? N : 1, 0);
-        std::vector<int32_t> ord;
         op.ord_off.assign(P, 0);
+        int64_t ord_total = 0;
+        for (int p = 0; p < P; p++) {
+            op.ord_off[p] = ord_total;
+            ord_total += (int64_t)op.trees[p]->nlevels * op.trees[p]->n;
+        }
+        std::vector<int32_t> ord((size_t)ord_total);
+        // one sort per inner node with two or more points (leaves keep
+        // ascending ids); nodes own disjoint ranges and ids are distinct, so
+        // the sorts run in parallel with the result of a sequential pass
+        struct SortTask {
+            int p, l;
+            int64_t g;
+        };
+        std::vector<SortTask> sorts;
         for (int p = 0; p < P; p++) {
             const hodlr_tree &t = *op.trees[p];
             const int L = t.nlevels - 1;
@@ -372,19 +406,27 @@ void assemble(hodlr_op &op) {
                     lc[pb + q] = (int32_t)t.count[g];
                 }
             }
-            op.ord_off[p] = (int64_t)ord.size();
-            ord.resize(ord.size() + (size_t)t.nlevels * t.n);
             for (int l = 0; l <= L; l++) {
                 int32_t *o = ord.data() + op.ord_off[p] + (size_t)l * t.n;
                 for (int64_t q = 0; q < t.n; q++) o[q] = (int32_t)(pb + q);
-                if (l == L) continue;  // leaves are already ascending
-                for (int64_t j = 0; j < t.level_size(l); j++) {
-                    const int64_t g = t.node(l, j);
-                    std::sort(o + t.start[g], o + t.start[g] + t.count[g],
-                              [&](int32_t a, int32_t b) { return t.perm[a - pb] < t.perm[b - pb]; });
-                }
+                if (l == L) continue;
+                for (int64_t j = 0; j < t.level_size(l); j++)
+                    if (t.count[t.node(l, j)] > 1) sorts.push_back(SortTask{p, l, t.node(l, j)});
             }
         }
+        parallel_for(
+            (int64_t)sorts.size(),
+            [&](int64_t k0, int64_t k1) {
+                for (int64_t k = k0; k < k1; k++) {
+                    const SortTask &T = sorts[k];
+                    const hodlr_tree &t = *op.trees[T.p];
+                    const int64_t pb = op.pos_base[T.p];
+                    int32_t *o = ord.data() + op.ord_off[T.p] + (size_t)T.l * t.n;
+                    std::sort(o + t.start[T.g], o + t.start[T.g] + t.count[T.g],
+                              [&](int32_t a, int32_t b) { return t.perm[a - pb] < t.perm[b - pb]; });
+                }
+            },
+            16);
         upload(op.leaf_start, ls, s);
         upload(op.leaf_cnt, lc, s);
         upload(op.ord, ord, s);
@@ -666,6 +708,7 @@ void assemble(hodlr_op &op) {
         op.e_v += (double)count[g] * Tb[g];
     }
     op.e_u = (double)op.n_up;
+    mark("panel sizes");
 
     // skeleton slots: the pivot rows of the blocks (., c) of column node c
     // are contiguous (row slots), as are the pivot columns of the blocks
@@ -709,141 +752,259 @@ void assemble(hodlr_op &op) {
         }
     }
 
-    // near field and panel work per non-empty leaf
-    std::vector<NearLeaf> near;
-    std::vector<int64_t> colsrc;
-    std::vector<Segment> segs;
-    std::vector<QuadDesc> quads;
-    std::vector<PanelItem> panel;
-    op.n_np = 0;
+    // near field and panel work per non-empty leaf.  Each row node's blocks
+    // are cut into runs once: a run of low-rank blocks multiplies their
+    // contiguous t entries, a run of dense blocks whose column nodes are
+    // adjacent multiplies x directly; every leaf below the node takes the
+    // runs as segments of its slice of the node's U panel.  The leaves are
+    // laid out in two parallel passes (sizes, then entries at prefix-sum
+    // offsets), so the tables are the same as one sequential walk's.
+    struct Run {
+        int64_t coff, vec;
+        int32_t len, arena;
+    };
+    std::vector<int64_t> run_ptr(nn + 1, 0);
+    std::vector<Run> runs;
+    {
+        auto cut = [&](int64_t ga, Run *out) {  // runs of row node ga (written when out is set)
+            int64_t nr = 0;
+            Run cur{0, 0, 0, -1};
+            for (int64_t e = rb_ptr[ga]; e < rb_ptr[ga + 1]; e++) {
+                const int64_t b = rb_list[e];
+                if (!width[b]) continue;
+                const bool dn = op.flags[b] & 1;
+                const int arena = dn ? 2 : 1;
+                const int64_t vec = dn ? op.node_start[op.b_col[b]] : t_base[ga] + tcoff[b];
+                if (cur.arena == arena && cur.vec + cur.len == vec) {
+                    cur.len += width[b];
+                    continue;
+                }
+                if (cur.arena >= 0) {
+                    if (out) out[nr] = cur;
+                    nr++;
+                }
+                cur = Run{(int64_t)coff[b], vec, width[b], arena};
+            }
+            if (cur.arena >= 0) {
+                if (out) out[nr] = cur;
+                nr++;
+            }
+            return nr;
+        };
+        parallel_for(
+            nn,
+            [&](int64_t g0, int64_t g1) {
+                for (int64_t g = g0; g < g1; g++) run_ptr[g + 1] = R[g] ? cut(g, nullptr) : 0;
+            },
+            1024);
+        for (int64_t g = 0; g < nn; g++) run_ptr[g + 1] += run_ptr[g];
+        runs.resize((size_t)run_ptr[nn]);
+        parallel_for(
+            nn,
+            [&](int64_t g0, int64_t g1) {
+                for (int64_t g = g0; g < g1; g++)
+                    if (R[g]) cut(g, runs.data() + run_ptr[g]);
+            },
+            1024);
+    }
+    struct LeafRef {
+        int64_t g, j;
+        int p;
+    };
+    std::vector<LeafRef> leaves;
     for (int p = 0; p < P; p++) {
         const hodlr_tree &t = *op.trees[p];
-        const int L = t.nlevels - 1, d = t.ndim;
+        const int L = t.nlevels - 1;
         for (int64_t j = 0; j < t.level_size(L); j++) {
             const int64_t g = node_base[p] + t.node(L, j);
-            const int64_t nc = count[g];
-            if (!nc) continue;
-            NearLeaf nl;
-            nl.np_base = op.n_np;
-            nl.row0 = op.node_start[g];
-            nl.colsrc = (int64_t)colsrc.size();
-            nl.nc = (int32_t)nc;
-            nl.prob = p;
-            nl.pad = 0;
-            const int64_t seg0 = (int64_t)segs.size();
-            int64_t ncols = 0;
-            // one segment per non-empty neighbour block (self first), each
-            // multiplying a contiguous slice of x
-            auto add_near = [&](int64_t gb) {
-                const int64_t c = count[gb];
-                if (!c) return;
-                if ((int64_t)segs.size() > seg0 && segs.back().arena == 0 &&
-                    segs.back().vec + segs.back().len == op.node_start[gb]) {
-                    segs.back().len += (int32_t)c;  // neighbour adjacent in x: same segment
-                } else {
-                    Segment sg;
-                    sg.mat = op.n_np + ncols * nc;
-                    sg.vec = op.node_start[gb];
-                    sg.len = (int32_t)c;
-                    sg.arena = 0;
-                    segs.push_back(sg);
-                }
-                for (int64_t q = op.node_start[gb]; q < op.node_start[gb] + c; q++) colsrc.push_back(q);
-                ncols += c;
-            };
-            add_near(g);
-            const int64_t gl = t.node(L, j);
-            for (int64_t a = t.nbr_ptr[gl]; a < t.nbr_ptr[gl + 1]; a++) add_near(node_base[p] + t.node(L, t.nbr[a]));
-            nl.ncols = (int32_t)ncols;
-            near.push_back(nl);
-            op.n_np += nc * ncols;
-            // far field: the leaf's slice of the U panel of every ancestor
-            // (and of the leaf itself), cut into segments: a run of low-rank
-            // blocks multiplies their contiguous t entries, a run of dense
-            // blocks whose column nodes are adjacent multiplies x directly
-            for (int l = 1; l <= L; l++) {
-                const int64_t ga = node_base[p] + t.node(l, j >> (d * (L - l)));
-                if (!R[ga]) continue;
-                const int64_t slice = up_base[ga] + R[ga] * (op.node_start[g] - op.node_start[ga]);
-                Segment cur{0, 0, 0, -1};
-                for (int64_t e = rb_ptr[ga]; e < rb_ptr[ga + 1]; e++) {
-                    const int64_t b = rb_list[e];
-                    if (!width[b]) continue;
-                    const bool dn = op.flags[b] & 1;
-                    const int arena = dn ? 2 : 1;
-                    const int64_t vec = dn ? op.node_start[op.b_col[b]] : t_base[ga] + tcoff[b];
-                    if (cur.arena == arena && cur.vec + cur.len == vec) {
-                        cur.len += width[b];
-                        continue;
-                    }
-                    if (cur.arena >= 0) segs.push_back(cur);
-                    cur.mat = slice + (int64_t)coff[b] * nc;
-                    cur.vec = vec;
-                    cur.len = width[b];
-                    cur.arena = arena;
-                }
-                if (cur.arena >= 0) segs.push_back(cur);
-            }
-            const int64_t nseg = (int64_t)segs.size() - seg0;
-            // the leaf's quads of columns (shared by its row tiles)
-            const int64_t q0 = (int64_t)quads.size();
-            for (int64_t q = seg0; q < seg0 + nseg; q++) {
-                const Segment &S = segs[q];
-                for (int32_t c = 0; c < S.len; c += 4) {
-                    QuadDesc qd;
-                    qd.mat = S.mat + (int64_t)c * nc;
-                    if (S.vec + c > INT32_MAX) throw ArgError{HODLR_E_ARG, "vector offset beyond 32 bits"};
-                    qd.vec = (int32_t)(S.vec + c);
-                    qd.info = S.arena | (std::min<int32_t>(4, S.len - c) << 8);
-                    quads.push_back(qd);
-                }
-            }
-            for (int64_t r0 = 0; r0 < nc; r0 += 64) {
-                PanelItem pi;
-                pi.y0 = op.node_start[g] + r0;
-                pi.seg0 = seg0;
-                pi.nseg = (int32_t)nseg;
-                pi.h = (int32_t)std::min<int64_t>(64, nc - r0);
-                pi.ld = (int32_t)nc;
-                pi.r0 = (int32_t)r0;
-                const int G = 64 / pi.h;
-                int64_t n4 = 0, ng = 0;
-                for (int64_t q = seg0; q < seg0 + nseg; q++) {
-                    n4 += (segs[q].len + 3) / 4;
-                    ng += (segs[q].len + G - 1) / G;
-                }
-                if (n4 > INT32_MAX || ng > INT32_MAX) throw ArgError{HODLR_E_ARG, "leaf panel too wide"};
-                pi.nq4 = (int32_t)n4;
-                pi.nqg = (int32_t)ng;
-                pi.q0 = q0;
-                panel.push_back(pi);
-            }
+            if (count[g]) leaves.push_back(LeafRef{g, j, p});
         }
     }
+    const int64_t nleaf = (int64_t)leaves.size();
+    // The leaf's segments, passed to emit(index, segment) in order: its
+    // near-field blocks (self first, then its neighbours; a neighbour
+    // adjacent in x extends the previous segment), then the runs of every
+    // ancestor's U panel (and of the leaf's own).  Column sources are
+    // written when columns is set.  Returns the segment count.
+    auto leaf_segments = [&](const LeafRef &lf, int64_t np_base, int64_t *columns, int64_t &ncols, auto &&emit) {
+        const hodlr_tree &t = *op.trees[lf.p];
+        const int L = t.nlevels - 1, d = t.ndim;
+        const int64_t nc = count[lf.g];
+        int64_t ns = 0;
+        Segment pend{0, 0, 0, -1};  // the near segment still open
+        ncols = 0;
+        auto add_near = [&](int64_t gb) {
+            const int64_t c = count[gb];
+            if (!c) return;
+            if (pend.arena == 0 && pend.vec + pend.len == op.node_start[gb]) {
+                pend.len += (int32_t)c;
+            } else {
+                if (pend.arena == 0) emit(ns++, pend);
+                pend = Segment{np_base + ncols * nc, op.node_start[gb], (int32_t)c, 0};
+            }
+            if (columns)
+                for (int64_t q = 0; q < c; q++) columns[ncols + q] = op.node_start[gb] + q;
+            ncols += c;
+        };
+        add_near(lf.g);
+        const int64_t gl = t.node(L, lf.j);
+        for (int64_t a = t.nbr_ptr[gl]; a < t.nbr_ptr[gl + 1]; a++) add_near(node_base[lf.p] + t.node(L, t.nbr[a]));
+        if (pend.arena == 0) emit(ns++, pend);
+        for (int l = 1; l <= L; l++) {
+            const int64_t ga = node_base[lf.p] + t.node(l, lf.j >> (d * (L - l)));
+            if (!R[ga]) continue;
+            const int64_t slice = up_base[ga] + R[ga] * (op.node_start[lf.g] - op.node_start[ga]);
+            for (int64_t e = run_ptr[ga]; e < run_ptr[ga + 1]; e++)
+                emit(ns++, Segment{slice + runs[e].coff * nc, runs[e].vec, runs[e].len, runs[e].arena});
+        }
+        return ns;
+    };
+    // pass 1: sizes per leaf (segments, column sources, near entries, quads,
+    // panel items) and the 32-bit limits of the quad and panel tables
+    std::vector<int64_t> l_seg(nleaf + 1, 0), l_col(nleaf + 1, 0), l_np(nleaf + 1, 0), l_quad(nleaf + 1, 0),
+        l_item(nleaf + 1, 0);
+    std::atomic<int> too_wide{0};
+    parallel_for(
+        nleaf,
+        [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; i++) {
+                const LeafRef &lf = leaves[i];
+                const int64_t nc = count[lf.g];
+                // row tiles: 64-row tiles (G = 1), then one of h_last rows
+                const int h_last = (int)(nc - (nc - 1) / 64 * 64), G_last = 64 / h_last;
+                int64_t n4 = 0, ng_full = 0, ng_last = 0, ncols = 0;
+                const int64_t ns = leaf_segments(lf, 0, nullptr, ncols, [&](int64_t, const Segment &S) {
+                    n4 += (S.len + 3) / 4;
+                    if (S.vec + 4 * ((S.len - 1) / 4) > INT32_MAX) too_wide = 1;
+                    ng_full += S.len;
+                    ng_last += (S.len + G_last - 1) / G_last;
+                });
+                if (n4 > INT32_MAX || (nc > 64 && ng_full > INT32_MAX) || ng_last > INT32_MAX) too_wide = 2;
+                l_seg[i + 1] = ns;
+                l_col[i + 1] = ncols;
+                l_np[i + 1] = nc * ncols;
+                l_quad[i + 1] = n4;
+                l_item[i + 1] = (nc + 63) / 64;
+            }
+        },
+        256);
+    if (too_wide == 1) throw ArgError{HODLR_E_ARG, "vector offset beyond 32 bits"};
+    if (too_wide == 2) throw ArgError{HODLR_E_ARG, "leaf panel too wide"};
+    for (int64_t i = 0; i < nleaf; i++) {
+        l_seg[i + 1] += l_seg[i];
+        l_col[i + 1] += l_col[i];
+        l_np[i + 1] += l_np[i];
+        l_quad[i + 1] += l_quad[i];
+        l_item[i + 1] += l_item[i];
+    }
+    Table<NearLeaf> near((size_t)nleaf);
+    Table<int64_t> colsrc((size_t)l_col[nleaf]);
+    Table<Segment> segs((size_t)l_seg[nleaf]);
+    Table<QuadDesc> quads((size_t)l_quad[nleaf]);
+    Table<PanelItem> panel((size_t)l_item[nleaf]);
+    // pass 2: the entries
+    parallel_for(
+        nleaf,
+        [&](int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; i++) {
+                const LeafRef &lf = leaves[i];
+                const int64_t nc = count[lf.g];
+                const int64_t seg0 = l_seg[i];
+                Segment *S = segs.data() + seg0;
+                int64_t ncols = 0;
+                const int64_t nseg = leaf_segments(lf, l_np[i], colsrc.data() + l_col[i], ncols,
+                                                   [&](int64_t q, const Segment &sg) { S[q] = sg; });
+                NearLeaf nl;
+                nl.np_base = l_np[i];
+                nl.row0 = op.node_start[lf.g];
+                nl.colsrc = l_col[i];
+                nl.nc = (int32_t)nc;
+                nl.ncols = (int32_t)ncols;
+                nl.prob = lf.p;
+                nl.pad = 0;
+                near[i] = nl;
+                // the leaf's quads of columns (shared by its row tiles)
+                QuadDesc *qd = quads.data() + l_quad[i];
+                for (int64_t q = 0; q < nseg; q++)
+                    for (int32_t c = 0; c < S[q].len; c += 4) {
+                        qd->mat = S[q].mat + (int64_t)c * nc;
+                        qd->vec = (int32_t)(S[q].vec + c);
+                        qd->info = S[q].arena | (std::min<int32_t>(4, S[q].len - c) << 8);
+                        qd++;
+                    }
+                PanelItem *pi = panel.data() + l_item[i];
+                for (int64_t r0 = 0; r0 < nc; r0 += 64, pi++) {
+                    pi->y0 = op.node_start[lf.g] + r0;
+                    pi->seg0 = seg0;
+                    pi->nseg = (int32_t)nseg;
+                    pi->h = (int32_t)std::min<int64_t>(64, nc - r0);
+                    pi->ld = (int32_t)nc;
+                    pi->r0 = (int32_t)r0;
+                    const int G = 64 / pi->h;
+                    int64_t n4 = 0, ng = 0;
+                    for (int64_t q = 0; q < nseg; q++) {
+                        n4 += (S[q].len + 3) / 4;
+                        ng += (S[q].len + G - 1) / G;
+                    }
+                    pi->nq4 = (int32_t)n4;
+                    pi->nqg = (int32_t)ng;
+                    pi->q0 = l_quad[i];
+                }
+            }
+        },
+        256);
+    op.n_np = l_np[nleaf];
     op.e_near = (double)op.n_np;
+    mark("leaf panels");
 
     // projection tiles over the V panels; t index of every V column
     // t position of every V column (32-bit: read once per column per matvec)
-    std::vector<int32_t> tidx(nvcol > 0 ? nvcol : 1, 0);
+    // every entry is written below: the low-rank blocks' columns cover
+    // [0, Tb) of each V panel, the padding columns get 0
+    Table<int32_t> tidx(nvcol > 0 ? nvcol : 1);
+    Table<FillDest> dest(nb > 0 ? nb : 1);
+    if (nvcol == 0) tidx[0] = 0;
+    if (nb == 0) std::memset(dest.data(), 0, sizeof(FillDest));
+    parallel_for(
+        nn,
+        [&](int64_t g0, int64_t g1) {
+            for (int64_t g = g0; g < g1; g++)
+                for (int64_t k = Tb[g]; k < Tpad[g]; k++) tidx[vcol_base[g] + k] = 0;
+        },
+        4096);
+    std::atomic<int> t_over{0};
+    parallel_for(nb, [&](int64_t b0, int64_t b1) {
+        for (int64_t b = b0; b < b1; b++) {
+            const int64_t a = op.b_row[b], c = op.b_col[b];
+            FillDest fd;
+            fd.up_base = up_base[a];
+            fd.R = (int32_t)R[a];
+            fd.coff = coff[b];
+            fd.vp_base = vp_base[c];
+            fd.T = (int32_t)Tpad[c];
+            fd.voff = voff[b];
+            fd.sk_row = skel[b] ? skr_off[b] : -1;
+            fd.sk_col = skel[b] ? op.n_sk + skc_off[b] : -1;
+            dest[b] = fd;
+            if (!(op.flags[b] & 1) && !skel[b]) {
+                if (t_base[a] + tcoff[b] + op.rank[b] > INT32_MAX) {
+                    t_over = 1;
+                    continue;
+                }
+                for (int k = 0; k < op.rank[b]; k++)
+                    tidx[vcol_base[c] + voff[b] + k] = (int32_t)(t_base[a] + tcoff[b] + k);
+            }
+        }
+    });
+    if (t_over) throw ArgError{HODLR_E_ARG, "more than 2^31 low-rank coefficients per right-hand side"};
     std::vector<DenseBlock> dense;
-    std::vector<FillDest> dest(nb > 0 ? nb : 1);
     op.t_lowrank = 0;
     for (int64_t b = 0; b < nb; b++) {
-        const int64_t a = op.b_row[b], c = op.b_col[b];
-        FillDest fd;
-        fd.up_base = up_base[a];
-        fd.R = (int32_t)R[a];
-        fd.coff = coff[b];
-        fd.vp_base = vp_base[c];
-        fd.T = (int32_t)Tpad[c];
-        fd.voff = voff[b];
-        fd.sk_row = skel[b] ? skr_off[b] : -1;
-        fd.sk_col = skel[b] ? op.n_sk + skc_off[b] : -1;
-        dest[b] = fd;
         if (op.flags[b] & 1) {
+            const int64_t a = op.b_row[b];
             DenseBlock db;
             db.row0 = op.node_start[a];
-            db.col0 = op.node_start[c];
+            db.col0 = op.node_start[op.b_col[b]];
             db.up_base = up_base[a];
             db.m = op.b_m[b];
             db.n = op.b_n[b];
@@ -853,13 +1014,10 @@ void assemble(hodlr_op &op) {
             db.pad = 0;
             dense.push_back(db);
         } else if (!skel[b]) {
-            if (t_base[a] + tcoff[b] + op.rank[b] > INT32_MAX)
-                throw ArgError{HODLR_E_ARG, "more than 2^31 low-rank coefficients per right-hand side"};
-            for (int k = 0; k < op.rank[b]; k++)
-                tidx[vcol_base[c] + voff[b] + k] = (int32_t)(t_base[a] + tcoff[b] + k);
             op.t_lowrank += op.rank[b];
         }
     }
+    mark("block dests");
     std::vector<ProjItem> proj;
     std::vector<ProjReduce> red;
     int64_t slots = 0;
@@ -961,6 +1119,7 @@ void assemble(hodlr_op &op) {
     }
     if (nvcol > INT32_MAX) throw ArgError{HODLR_E_ARG, "too many V columns for 32-bit tile offsets"};
     op.n_slots = slots;
+    mark("proj items");
 
     // skeleton work: projection items per column node (kSkelTile pivot rows
     // x up to kSkelChunk sources), expand items per row node (kSkelTile
