@@ -7,14 +7,21 @@ Workload (one MI355X): NDIM = 3, N = 1,000,000 points on the 100^3 uniform
 grid of [-1,1]^3 (uniform_nodes, dimension 0 outermost), second-kind kernel
 A_ij = delta_ij + (1/N) / |x_i - x_j|, Nmax = 128, eps_ACA = 1e-8, d' = 0,
 one uniform [-1,1] right-hand side (SplitMix64), GMRES(100) from x0 = b to a
-relative residual of 1e-8.  A step is one solve with the compressed operator
-resident in HBM and b already on the device.
+relative residual of 1e-8.  The operator is the reference's default one:
+every compressed block with more than SYS_SIZE = 2 Nmax = 256 columns keeps
+only its pivots and core pseudo-inverse and has its entries evaluated on the
+fly in every matvec (myHeaders.hpp:48, LowRank_matrix.hpp:22-36); the same
+solve with every factor stored is the stored_factors leg.  A step is one
+solve with the compressed operator resident in HBM and b already on the
+device.
 
 The line also carries:
-  roofline      the matvec kernel with the most time in the timed mixed
-                solves; achieved = its algorithmic bytes per launch / its
-                mean launch time from HIP events recorded on the solver's
-                stream during the timed region;
+  roofline      the HBM-streaming matvec kernel with the most time in the
+                timed mixed solves; achieved = its algorithmic bytes per
+                launch / its mean launch time from HIP events recorded on the
+                solver's stream during the timed region;
+  skeleton      the on-the-fly phase of the skeleton blocks per matvec, its
+                entries/s and its fraction of the FP64 vector-ALU issue peak;
   cpu_baseline  the FP64 CPU restatement (oracle/) in the reference's
                 execution model (skeleton blocks for |cols| > 2 Nmax
                 re-evaluate entries per matvec) on the host cores: the whole
@@ -948,7 +955,11 @@ def main():
                          "solved as one batch per step; serve: a Poisson stream of solve requests; "
                          "skeleton: the grid100 solve with on-the-fly skeleton blocks")
     ap.add_argument("--sys-size", type=int, default=-1,
-                    help="skeleton: blocks with more columns are skeletons (default 2 Nmax)")
+                    help="grid100 / skeleton: blocks with more columns are skeletons (default 2 Nmax, the "
+                         "reference's SYS_SIZE)")
+    ap.add_argument("--operator", choices=["skeleton", "stored"], default="skeleton",
+                    help="grid100: the reference's default operator (skeleton blocks for |cols| > sys_size, "
+                         "entries evaluated on the fly) or every factor stored")
     ap.add_argument("--requests", type=int, default=100, help="serve: requests in the stream")
     ap.add_argument("--sub-requests", type=int, default=100,
                     help="serve: requests of the second stream, offered at 0.8 x the served rate (0: none)")
@@ -999,17 +1010,24 @@ def main():
     tree = H.Tree(pts, -np.ones(3), np.ones(3), args.nmax, 0)
     t_tree = time.time() - t0
     log("tree: N=%d levels=%d admissible=%d (%.2fs)" % (n, tree.nlevels, tree.info.n_admissible, t_tree))
+    # the reference's default operator keeps skeleton blocks for every
+    # compressed block with more than SYS_SIZE = 2 Nmax columns
+    # (myHeaders.hpp:48, LowRank_matrix.hpp:22-36); --operator stored keeps
+    # every factor
+    sys_size = (args.sys_size if args.sys_size >= 0 else 2 * args.nmax) if args.operator == "skeleton" else -1
     t0 = time.time()
-    op = H.Operator(tree, kern, args.eps, H.PREC_F64 | H.PREC_F32)
+    op = H.Operator(tree, kern, args.eps, H.PREC_F64 | H.PREC_F32, sys_size=sys_size)
     t_asm = time.time() - t0
     st = op.stats()
-    log("assembled in %.2fs: e_near %.3g e_far %.3g, %.1f GB FP64 + %.1f GB FP32"
-        % (t_asm, st["e_near"], st["e_far"], st["bytes_f64"] * 1e-9, st["bytes_f32"] * 1e-9))
+    log("assembled in %.2fs (sys_size %d): e_near %.3g e_far %.3g, %d skeleton blocks (%.3g entries on the fly), "
+        "%.1f GB FP64 + %.1f GB FP32" % (t_asm, sys_size, st["e_near"], st["e_far"], st["n_skeleton"], st["e_skel"],
+                                        st["bytes_f64"] * 1e-9, st["bytes_f32"] * 1e-9))
 
     if args.matvec_only:
         for prec in (H.PREC_F64, H.PREC_F32):
             ms = op.matvec_timing(prec, 1, args.matvec_only)
-            log("matvec prec %d: %.3f ms (proj %.3f, panel %.3f)" % (prec, ms[0], ms[1], ms[2]))
+            log("matvec prec %d: %.3f ms (proj %.3f, panel %.3f, skeleton %.3f)"
+                % (prec, ms[0], ms[1], ms[2], ms[3]))
         return
 
     ctx = tree.ctx
@@ -1069,9 +1087,30 @@ def main():
     name, tot_ms, c, bytes_ = max(cands, key=lambda t: t[1])
     avg_ms = tot_ms / max(c, 1)
     achieved = bytes_ / (avg_ms * 1e-3) * 1e-9
-    traffic, traffic_src = measured_traffic(name, bytes_)
-    mv32 = (ms[2] + ms[3]) / max(cnt[1], 1)
-    mv64 = (ms[0] + ms[1]) / max(cnt[0], 1)
+    traffic_file = "matvec_traffic_grid100_N1e6%s.json" % ("_sys%d" % sys_size if sys_size >= 0 else "")
+    traffic, traffic_src = measured_traffic(name, bytes_, traffic_file)
+    # whole matvec (stored part + skeleton blocks) and its stored part alone
+    mv32 = (ms[2] + ms[3] + ms[5]) / max(cnt[1], 1)
+    mv64 = (ms[0] + ms[1] + ms[4]) / max(cnt[0], 1)
+    st32 = (ms[2] + ms[3]) / max(cnt[1], 1)
+    st64 = (ms[0] + ms[1]) / max(cnt[0], 1)
+    skeleton = None
+    if st["n_skeleton"]:
+        # the on-the-fly phase (skel_proj, skel_reduce, skel_core and one
+        # skel_expand launch per level) against the FP64 vector-ALU issue peak
+        sk = {"f64": ms[4] / max(cnt[0], 1), "f32": ms[5] / max(cnt[1], 1)}
+        ent = {k: st["e_skel"] / (v * 1e-3) if v else 0.0 for k, v in sk.items()}
+        skeleton = {
+            "sys_size": sys_size, "blocks": int(st["n_skeleton"]), "entries_per_matvec": st["e_skel"],
+            "cores_gb": st["bytes_core"] * 1e-9,
+            "phase_ms": sk, "entries_per_s": ent,
+            "roofline": {"bound": "valu", "unit": "FP64 VALU lane-ops/s", "peak": VALU_F64_LANE_OPS,
+                         "slots_per_entry": SKEL_SLOTS,
+                         "achieved": {k: ent[k] * SKEL_SLOTS[k] for k in ent},
+                         "frac": {k: ent[k] * SKEL_SLOTS[k] / VALU_F64_LANE_OPS for k in ent}},
+        }
+        log("skeleton phase per matvec: f64 %.3f ms, f32 %.3f ms (%.3g / %.3g entries/s)"
+            % (sk["f64"], sk["f32"], ent["f64"], ent["f32"]))
 
     n_mv64 = inf64[-1]["matvecs_f64"]
     cpu = cpu_1t = None
@@ -1100,9 +1139,12 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "3D %d^3 uniform grid, N=%d, A=I+(1/N)/r, Nmax=%d, eps_ACA=%g, d'=0, "
-                        "GMRES(100) to rel. residual %g, 1 RHS (FP64: x0=b; mixed: x0=0); every factor stored "
-                        "(sys_size -1)" % (n1, n, args.nmax, args.eps, args.tol),
-            "n": n, "nmax": args.nmax, "eps_aca": args.eps, "dprime": 0, "tol": args.tol, "sys_size": -1,
+                        "GMRES(100) to rel. residual %g, 1 RHS (FP64: x0=b; mixed: x0=0); %s"
+                        % (n1, n, args.nmax, args.eps, args.tol,
+                           "the reference's default operator: skeleton blocks for |cols| > SYS_SIZE = %d "
+                           "(2 Nmax), their entries evaluated on the fly" % sys_size if sys_size >= 0
+                           else "every factor stored (sys_size -1)"),
+            "n": n, "nmax": args.nmax, "eps_aca": args.eps, "dprime": 0, "tol": args.tol, "sys_size": sys_size,
         },
         "p50_ms": float(np.percentile(latm, 50) * 1e3),
         "p99_ms": float(np.percentile(latm, 99) * 1e3),
@@ -1118,12 +1160,16 @@ def main():
         "dense_check": {"rows": int(len(rows)), "matvec_rel_err": mv_err, "mixed_solution_resid": dres},
         "resid_fp64_estimate": inf64[-1]["resid"],
         "matvec_ms": {"f64": mv64, "f32": mv32},
-        "matvec_hbm_gbs": {"f64": kb64[0] / (mv64 * 1e-3) * 1e-9 if mv64 else None,
-                           "f32": kb32[0] / (mv32 * 1e-3) * 1e-9 if mv32 else None},
+        # the stored factors' bytes over the time of the kernels that stream them
+        "matvec_hbm_gbs": {"f64": kb64[0] / (st64 * 1e-3) * 1e-9 if st64 else None,
+                           "f32": kb32[0] / (st32 * 1e-3) * 1e-9 if st32 else None},
+        "skeleton": skeleton,
         "assemble_s": t_asm,
         "aca_rank_pass_s": st["rank_seconds"],
         "fill_s": st["fill_seconds"],
         "operator_gb": {"f64": st["bytes_f64"] * 1e-9, "f32": st["bytes_f32"] * 1e-9},
+        "stored_form_gb": {"f64": (st["bytes_f64"] - st["bytes_core"] + 8.0 * st["e_skel"]) * 1e-9,
+                           "f32": (st["bytes_f32"] + 4.0 * st["e_skel"]) * 1e-9},
         # the reference's print_matrix_statistics figures (HDD_matrix.hpp:69-85)
         "operator_stats": {"n_flop_ref": st["n_flop_ref"], "max_rank": int(st["max_rank"]),
                            "blocks": int(st["n_blocks"]), "dense_blocks": int(st["n_dense"]),
@@ -1148,19 +1194,18 @@ def main():
                  "algorithmic_bytes_per_launch": by,
                  "achieved_gbs": by / (tm / max(cn, 1) * 1e-3) * 1e-9 if tm else None,
                  "frac": by / (tm / max(cn, 1) * 1e-3) * 1e-9 / HBM_PEAK_GBS if tm else None,
-                 "traffic": measured_traffic(kn, by)[0]}
+                 "traffic": measured_traffic(kn, by, traffic_file)[0]}
             for kn, tm, cn, by in cands},
         "cpu_baseline": cpu,
         "cpu_baseline_1t": cpu_1t,
     }
     if not args.no_extra:
         # the batched and served forms of the same solve, each on its own
-        # workload; the same solve with the reference's memory-efficient
-        # (skeleton) operator; and a 160^3 grid (N = 4.1M) whose stored
-        # FP64 + FP32 operator (~330 GB) exceeds HBM, in skeleton form --
-        # reported beside the headline.  Each runs in a child
-        # process of its own (a fresh device-memory state), after this
-        # process has released its operator.
+        # workload; the same solve with every factor stored; and a 160^3
+        # grid (N = 4.1M) whose stored FP64 + FP32 operator (~330 GB)
+        # exceeds HBM, in skeleton form -- reported beside the headline.
+        # Each runs in a child process of its own (a fresh device-memory
+        # state), after this process has released its operator.
         op = None
         del db, dx
         import subprocess
@@ -1168,7 +1213,8 @@ def main():
                 ("serving", ["--workload", "serve", "--requests", "100", "--sub-requests", "100"]),
                 ("random3d_N1e4", ["--workload", "random3d", "--steps", "20", "--warmup", "2"]),
                 ("gauss_legendre_4d", ["--workload", "gl4d", "--steps", "10", "--warmup", "2"]),
-                ("skeleton", ["--workload", "skeleton", "--steps", "5", "--warmup", "1"]),
+                ("stored_factors", ["--workload", "grid100", "--operator", "stored", "--no-extra", "--no-cpu",
+                                    "--steps", "5", "--warmup", "1"]),
                 ("skeleton_beyond_hbm", ["--workload", "skeleton", "--n1d", "160", "--sys-size", "128",
                                          "--steps", "3", "--warmup", "1"]))
         for key, extra in legs:

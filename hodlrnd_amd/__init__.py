@@ -536,7 +536,9 @@ class Operator:
         self.ctx.check(lib.hodlr_op_profile(self.h, 1 if enable else 0), "profile")
 
     def profile_read(self, reset=True):
-        ms = np.zeros(4)
+        """Summed ms {f64 proj, f64 panel, f32 proj, f32 panel, f64 skeleton,
+        f32 skeleton} and matvec counts {f64, f32} since the last reset."""
+        ms = np.zeros(6)
         cnt = np.zeros(2, dtype=np.int64)
         self.ctx.check(lib.hodlr_op_profile_read(self.h, _p(ms), _p(cnt), 1 if reset else 0),
                        "profile_read")

@@ -743,6 +743,8 @@ int hodlr_op_profile_read(hodlr_op *op, double *ms, int64_t *counts, int reset) 
             ms[1] = op->prof_ms[0][1];
             ms[2] = op->prof_ms[1][0];
             ms[3] = op->prof_ms[1][1];
+            ms[4] = op->prof_ms[0][2];
+            ms[5] = op->prof_ms[1][2];
         }
         if (counts) {
             counts[0] = op->prof_count[0];

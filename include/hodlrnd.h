@@ -323,9 +323,10 @@ int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);
  * product (0 without skeleton blocks), averages per launch. */
 int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms);
 /* Live timing: while enabled, every matvec records HIP events around its
- * projection and panel launches on the stream it runs on.  read returns the
- * summed milliseconds ms[4] = {f64 proj, f64 panel, f32 proj, f32 panel} and
- * counts[2] = matvecs timed {f64, f32} since the last reset. */
+ * projection, panel and skeleton launches on the stream it runs on.  read
+ * returns the summed milliseconds ms[6] = {f64 proj, f64 panel, f32 proj,
+ * f32 panel, f64 skeleton blocks, f32 skeleton blocks} and counts[2] =
+ * matvecs timed {f64, f32} since the last reset. */
 int hodlr_op_profile(hodlr_op *op, int enable);
 int hodlr_op_profile_read(hodlr_op *op, double *ms, int64_t *counts, int reset);
 /* algorithmic HBM bytes of one launch of each matvec kernel (same order) */

@@ -12,7 +12,15 @@ lowest index wins).
     (the leaf, its parent, ... up to level 1); on those blocks ranks and
     dense flags are identical and on those leaves' rows the FP64 HODLR
     matvec matches the oracle's to 1e-12; the mixed solve is certified
-    below 1e-8 at full size.
+    below 1e-8 at full size;
+  * the same 100^3 problem with the reference's default operator (skeleton
+    blocks for |cols| > SYS_SIZE = 2 Nmax = 256, LowRank_matrix.hpp:22-36),
+    the headline operator: on 16 leaves' chains, ranks and dense flags equal
+    the oracle's skeleton-form assembly, the FP64 matvec rows match the
+    oracle's skeleton form within 10 eps_ACA (after the core pseudo-inverse
+    any two roundings of A(Is, J) x differ by cond(C) eps) and the dense
+    product within 1.5x the oracle's own error; the mixed solve is certified
+    below 1e-8.
 """
 import gc
 import sys
@@ -145,6 +153,54 @@ def test_grid100_full_size_sampled_chains():
     del ref
     gc.collect()
     b = O.splitmix_uniform(44, n)
+    xm, im = A.gmres(b, tol=1e-8, mode=H.SOLVE_MIXED, x0=np.zeros(n))
+    say("mixed", im["iterations"], im["outer"], im["true_resid"])
+    assert im["converged"] and im["true_resid"] < 1e-8
+    r64 = relerr(A.matvec(xm), b)
+    say("host residual of the mixed solution", r64)
+    assert r64 < 1.01e-8
+    del A, T
+    gc.collect()
+
+
+def test_grid100_skeleton_full_size_sampled_chains():
+    say("case grid100 skeleton N=1e6 Nmax=128 eps=1e-8 sys_size=256 (16 sampled leaf chains)")
+    pts = Wl.grid_points(100)
+    n = len(pts)
+    lo, hi = -np.ones(3), np.ones(3)
+    kern = H.Kernel(H.KERNEL_INV_R, 3, 1.0 / n, 1.0)
+    T = H.Tree(pts, lo, hi, 128, 0)
+    A = H.Operator(T, kern, 1e-8, sys_size=256)
+    st = A.stats()
+    say("gpu operator: blocks", st["n_blocks"], "skeleton", st["n_skeleton"], "dense", st["n_dense"])
+    assert st["n_skeleton"] > 0
+    ot = O.Tree(pts, lo, hi, 128, 0)
+    stride, offset = chain_sample(ot.nlevels, 3, 1234, 2048)
+    ok = O.KernelDesc(O.K_INV, 3, 1.0 / n, 1.0)
+    ref = O.Operator(ot, ok, 1e-8, sys_size=256, nthreads=16, sample=(stride, offset))
+    lv, nd = block_row_nodes(ot)
+    assert len(lv) == st["n_blocks"]
+    sel = (nd % stride[lv]) == offset[lv]
+    r_gpu, d_gpu = A.block_ranks()
+    r_ref, d_ref = ref.block_ranks()
+    say("sampled blocks", int(sel.sum()), "oracle skeleton blocks among them", ref.stats()["n_skeleton"])
+    assert ref.stats()["n_skeleton"] > 0
+    assert np.array_equal(r_gpu[sel], r_ref[sel]) and np.array_equal(d_gpu[sel], d_ref[sel])
+    leaves = 1234 + 2048 * np.arange(16)
+    rows = np.concatenate([ot.points(5, int(j)) for j in leaves])
+    x = O.splitmix_uniform(45, n)
+    y = A.matvec(x)
+    y_ref = ref.matvec(x, nthreads=16)
+    e_ref = relerr(y[rows], y_ref[rows])
+    y_dense = A.dense_rows(rows, x)
+    e_dense, e_dense_ref = relerr(y[rows], y_dense), relerr(y_ref[rows], y_dense)
+    e32 = relerr(A.matvec(x, prec=H.PREC_F32)[rows], y[rows])
+    say("rows", len(rows), "vs oracle", e_ref, "vs dense: gpu", e_dense, "oracle", e_dense_ref, "fp32", e32)
+    assert e_ref < 10 * 1e-8 and e32 < 1e-5
+    assert e_dense < 1e-5 and e_dense < 1.5 * e_dense_ref
+    del ref
+    gc.collect()
+    b = O.splitmix_uniform(46, n)
     xm, im = A.gmres(b, tol=1e-8, mode=H.SOLVE_MIXED, x0=np.zeros(n))
     say("mixed", im["iterations"], im["outer"], im["true_resid"])
     assert im["converged"] and im["true_resid"] < 1e-8
