@@ -381,6 +381,7 @@ def bench_batch(args, H):
     traffic_b, traffic_b_src = measured_traffic(name, bytes_, "matvec_traffic_batch256_k8.json", kname)
     mv32 = (ms[2] + ms[3]) / max(cnt[1], 1)
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)
+    read_gbs = H.stream_read_gbps(8 << 30, 5)
     cpu = None
     cpu_omp = None
     if not args.no_cpu:
@@ -434,6 +435,7 @@ def bench_batch(args, H):
         "roofline": {
             "bound": "hbm", "kernel": name, "kernel_form": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "measured_copy_gbs": copy_gbs,
+            "measured_read_gbs": read_gbs, "frac_of_measured_read": achieved / read_gbs if read_gbs else None,
             "traffic": traffic_b,
             "traffic_source": traffic_b_src,
             "algorithmic_bytes_per_launch": bytes_, "avg_launch_ms": avg_ms,
@@ -1002,6 +1004,7 @@ def main():
         return
 
     copy_gbs = H.stream_copy_gbps(1 << 30, 5)
+    read_gbs = H.stream_read_gbps(8 << 30, 5)
     n1 = args.n1d
     pts = make_points(n1)
     n = len(pts)
@@ -1182,6 +1185,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "measured_copy_gbs": copy_gbs,
+            # a plain streaming read of 8 GiB on the same card: the practical
+            # ceiling of a kernel that only reads
+            "measured_read_gbs": read_gbs,
+            "frac_of_measured_read": achieved / read_gbs if read_gbs else None,
             "traffic": traffic,
             "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": bytes_,

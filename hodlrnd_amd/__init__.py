@@ -129,6 +129,7 @@ SIGNATURES = {
     "hodlr_op_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "hodlr_op_profile_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "hodlr_stream_copy": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p]),
+    "hodlr_stream_read": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_void_p]),
     "hodlr_device_alloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p]),
     "hodlr_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hodlr_copy_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
@@ -214,6 +215,14 @@ def stream_copy_gbps(nbytes=1 << 30, reps=5, ctx=None):
     ctx = ctx or default_context()
     g = C.c_double()
     ctx.check(lib.hodlr_stream_copy(ctx.h, int(nbytes), reps, C.byref(g)), "stream_copy")
+    return g.value
+
+
+def stream_read_gbps(nbytes=4 << 30, reps=5, ctx=None):
+    """Measured device streaming-read bandwidth (GB/s)."""
+    ctx = ctx or default_context()
+    g = C.c_double()
+    ctx.check(lib.hodlr_stream_read(ctx.h, int(nbytes), reps, C.byref(g)), "stream_read")
     return g.value
 
 

@@ -783,6 +783,32 @@ int hodlr_stream_copy(hodlr_ctx *ctx, int64_t bytes, int reps, double *gbps) {
     });
 }
 
+int hodlr_stream_read(hodlr_ctx *ctx, int64_t bytes, int reps, double *gbps) {
+    if (!ctx || bytes < 16 || reps < 1 || !gbps) return HODLR_E_ARG;
+    return guard(ctx, [&] {
+        set_device(ctx);
+        hipStream_t s = ctx->stream;
+        bytes = bytes / 16 * 16;
+        DevBuf a, o;
+        a.alloc((size_t)bytes);
+        o.alloc(sizeof(float));
+        HIP_CHECK(hipMemsetAsync(a.p, 0, (size_t)bytes, s));  // zeros: the kernel stores nothing
+        HIP_CHECK(launch_stream_read(a.p, bytes, o.as<float>(), s));  // warm-up
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) HIP_CHECK(launch_stream_read(a.p, bytes, o.as<float>(), s));
+        HIP_CHECK(hipEventRecord(e1, s));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *gbps = (double)bytes * reps / (ms * 1e-3) * 1e-9;
+    });
+}
+
 int hodlr_device_alloc(hodlr_ctx *ctx, int64_t bytes, void **ptr) {
     if (!ctx || !ptr || bytes < 0) return HODLR_E_ARG;
     return guard(ctx, [&] {

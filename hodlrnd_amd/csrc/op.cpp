@@ -1172,6 +1172,23 @@ void assemble(hodlr_op &op) {
                                                         (int32_t)Sa[g], nprob[g], 0});
             }
         }
+        if (verbose()) {  // lanes doing counted work: one target per lane up to 64, else two
+            auto slots = [](int64_t nt) { return nt > kWave ? 2 * kWave : kWave; };
+            double up = 0, sp = 0, ue = 0, se = 0;
+            size_t nex = 0;
+            for (const SkelProj &it : skproj) {
+                up += (double)it.ntgt * it.nsrc;
+                sp += (double)slots(it.ntgt) * it.nsrc;
+            }
+            for (const auto &lv : bylev)
+                for (const SkelExpand &it : lv) {
+                    nex++;
+                    ue += (double)it.h * it.nc;
+                    se += (double)slots(it.h) * it.nc;
+                }
+            std::fprintf(stderr, "[hodlr] skeleton items: proj %zu, lane use %.3f; expand %zu, lane use %.3f\n",
+                         skproj.size(), sp > 0 ? up / sp : 0.0, nex, se > 0 ? ue / se : 0.0);
+        }
         // one launch per (kernel family, NDIM) group -- per tree level as well
         // for expand items -- with the longest items first
         auto key = [&](int p) { return 16 * op.kds[p].family + op.trees[p]->ndim; };

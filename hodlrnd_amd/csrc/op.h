@@ -20,6 +20,7 @@ hipError_t launch_f32_result(const float *y, const double *x, const double *diag
                              hipStream_t s);
 hipError_t launch_add_diag_f32(float *y, const float *x, const double *diag, int64_t n, int k, hipStream_t s);
 hipError_t launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
+hipError_t launch_stream_read(const void *a, int64_t bytes, float *out, hipStream_t s);
 template <typename T>
 hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjItem *pw, int64_t npw_big,
                          const ProjReduce *pr, int64_t npr,

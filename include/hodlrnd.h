@@ -338,6 +338,11 @@ int hodlr_matvec_kernel_bytes(hodlr_op *op, int prec, int nrhs, double *bytes);
  * matvec's HBM figures are compared with, and the known-size kernel used to
  * scale the FETCH_SIZE / WRITE_SIZE hardware counters. */
 int hodlr_stream_copy(hodlr_ctx *ctx, int64_t bytes, int reps, double *gbps);
+/* Measurement helper with no counterpart in the reference: reads `bytes` of
+ * one device buffer `reps` times with a plain streaming kernel (16-byte
+ * loads, eight in flight per lane) and returns the achieved read GB/s -- the
+ * measured ceiling of kernels that mostly read, such as the matvec's. */
+int hodlr_stream_read(hodlr_ctx *ctx, int64_t bytes, int reps, double *gbps);
 
 /* device memory helpers for callers that keep vectors resident */
 int hodlr_device_alloc(hodlr_ctx *ctx, int64_t bytes, void **ptr);

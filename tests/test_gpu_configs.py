@@ -99,3 +99,14 @@ def test_config3_gauss_legendre_4d(dprime):
     say("mixed solve", info["iterations"], info["outer"], info["true_resid"])
     assert info["converged"] and info["true_resid"] < 1e-8
     assert np.linalg.norm(b - A.matvec(xs)) / np.linalg.norm(b) < 1.01e-8
+
+
+def test_measurement_ceilings():
+    """The streaming copy and read helpers the bench line reports beside its
+    roofline: plausible, and the read ceiling below the nominal 8 TB/s."""
+    c = H.stream_copy_gbps(1 << 28, 3)
+    r = H.stream_read_gbps(1 << 30, 3)
+    say("copy %.0f GB/s, read %.0f GB/s" % (c, r))
+    assert 500 < c < 8000 and 1000 < r < 8000
+    with pytest.raises(H.HodlrError):
+        H.stream_read_gbps(8, 1)
