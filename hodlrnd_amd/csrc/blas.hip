@@ -246,12 +246,14 @@ __device__ __forceinline__ double col_sum(double x) {
 }
 
 // partial[(chunk * nv + v) * K + c] = sum over the chunk's rows of V_v[row, c] * w[row, c].
-// Basis vectors are taken G at a time, so w is read once per G of them.
+// Basis vectors are taken G at a time, so w is read once per G of them; a
+// thread takes R of its rows per step (in row order), so (G + 1) R loads are
+// in flight even when an engine's few chunks put few waves on the chip.
 template <int K>
 __global__ __launch_bounds__(256) void bmdot_kernel(const double *const *__restrict__ V, int nv,
                                                     const double *__restrict__ w, const Chunk *__restrict__ chunks,
                                                     double *__restrict__ partial) {
-    constexpr int RP = 256 / K, G = 8;
+    constexpr int RP = 256 / K, G = 8, R = 4;
     __shared__ double red[G][4][K];
     const Chunk C = chunks[blockIdx.x];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, c = tid % K, ro = tid / K;
@@ -263,7 +265,26 @@ __global__ __launch_bounds__(256) void bmdot_kernel(const double *const *__restr
         double acc[G];
 #pragma unroll
         for (int g = 0; g < G; g++) acc[g] = 0.0;
-        for (int r = ro; r < C.rows; r += RP) {
+        int r = ro;
+        for (; r + (R - 1) * RP < C.rows; r += R * RP) {
+            int64_t e[R];
+            double wv[R];
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                e[q] = (C.row0 + r + q * RP) * K + c;
+                wv[q] = w[e[q]];
+            }
+#pragma unroll
+            for (int g = 0; g < G; g++)
+                if (g < gn) {
+                    double a[R];
+#pragma unroll
+                    for (int q = 0; q < R; q++) a[q] = vp[g][e[q]];
+#pragma unroll
+                    for (int q = 0; q < R; q++) acc[g] = fma(a[q], wv[q], acc[g]);
+                }
+        }
+        for (; r < C.rows; r += RP) {
             const int64_t e = (C.row0 + r) * K + c;
             const double wv = w[e];
 #pragma unroll
@@ -300,22 +321,36 @@ __global__ __launch_bounds__(256) void bfinish_kernel(const double *__restrict__
 
 // w[row, c] += sign * sum_v coef[(prob * nv + v) * K + c] * V_v[row, c]
 // (per element: w first, then the basis vectors in order).  The chunk's
-// nv x K coefficients sit in dynamic LDS.
+// nv x K coefficients sit in dynamic LDS; a thread updates R of its rows at
+// once, so R loads of each basis vector are in flight.
 template <int K>
 __global__ __launch_bounds__(256) void bgemv_kernel(const double *const *__restrict__ V, int nv,
                                                     const double *__restrict__ coef, double sign,
                                                     double *__restrict__ w, const Chunk *__restrict__ chunks) {
-    constexpr int RP = 256 / K;
+    constexpr int RP = 256 / K, R = 8;
     extern __shared__ double cs[];
     const Chunk C = chunks[blockIdx.x];
     for (int e = threadIdx.x; e < nv * K; e += blockDim.x) cs[e] = sign * coef[(int64_t)C.prob * nv * K + e];
     __syncthreads();
     const int c = threadIdx.x % K, ro = threadIdx.x / K;
-    for (int r = ro; r < C.rows; r += RP) {
-        const int64_t e = (C.row0 + r) * K + c;
-        double acc = w[e];
-        for (int v = 0; v < nv; v++) acc = fma(cs[v * K + c], V[v][e], acc);
-        w[e] = acc;
+    for (int r0 = ro; r0 < C.rows; r0 += R * RP) {
+        int64_t e[R];
+        double acc[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const int r = r0 + q * RP < C.rows ? r0 + q * RP : r0;  // past the chunk: r0 again, not stored
+            e[q] = (C.row0 + r) * K + c;
+            acc[q] = w[e[q]];
+        }
+        for (int v = 0; v < nv; v++) {
+            const double cf = cs[v * K + c];
+            const double *vv = V[v];
+#pragma unroll
+            for (int q = 0; q < R; q++) acc[q] = fma(cf, vv[e[q]], acc[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++)
+            if (r0 + q * RP < C.rows) w[e[q]] = acc[q];
     }
 }
 
