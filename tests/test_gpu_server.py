@@ -259,3 +259,37 @@ def test_server_poisson_stream_vs_oracle():
     srv.close()
     say("stream: ops", len(ops), "requests", count, "columns vs oracle", checked, "stats", st)
     assert len(ops) >= 3 and checked >= 5 and st["requests_done"] == count
+
+
+def test_server_largest_operator_vs_oracle():
+    """The serving workload's upper end: an operator of N = 200,000 points
+    (3-D uniform, A = I + (1/N)/r, eps_ACA 1e-8, Nmax 64) built by the server
+    next to a small one, 4 right-hand sides on it and 2 on the small one in
+    flight together.  Every column is certified below 1e-8, and each column
+    of the large request matches the FP64 oracle's GMRES solution on the
+    oracle's own assembly of that operator (GMRES.hpp:160-303) within 100 tol."""
+    say("case server_largest_operator N=200000")
+    from oracle import pyoracle as O
+    n, n2 = 200000, 3000
+    P, P2 = pts3(n, 901), pts3(n2, 902)
+    srv = H.Server(slots=8, tol=1e-8, builders=2)
+    big = srv.submit_build(P, kern(n))
+    small = srv.submit_build(P2, kern(n2))
+    B, B2 = rhs(n, 4, 910), rhs(n2, 2, 920)
+    r_big = srv.submit_solve(big, B)
+    r_small = srv.submit_solve(small, B2)
+    X, info = srv.wait(r_big, timeout=300)
+    X2, info2 = srv.wait(r_small, timeout=300)
+    st = srv.stats()
+    srv.close()
+    say("served", info, info2, st)
+    assert info["rc"] == 0 and info["converged"] == 1 and info["max_resid"] < 1e-8
+    assert info2["rc"] == 0 and info2["converged"] == 1 and info2["max_resid"] < 1e-8
+    T = O.Tree(P, -np.ones(3), np.ones(3), 64, 0)
+    ref = O.Operator(T, O.KernelDesc(O.K_INV, 3, 1.0 / n, 1.0), 1e-8, nthreads=16)
+    for c in range(B.shape[1]):
+        xr, ir = ref.gmres(B[:, c], tol=1e-10, nthreads=16)
+        e = relerr(X[:, c], xr)
+        say("column", c, "vs oracle", e, "oracle iterations", ir["iterations"])
+        assert e < 100 * 1e-8
+    del ref, T
