@@ -20,8 +20,11 @@ The line also carries:
                 timed mixed solves; achieved = its algorithmic bytes per
                 launch / its mean launch time from HIP events recorded on the
                 solver's stream during the timed region;
-  skeleton      the on-the-fly phase of the skeleton blocks per matvec, its
-                entries/s and its fraction of the FP64 vector-ALU issue peak;
+  skeleton      the on-the-fly phase of the skeleton blocks: the schedule the
+                solves used (on a second stream beside the stored passes,
+                compute-bound beside memory-bound work), the whole matvec in
+                it and on one stream, and the phase alone on one stream: its
+                entries/s and fraction of the FP64 vector-ALU issue peak;
   cpu_baseline  the FP64 CPU restatement (oracle/) in the reference's
                 execution model (skeleton blocks for |cols| > 2 Nmax
                 re-evaluate entries per matvec) on the host cores: the whole
@@ -506,12 +509,20 @@ def bench_skeleton(args, H):
         "matvec, %.1f GB FP64 + %.1f GB FP32 (stored form: %.1f + %.1f GB)"
         % (n, sys_size, t_asm, st["n_skeleton"], st["e_skel"], st["bytes_f64"] * 1e-9, st["bytes_f32"] * 1e-9,
            stored64 * 1e-9, stored32 * 1e-9))
+    # the whole product in the schedule the solves use (skeleton phases
+    # beside the stored passes), then on one stream, where the skeleton
+    # phases' own time is the "skeleton" figure
     mvt = {}
     for prec, key in ((H.PREC_F64, "f64"), (H.PREC_F32, "f32")):
+        sched = op.skeleton_schedule(prec, 1)
         ms = op.matvec_timing(prec, 1, 10)
-        mvt[key] = {"total": ms[0], "proj": ms[1], "panel": ms[2], "skeleton": ms[3]}
-        log("matvec %s: %.3f ms (stored part: proj %.3f, panel %.3f; skeleton %.3f)"
-            % (key, ms[0], ms[1], ms[2], ms[3]))
+        op.set_skeleton_schedule(H.SKEL_SERIAL)
+        m1 = op.matvec_timing(prec, 1, 10)
+        op.set_skeleton_schedule(H.SKEL_AUTO)
+        mvt[key] = {"total": ms[0], "proj": ms[1], "panel": ms[2], "skeleton_span": ms[3], "schedule": sched,
+                    "total_one_stream": m1[0], "skeleton": m1[3]}
+        log("matvec %s: %.3f ms in schedule %d (stored part: proj %.3f, panel %.3f; skeleton span %.3f); "
+            "one stream %.3f ms (skeleton alone %.3f)" % (key, ms[0], sched, ms[1], ms[2], ms[3], m1[0], m1[3]))
     ctx = tree.ctx
     perm = tree.perm()
     b = splitmix_uniform(1, n)
@@ -1092,28 +1103,46 @@ def main():
     achieved = bytes_ / (avg_ms * 1e-3) * 1e-9
     traffic_file = "matvec_traffic_grid100_N1e6%s.json" % ("_sys%d" % sys_size if sys_size >= 0 else "")
     traffic, traffic_src = measured_traffic(name, bytes_, traffic_file)
-    # whole matvec (stored part + skeleton blocks) and its stored part alone
-    mv32 = (ms[2] + ms[3] + ms[5]) / max(cnt[1], 1)
-    mv64 = (ms[0] + ms[1] + ms[4]) / max(cnt[0], 1)
+    # whole matvec (stored part + skeleton blocks) in the schedule the solves
+    # used, timed on its own; the stored part's kernels as timed in the solves
+    precs = (("f64", H.PREC_F64), ("f32", H.PREC_F32))
+    t_whole = {k: op.matvec_timing(p, 1, 10) for k, p in precs}
+    mv32, mv64 = t_whole["f32"][0], t_whole["f64"][0]
     st32 = (ms[2] + ms[3]) / max(cnt[1], 1)
     st64 = (ms[0] + ms[1]) / max(cnt[0], 1)
     skeleton = None
+    one_stream = None
     if st["n_skeleton"]:
-        # the on-the-fly phase (skel_proj, skel_reduce, skel_core and one
-        # skel_expand launch per level) against the FP64 vector-ALU issue peak
-        sk = {"f64": ms[4] / max(cnt[0], 1), "f32": ms[5] / max(cnt[1], 1)}
+        # the skeleton phases (skel_proj, skel_reduce, skel_core and one
+        # skel_expand launch per level) run beside the stored passes on a
+        # second stream; alone on one stream they are timed against the FP64
+        # vector-ALU issue peak
+        sched = {k: op.skeleton_schedule(p, 1) for k, p in precs}
+        op.set_skeleton_schedule(H.SKEL_SERIAL)
+        t_one = {k: op.matvec_timing(p, 1, 10) for k, p in precs}
+        op.set_skeleton_schedule(H.SKEL_AUTO)
+        sk = {k: t_one[k][3] for k, _ in precs}
         ent = {k: st["e_skel"] / (v * 1e-3) if v else 0.0 for k, v in sk.items()}
         skeleton = {
             "sys_size": sys_size, "blocks": int(st["n_skeleton"]), "entries_per_matvec": st["e_skel"],
             "cores_gb": st["bytes_core"] * 1e-9,
+            "schedule": {k: {1: "one stream", 2: "beside the stored passes",
+                             3: "beside the stored passes, occupancy capped"}[v] for k, v in sched.items()},
+            "span_beside_ms": {"f64": ms[4] / max(cnt[0], 1), "f32": ms[5] / max(cnt[1], 1)},
+            "matvec_ms_one_stream": {k: t_one[k][0] for k, _ in precs},
             "phase_ms": sk, "entries_per_s": ent,
             "roofline": {"bound": "valu", "unit": "FP64 VALU lane-ops/s", "peak": VALU_F64_LANE_OPS,
                          "slots_per_entry": SKEL_SLOTS,
                          "achieved": {k: ent[k] * SKEL_SLOTS[k] for k in ent},
                          "frac": {k: ent[k] * SKEL_SLOTS[k] / VALU_F64_LANE_OPS for k in ent}},
         }
-        log("skeleton phase per matvec: f64 %.3f ms, f32 %.3f ms (%.3g / %.3g entries/s)"
-            % (sk["f64"], sk["f32"], ent["f64"], ent["f32"]))
+        # the dominant kernel when the skeleton phases are not beside it
+        t_alone = t_one[name.split("_")[-1]][1 if "_proj_" in name else 2]
+        one_stream = {"avg_launch_ms": t_alone, "achieved": bytes_ / (t_alone * 1e-3) * 1e-9,
+                      "frac": bytes_ / (t_alone * 1e-3) * 1e-9 / HBM_PEAK_GBS}
+        log("skeleton phase alone per matvec: f64 %.3f ms, f32 %.3f ms (%.3g / %.3g entries/s); schedule %s; "
+            "whole matvec f64 %.3f ms (one stream %.3f), f32 %.3f ms (one stream %.3f)"
+            % (sk["f64"], sk["f32"], ent["f64"], ent["f32"], sched, mv64, t_one["f64"][0], mv32, t_one["f32"][0]))
 
     n_mv64 = inf64[-1]["matvecs_f64"]
     cpu = cpu_1t = None
@@ -1193,6 +1222,10 @@ def main():
             "traffic_source": traffic_src,
             "algorithmic_bytes_per_launch": bytes_,
             "avg_launch_ms": avg_ms,
+            # the same kernel timed with the skeleton phases after it on one
+            # stream instead of beside it (the solves run them beside it:
+            # the kernel shares its CUs, the whole product is faster)
+            "one_stream": one_stream,
         },
         # every pass of the timed mixed solves (the FP32 projection and panel
         # product take nearly the same time, so which one is dominant varies)

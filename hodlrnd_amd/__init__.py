@@ -35,6 +35,10 @@ class HodlrError(RuntimeError):
 
 MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT, MV_PANEL_VEC, MV_PROJ_PACK, MV_PANEL_WIDE, MV_PROJ_WAVE = \
     1, 2, 4, 8, 16, 32, 64
+# where a matvec evaluates its skeleton blocks: after the stored passes on the
+# calling stream, beside them on a second stream, or beside them with capped
+# occupancy; AUTO = the tuner's pick, else a cost model (same bits for all)
+SKEL_AUTO, SKEL_SERIAL, SKEL_OVERLAP, SKEL_CAPPED = 0, 1, 2, 3
 
 
 class KernelDesc(C.Structure):
@@ -97,6 +101,8 @@ SIGNATURES = {
     "hodlr_op_tune_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "hodlr_op_set_matvec_variant": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "hodlr_op_get_matvec_variant": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
+    "hodlr_op_set_skeleton_schedule": (C.c_int, [C.c_void_p, C.c_int]),
+    "hodlr_op_get_skeleton_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p]),
     "hodlr_server_create": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p]),
     "hodlr_server_destroy": (None, [C.c_void_p]),
     "hodlr_server_submit_build": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -530,6 +536,15 @@ class Operator:
     def get_variant(self, prec, nrhs):
         v = C.c_int()
         self.ctx.check(lib.hodlr_op_get_matvec_variant(self.h, prec, nrhs, C.byref(v)), "get_variant")
+        return v.value
+
+    def set_skeleton_schedule(self, mode):
+        self.ctx.check(lib.hodlr_op_set_skeleton_schedule(self.h, mode), "set_skeleton_schedule")
+
+    def skeleton_schedule(self, prec=PREC_F64, nrhs=1):
+        """The schedule a (prec, nrhs) product of this operator takes."""
+        v = C.c_int()
+        self.ctx.check(lib.hodlr_op_get_skeleton_schedule(self.h, prec, nrhs, C.byref(v)), "get_skeleton_schedule")
         return v.value
 
     def set_graphs(self, enable=True):

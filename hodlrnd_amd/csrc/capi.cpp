@@ -691,6 +691,19 @@ int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant) 
     return HODLR_OK;
 }
 
+int hodlr_op_set_skeleton_schedule(hodlr_op *op, int mode) {
+    if (!op || mode < HODLR_SKEL_AUTO || mode > HODLR_SKEL_CAPPED) return HODLR_E_ARG;
+    op->skel_mode = mode;
+    return HODLR_OK;
+}
+
+int hodlr_op_get_skeleton_schedule(hodlr_op *op, int prec, int nrhs, int *mode) {
+    if (!op || !mode) return HODLR_E_ARG;
+    if (variant_slot(prec, nrhs) < 0) return HODLR_E_ARG;
+    *mode = op->n_skcore ? op->skel_schedule(prec, nrhs) : HODLR_SKEL_SERIAL;
+    return HODLR_OK;
+}
+
 int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms) {
     if (!op || !ms || reps < 1) return HODLR_E_ARG;
     return guard(op->ctx, [&] {
@@ -703,23 +716,26 @@ int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms) 
         dx.alloc(es * n * nrhs);
         dy.alloc(es * n * nrhs);
         HIP_CHECK(hipMemset(dx.p, 0, es * n * nrhs));
-        std::vector<hipEvent_t> ev((size_t)4 * reps);
+        const size_t E = kMatvecEvents;
+        std::vector<hipEvent_t> ev(E * reps);
         for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
         op->matvec_dev(prec, nrhs, dx.p, dy.p, s);  // warm-up
-        for (int r = 0; r < reps; r++) op->matvec_dev(prec, nrhs, dx.p, dy.p, s, &ev[(size_t)4 * r]);
+        for (int r = 0; r < reps; r++) op->matvec_dev(prec, nrhs, dx.p, dy.p, s, &ev[E * r]);
         HIP_CHECK(hipStreamSynchronize(s));
-        double a = 0, b = 0, c = 0;
+        double a = 0, b = 0, c = 0, t = 0;
         for (int r = 0; r < reps; r++) {
-            float t01 = 0, t12 = 0, t23 = 0;
-            HIP_CHECK(hipEventElapsedTime(&t01, ev[4 * r], ev[4 * r + 1]));
-            HIP_CHECK(hipEventElapsedTime(&t12, ev[4 * r + 1], ev[4 * r + 2]));
-            HIP_CHECK(hipEventElapsedTime(&t23, ev[4 * r + 2], ev[4 * r + 3]));
+            float t01 = 0, t12 = 0, t03 = 0, t45 = 0;
+            HIP_CHECK(hipEventElapsedTime(&t01, ev[E * r], ev[E * r + 1]));
+            HIP_CHECK(hipEventElapsedTime(&t12, ev[E * r + 1], ev[E * r + 2]));
+            HIP_CHECK(hipEventElapsedTime(&t03, ev[E * r], ev[E * r + 3]));
+            if (op->n_skcore) HIP_CHECK(hipEventElapsedTime(&t45, ev[E * r + 4], ev[E * r + 5]));
             a += t01;
             b += t12;
-            c += t23;
+            c += t45;
+            t += t03;
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
-        ms[0] = (a + b + c) / reps;
+        ms[0] = t / reps;
         ms[1] = a / reps;
         ms[2] = b / reps;
         ms[3] = c / reps;

@@ -17,7 +17,9 @@
 // kernels.  Every call on one GPU shares that GPU's array: calls that
 // overlap on one GPU (different streams or threads) race and must be
 // serialised by the caller.  No kernel reads an intermediate it has not
-// written in the same product, so nothing needs clearing between calls.
+// written in the same product (the skeleton phases' FP64 result vector is
+// zeroed by a kernel of the product itself), so nothing needs clearing
+// between calls.
 #include <mutex>
 #include <vector>
 
@@ -73,10 +75,10 @@ extern "C" int hodlr_matvec_device_noalloc(hodlr_op *op, int prec, int nrhs, con
             throw ArgError{HODLR_E_ARG, "FP64 skeleton operators support at most 16 RHS"};
         if (ctx->device < 0) throw ArgError{HODLR_E_STATE, "host-only context (device -1) cannot run device work"};
         HIP_CHECK(hipSetDevice(ctx->device));
-        size_t sz[5];
+        size_t sz[6];
         op->matvec_work_sizes(prec, nrhs, sz);
-        size_t off[5], total = 0;
-        for (int i = 0; i < 5; i++) {
+        size_t off[6], total = 0;
+        for (int i = 0; i < 6; i++) {
             off[i] = total;
             total += (sz[i] + kScratchAlign - 1) / kScratchAlign * kScratchAlign;
         }
@@ -92,7 +94,11 @@ extern "C" int hodlr_matvec_device_noalloc(hodlr_op *op, int prec, int nrhs, con
             w.skt1 = reinterpret_cast<double *>(base + off[2]);
             w.skt2 = reinterpret_cast<double *>(base + off[3]);
             w.skpart = reinterpret_cast<double *>(base + off[4]);
+            w.skys = reinterpret_cast<double *>(base + off[5]);
         }
+        // the skeleton phases stay on the caller's stream (w.overlap false):
+        // the operator's second stream is created on first use, which this
+        // entry point must not do
         op->matvec_dev_on(prec, nrhs, x, y, stream ? static_cast<hipStream_t>(stream) : ctx->stream, w);
         return HODLR_OK;
     } catch (const HipError &e) {

@@ -1491,6 +1491,7 @@ void hodlr_op::ensure_work(int prec, int nrhs) {
         skt1.alloc(sizeof(double) * (size_t)n_sk * nrhs);
         skt2.alloc(sizeof(double) * (size_t)n_sk * nrhs);
         skpart.alloc(sizeof(double) * (size_t)(n_skslots > 0 ? n_skslots : 1) * nrhs);
+        skys.alloc(sizeof(double) * (size_t)N * nrhs);
         cap_rhs_sk = nrhs;
         work_gen++;
     }
@@ -1510,6 +1511,9 @@ void hodlr_op::ensure_work(int prec, int nrhs) {
 
 hodlr_op::~hodlr_op() {
     for (auto &e : ev_pool) (void)hipEventDestroy(e);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
 }
 
 void hodlr_op::profile_collect() {
@@ -1520,7 +1524,7 @@ void hodlr_op::profile_collect() {
         float a = 0, b = 0, c = 0;
         HIP_CHECK(hipEventElapsedTime(&a, ev_pool[pe.second], ev_pool[pe.second + 1]));
         HIP_CHECK(hipEventElapsedTime(&b, ev_pool[pe.second + 1], ev_pool[pe.second + 2]));
-        HIP_CHECK(hipEventElapsedTime(&c, ev_pool[pe.second + 2], ev_pool[pe.second + 3]));
+        if (n_skcore) HIP_CHECK(hipEventElapsedTime(&c, ev_pool[pe.second + 4], ev_pool[pe.second + 5]));
         const int p = pe.first == HODLR_PREC_F64 ? 0 : 1;
         prof_ms[p][0] += a;
         prof_ms[p][1] += b;
@@ -1535,13 +1539,13 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
     using namespace hodlr;
     ensure_work(prec, nrhs);
     if (profiling && !ev) {
-        const size_t need = 4 * (ev_pending.size() + 1);
+        const size_t need = kMatvecEvents * (ev_pending.size() + 1);
         while (ev_pool.size() < need) {
             hipEvent_t e;
             HIP_CHECK(hipEventCreate(&e));
             ev_pool.push_back(e);
         }
-        const size_t first = 4 * ev_pending.size();
+        const size_t first = kMatvecEvents * ev_pending.size();
         ev_pending.push_back({prec, first});
         ev = &ev_pool[first];
     }
@@ -1551,25 +1555,92 @@ void hodlr_op::matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_
     w.skt1 = skt1.as<double>();
     w.skt2 = skt2.as<double>();
     w.skpart = skpart.as<double>();
+    w.skys = skys.as<double>();
+    const int mode = n_skcore ? skel_schedule(prec, nrhs) : HODLR_SKEL_SERIAL;
+    if (mode != HODLR_SKEL_SERIAL) {
+        // not while s is being captured into a graph: the second stream
+        // would have to join the capture
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_CHECK(hipStreamIsCapturing(s, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            if (!side) {
+                HIP_CHECK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+                HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+                HIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+            }
+            w.skel = mode;
+        }
+    }
     matvec_dev_on(prec, nrhs, x, y, s, w, ev, add_diag);
 }
 
-void hodlr_op::matvec_work_sizes(int prec, int nrhs, size_t sz[5]) const {
+// The schedule of the skeleton phases for one (precision, RHS count).
+// Without a caller's or the tuner's choice a cost model decides.  The stored
+// passes move their algorithmic bytes at about 5.7 TB/s; the skeleton
+// phases evaluate about 1.7e12 entries/s with every CU to themselves (1 RHS;
+// each further RHS adds one FMA to ~12 instructions per entry).  Uncapped
+// beside the stored passes they add about 0.65 of their own time to the
+// product; capped they take about 2.2x their own time but leave the stored
+// passes close to full speed.  So capping pays while 2.2 t_skel < t_stored +
+// 0.65 t_skel.  Measured on the N = 10^6 grid operator (FP32: stored passes
+// 2.88 ms, skeleton phases 1.51 ms alone): one stream 4.40 ms, uncapped
+// 3.87 ms, capped 3.35 ms; FP64 (5.72 and 1.59 ms): 7.33, 6.48, 6.12 ms.
+// An operator of mostly skeleton blocks keeps the whole machine for them.
+int hodlr_op::skel_schedule(int prec, int nrhs) const {
+    if (skel_mode != HODLR_SKEL_AUTO) return skel_mode;
+    int lg = 0;
+    while ((1 << lg) < nrhs && lg < 5) lg++;
+    const int pi = prec == HODLR_PREC_F64 ? 0 : 1;
+    if (skel_pick[pi][lg]) return skel_pick[pi][lg];
+    const double t_stored = (pi == 0 ? stats.matvec_bytes_f64 : stats.matvec_bytes_f32) / 5.7e12;
+    const double t_skel = e_skel * (11.0 + nrhs) / 12.0 / 1.7e12;
+    return 1.55 * t_skel <= t_stored ? HODLR_SKEL_CAPPED : HODLR_SKEL_OVERLAP;
+}
+
+void hodlr_op::matvec_work_sizes(int prec, int nrhs, size_t sz[6]) const {
     using namespace hodlr;
     const size_t es = prec == HODLR_PREC_F64 ? sizeof(double) : sizeof(float);
     sz[0] = es * (size_t)(n_t > 0 ? n_t : 1) * nrhs;
     sz[1] = es * (size_t)(n_slots > 0 ? n_slots : 1) * kProjTile * nrhs;
-    sz[2] = sz[3] = sz[4] = 0;
+    sz[2] = sz[3] = sz[4] = sz[5] = 0;
     if (n_skcore) {
         sz[2] = sz[3] = sizeof(double) * (size_t)n_sk * nrhs;
         sz[4] = sizeof(double) * (size_t)(n_skslots > 0 ? n_skslots : 1) * nrhs;
+        sz[5] = sizeof(double) * (size_t)N * nrhs;
     }
 }
 
 void hodlr_op::matvec_dev_on(int prec, int nrhs, const void *x, void *y, hipStream_t s, const hodlr::MatvecWork &w,
                              hipEvent_t *ev, bool add_diag) {
     using namespace hodlr;
-    if (prec == HODLR_PREC_F64) {
+    const bool f64 = prec == HODLR_PREC_F64;
+    if (!f64 && !(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
+    // skeleton phases: they read x and write only their own buffers (ys
+    // last), so on the second stream they run beside the stored blocks'
+    // passes; y += ys follows both.  Issued before the stored passes, so a
+    // stream sharing a hardware queue with the second one never waits
+    // ahead of the work it waits for.
+    const bool skel = n_skcore > 0;
+    const bool over = skel && w.skel != HODLR_SKEL_SERIAL;
+    if (over && !side) throw ArgError{HODLR_E_STATE, "skeleton schedule needs the operator's second stream"};
+    auto skeleton = [&](hipStream_t ss) {
+        const SkeletonPlan p = skel_plan(w.skel == HODLR_SKEL_CAPPED ? kSkelCapLds : 0);
+        if (ev) HIP_CHECK(hipEventRecord(ev[4], ss));
+        if (f64)
+            HIP_CHECK(launch_skeleton<double>(nrhs, p, static_cast<const double *>(x), w.skt1, w.skpart, w.skt2,
+                                              w.skys, N, ss));
+        else
+            HIP_CHECK(launch_skeleton<float>(nrhs, p, static_cast<const float *>(x), w.skt1, w.skpart, w.skt2,
+                                             w.skys, N, ss));
+        if (ev) HIP_CHECK(hipEventRecord(ev[5], ss));
+    };
+    if (over) {
+        HIP_CHECK(hipEventRecord(ev_fork, s));
+        HIP_CHECK(hipStreamWaitEvent(side, ev_fork, 0));
+        skeleton(side);
+        HIP_CHECK(hipEventRecord(ev_join, side));
+    }
+    if (f64)
         HIP_CHECK(launch_matvec<double>(nrhs, proj.as<ProjItem>(), n_proj, projw.as<ProjItem>(), n_projw_big,
                                         red.as<ProjReduce>(), n_red,
                                         panel.as<PanelItem>(), n_panel,
@@ -1577,12 +1648,7 @@ void hodlr_op::matvec_dev_on(int prec, int nrhs, const void *x, void *y, hipStre
                                         UP64.as<double>(), static_cast<const double *>(x), static_cast<double *>(w.t),
                                         static_cast<double *>(w.part), tidx.as<int32_t>(), static_cast<double *>(y),
                                         variant_for(prec, nrhs), ev, s));
-        if (n_skcore)
-            HIP_CHECK(launch_skeleton<double>(nrhs, skel_plan(), static_cast<const double *>(x), w.skt1, w.skpart,
-                                              w.skt2, static_cast<double *>(y), s));
-        if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
-    } else {
-        if (!(prec_flags & HODLR_PREC_F32)) throw ArgError{HODLR_E_STATE, "operator has no FP32 copy"};
+    else
         HIP_CHECK(launch_matvec<float>(nrhs, proj.as<ProjItem>(), n_proj, projw.as<ProjItem>(), n_projw_big,
                                        red.as<ProjReduce>(), n_red,
                                        panel.as<PanelItem>(), n_panel,
@@ -1591,17 +1657,19 @@ void hodlr_op::matvec_dev_on(int prec, int nrhs, const void *x, void *y, hipStre
                                        static_cast<const float *>(x), static_cast<float *>(w.t),
                                        static_cast<float *>(w.part), tidx.as<int32_t>(), static_cast<float *>(y),
                                        variant_for(prec, nrhs), ev, s));
-        if (n_skcore)
-            HIP_CHECK(launch_skeleton<float>(nrhs, skel_plan(), static_cast<const float *>(x), w.skt1, w.skpart,
-                                             w.skt2, static_cast<float *>(y), s));
-        if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
-        if (add_diag)
-            HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x),
-                                          diag.as<double>(), N, nrhs, s));
+    if (skel) {
+        if (over) HIP_CHECK(hipStreamWaitEvent(s, ev_join, 0));
+        else skeleton(s);
+        if (f64) HIP_CHECK(launch_skeleton_add<double>(static_cast<double *>(y), w.skys, N * nrhs, s));
+        else HIP_CHECK(launch_skeleton_add<float>(static_cast<float *>(y), w.skys, N * nrhs, s));
     }
+    if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
+    if (!f64 && add_diag)
+        HIP_CHECK(launch_add_diag_f32(static_cast<float *>(y), static_cast<const float *>(x),
+                                      diag.as<double>(), N, nrhs, s));
 }
 
-hodlr::SkeletonPlan hodlr_op::skel_plan() const {
+hodlr::SkeletonPlan hodlr_op::skel_plan(int lds_min) const {
     using namespace hodlr;
     SkeletonPlan p;
     p.proj = skproj.as<SkelProj>();
@@ -1618,6 +1686,7 @@ hodlr::SkeletonPlan hodlr_op::skel_plan() const {
     p.SK = SK.as<int32_t>();
     p.nrow_slots = n_sk;
     p.G = GA.as<double>();
+    p.lds_min = lds_min;
     return p;
 }
 
@@ -1656,7 +1725,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma | kMatvecPanelWide);
             for (int v : cands) {
                 mv_variant[pi][lg] = v;
-                hipEvent_t ev[4];
+                hipEvent_t ev[kMatvecEvents];
                 for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
                 matvec_dev(prec, k, dx.p, dy.p, s);  // warm-up
                 double ph[2] = {0, 0};
@@ -1680,6 +1749,35 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             }
             mv_variant[pi][lg] = pick[0] | pick[1];
             work_gen++;
+            // the skeleton phases' schedule, whole products timed
+            int skel_best = 0;
+            double skel_ms[4] = {0, 0, 0, 0};
+            if (n_skcore && (prec == HODLR_PREC_F32 || k <= 16)) {
+                const int keep = skel_mode;
+                double best_t = 1e300;
+                hipEvent_t ev[kMatvecEvents];
+                for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+                for (int m = HODLR_SKEL_SERIAL; m <= HODLR_SKEL_CAPPED; m++) {
+                    skel_mode = m;
+                    matvec_dev(prec, k, dx.p, dy.p, s);  // warm-up
+                    double t = 0;
+                    for (int r = 0; r < reps; r++) {
+                        matvec_dev(prec, k, dx.p, dy.p, s, ev);
+                        HIP_CHECK(hipEventSynchronize(ev[3]));
+                        float a = 0;
+                        HIP_CHECK(hipEventElapsedTime(&a, ev[0], ev[3]));
+                        t += a;
+                    }
+                    skel_ms[m] = t / reps;
+                    if (t < best_t) {
+                        best_t = t;
+                        skel_best = m;
+                    }
+                }
+                for (auto &e : ev) (void)hipEventDestroy(e);
+                skel_mode = keep;
+                skel_pick[pi][lg] = skel_best;
+            }
             if (verbose())
                 std::fprintf(stderr, "[hodlr] tune %s K=%d: proj %s %.3f ms, panel %s %.3f ms\n",
                              pi == 0 ? "f64" : "f32", k,
@@ -1689,6 +1787,10 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                              pick[1] & kMatvecPanelMfma ? (pick[1] & kMatvecPanelWide ? "mfma-wide" : "mfma")
                                  : (pick[1] & kMatvecPanelVec ? "vec" : (pick[1] ? "flat" : "seg")),
                              best[1] / reps);
+            if (verbose() && skel_best)
+                std::fprintf(stderr, "[hodlr] tune %s K=%d: skeleton schedule %d (whole product: one stream %.3f ms, "
+                             "beside %.3f ms, capped beside %.3f ms)\n", pi == 0 ? "f64" : "f32", k, skel_best,
+                             skel_ms[1], skel_ms[2], skel_ms[3]);
         }
     }
 }

@@ -29,11 +29,14 @@ hipError_t launch_matvec(int nrhs, const ProjItem *pi, int64_t npi, const ProjIt
                          const T *x, T *t, T *part, const int32_t *tidx, T *y, int variant, hipEvent_t *ev,
                          hipStream_t s);
 
-// skeleton blocks' on-the-fly product, added into y (t1, t2: FP64 slot
-// values, part: partial projections; all nrhs-interleaved)
+// skeleton blocks' on-the-fly product into ys (FP64, nrows x nrhs, zeroed
+// here; t1, t2: FP64 slot values, part: partial projections; all
+// nrhs-interleaved); launch_skeleton_add then adds ys into y
 template <typename T>
-hipError_t launch_skeleton(int nrhs, const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2, T *y,
-                           hipStream_t s);
+hipError_t launch_skeleton(int nrhs, const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2,
+                           double *ys, int64_t nrows, hipStream_t s);
+template <typename T>
+hipError_t launch_skeleton_add(T *y, const double *ys, int64_t n, hipStream_t s);
 
 hipError_t launch_bmdot(const double *const *V, int nv, const double *w, const Chunk *chunks, int nchunk,
                         double *partial, const int32_t *pchunk, int nprob, int K, double *out, hipStream_t s);
@@ -79,10 +82,23 @@ hipError_t launch_bgmres_backsolve(int NI, int K, int kmax, int ld, const double
 // device buffers of one matvec's intermediates: the projection's V^T x
 // values and partial-sum slots (operator precision), the skeleton phase's
 // slot values and partial sums (FP64)
+// events of one timed matvec (hodlr_op::matvec_dev)
+constexpr int kMatvecEvents = 6;
+
 struct MatvecWork {
     void *t = nullptr, *part = nullptr;
-    double *skt1 = nullptr, *skt2 = nullptr, *skpart = nullptr;
+    double *skt1 = nullptr, *skt2 = nullptr, *skpart = nullptr, *skys = nullptr;
+    // HODLR_SKEL_SERIAL: the skeleton phases on the calling stream; _OVERLAP
+    // / _CAPPED: on the operator's second stream (which must exist)
+    int skel = HODLR_SKEL_SERIAL;
 };
+
+// dynamic LDS of a skeleton workgroup under HODLR_SKEL_CAPPED: at most 6
+// of them fit in a CU's 160 KiB, leaving 16 KiB for the stored blocks'
+// workgroups (measured on the N = 10^6 grid operator: 20-24 KiB best for
+// FP32, 24-26 KiB for FP64; 16 KiB leaves the projection short of slots,
+// 32 KiB starves the skeleton phases)
+constexpr int kSkelCapLds = 24576;
 
 }  // namespace hodlr
 
@@ -157,8 +173,19 @@ struct hodlr_op {
     std::vector<hodlr::SkelLaunch> skproj_l, skex_l;
     double e_skel = 0;
     int cap_rhs_sk = 0;
-    hodlr::DevBuf skt1, skt2, skpart;
-    hodlr::SkeletonPlan skel_plan() const;
+    hodlr::DevBuf skt1, skt2, skpart, skys;
+    hodlr::SkeletonPlan skel_plan(int lds_min = 0) const;
+    // where a matvec's skeleton phases run (HODLR_SKEL_*): on the calling
+    // stream after the stored blocks' passes, or on `side` (created on first
+    // use) beside them, with all the occupancy they can get or capped so the
+    // stored passes keep most of every CU.  Every schedule gives the same
+    // bits.  skel_mode is the caller's choice (auto: skel_pick, else the
+    // cost model), skel_pick[prec f64/f32][log2 nrhs] the tuner's (0: none)
+    int skel_mode = HODLR_SKEL_AUTO;
+    int skel_pick[2][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};
+    int skel_schedule(int prec, int nrhs) const;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     hodlr_op_stats stats{};
 
@@ -166,9 +193,10 @@ struct hodlr_op {
     std::shared_ptr<void> solver_ws;
     std::shared_ptr<void> batch_ws;
 
-    // live kernel timing: four events per matvec (before the projection,
-    // the panel product, the skeleton product, and after it) recorded on
-    // its stream while profiling is on
+    // live kernel timing: kMatvecEvents events per matvec (before the
+    // projection, before the panel product, after the panel product, after
+    // the whole product, before and after the skeleton phases) recorded on
+    // the streams they run on while profiling is on
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, size_t>> ev_pending;  // (precision, first event index)
@@ -179,7 +207,7 @@ struct hodlr_op {
 
     // FP32: with add_diag false the result omits the diagonal (the FP32
     // copy stores the self blocks with a zero diagonal; callers add diag x).
-    // ev (nullable): four events, see profiling
+    // ev (nullable): kMatvecEvents events, see profiling
     void matvec_dev(int prec, int nrhs, const void *x, void *y, hipStream_t s, hipEvent_t *ev = nullptr,
                     bool add_diag = true);
     void ensure_work(int prec, int nrhs);
@@ -188,8 +216,8 @@ struct hodlr_op {
     void matvec_dev_on(int prec, int nrhs, const void *x, void *y, hipStream_t s, const hodlr::MatvecWork &w,
                        hipEvent_t *ev = nullptr, bool add_diag = true);
     // bytes of the intermediates of one (prec, nrhs) product, in the order
-    // of MatvecWork: t, part, skt1, skt2, skpart (zero where unused)
-    void matvec_work_sizes(int prec, int nrhs, size_t sz[5]) const;
+    // of MatvecWork: t, part, skt1, skt2, skpart, skys (zero where unused)
+    void matvec_work_sizes(int prec, int nrhs, size_t sz[6]) const;
 
     // called by assemble once the exact arena footprint (bytes, all
     // requested precisions) is known and before it is allocated; may block

@@ -128,7 +128,7 @@ struct Engine {
     static double estimate_bytes(const hodlr_op &o, int K, int basis) {
         const double nk = (double)o.N * K;
         double b = 8.0 * nk * 6 + 4.0 * nk * 2 + 8.0 * (double)std::max<int64_t>(o.N, 1) + 8.0 * nk * basis;
-        size_t sz[5];
+        size_t sz[6];
         if (o.cap_rhs32 < K && (o.prec_flags & HODLR_PREC_F32)) {
             o.matvec_work_sizes(HODLR_PREC_F32, K, sz);
             b += (double)(sz[0] + sz[1]);

@@ -16,13 +16,19 @@
 //                                into items whose partial sums are added in
 //                                a fixed order (deterministic, no atomics)
 //   core    t2 = G t1           one wave per block, lanes over G's rows
-//   expand  y_I += A(I, Js) t2  per row node I: two rows per lane, the pivot
+//   expand  ys_I += A(I, Js) t2 per row node I: two rows per lane, the pivot
 //                                columns of every skeleton block (I, .) as
 //                                sources; one launch per tree level, so each
 //                                row has one writer per launch
 //
+// ys is an FP64 vector of its own (zeroed first), added to y once the
+// stored blocks' product has written y (launch_skeleton_add).  The phases
+// touch y only in that last step, so they can run on a second stream while
+// the stored blocks' passes stream their arenas, and the result does not
+// depend on whether they did.
+//
 // Entries are evaluated in FP64 for both operator precisions (an FP32
-// operator reads x and adds into y in FP32): the product through C^+
+// operator reads x in FP32 and rounds y + ys to FP32): the product through C^+
 // cancels large terms, so entries rounded to FP32 would lose the accuracy
 // that FP32 storage of ACA factors keeps.  1/r is the hardware reciprocal
 // square root refined by Newton steps (two for the FP64 operator: about
@@ -263,7 +269,7 @@ __global__ __launch_bounds__(256) void skel_core_kernel(const SkelCore *__restri
 template <typename T, int NR, int ND, int FAM>
 __device__ __forceinline__ void expand_item(const SkelExpand &E, const KernelDev &K,
                                             const int32_t *__restrict__ SKC, const double *__restrict__ t2,
-                                            T *__restrict__ y, int lane, double *lds) {
+                                            double *__restrict__ ys, int lane, double *lds) {
     constexpr int MD = ND > 0 ? ND : kMaxDimDev;
     const int nd = ND > 0 ? ND : K.ndim;
     const bool act0 = lane < E.h, act1 = lane + kWave < E.h;
@@ -289,14 +295,14 @@ __device__ __forceinline__ void expand_item(const SkelExpand &E, const KernelDev
                                                  });
     const double sc = K.scale;
     if (act0) {
-        T *o = y + ip0 * NR;
+        double *o = ys + ip0 * NR;
 #pragma unroll
-        for (int c = 0; c < NR; c++) o[c] = (T)((double)o[c] + sc * acc0[c]);
+        for (int c = 0; c < NR; c++) o[c] = o[c] + sc * acc0[c];
     }
     if (act1) {
-        T *o = y + ip1 * NR;
+        double *o = ys + ip1 * NR;
 #pragma unroll
-        for (int c = 0; c < NR; c++) o[c] = (T)((double)o[c] + sc * acc1[c]);
+        for (int c = 0; c < NR; c++) o[c] = o[c] + sc * acc1[c];
     }
 }
 
@@ -304,79 +310,109 @@ template <typename T, int NR, int ND, int FAM>
 __global__ __launch_bounds__(64) void skel_expand_kernel(const SkelExpand *__restrict__ items,
                                                          const KernelDev *__restrict__ Ks,
                                                          const int32_t *__restrict__ SKC,
-                                                         const double *__restrict__ t2, T *__restrict__ y) {
+                                                         const double *__restrict__ t2, double *__restrict__ ys) {
     extern __shared__ double lds[];
     const SkelExpand E = items[blockIdx.x];
     const KernelDev K = Ks[E.prob];
-    expand_item<T, NR, ND, FAM>(E, K, SKC, t2, y, threadIdx.x, lds);
+    expand_item<T, NR, ND, FAM>(E, K, SKC, t2, ys, threadIdx.x, lds);
+}
+
+// ys = 0 (the expand launches add into it)
+__global__ __launch_bounds__(256) void skel_zero_kernel(double *__restrict__ ys, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) ys[i] = 0.0;
+}
+
+// y += ys, rounded once to the operator precision
+template <typename T>
+__global__ __launch_bounds__(256) void skel_add_kernel(T *__restrict__ y, const double *__restrict__ ys, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        y[i] = (T)((double)y[i] + ys[i]);
+}
+
+inline unsigned stream_blocks(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 8192 ? (b > 0 ? b : 1) : 8192);
 }
 
 inline dim3 wave_grid(int64_t n) { return dim3((unsigned)((n + 3) / 4)); }
 
 template <typename T, int NR, int ND, int FAM>
 void launch_group(bool proj, const SkeletonPlan &p, const SkelLaunch &L, size_t lds, const T *x, double *t1,
-                  double *part, const double *t2, T *y, hipStream_t s) {
+                  double *part, const double *t2, double *ys, hipStream_t s) {
     const dim3 g((unsigned)(L.end - L.begin));
     if (proj)
         skel_proj_kernel<T, NR, ND, FAM><<<g, 64, lds, s>>>(p.proj + L.begin, p.K, p.SK, x, t1, part);
     else
-        skel_expand_kernel<T, NR, ND, FAM><<<g, 64, lds, s>>>(p.ex + L.begin, p.K, p.SK + p.nrow_slots, t2, y);
+        skel_expand_kernel<T, NR, ND, FAM><<<g, 64, lds, s>>>(p.ex + L.begin, p.K, p.SK + p.nrow_slots, t2, ys);
 }
 
 template <typename T, int NR, int FAM>
 void launch_nd(bool proj, const SkeletonPlan &p, const SkelLaunch &L, size_t lds, const T *x, double *t1,
-               double *part, const double *t2, T *y, hipStream_t s) {
+               double *part, const double *t2, double *ys, hipStream_t s) {
     switch (L.key % 16) {
-    case 2: launch_group<T, NR, 2, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
-    case 3: launch_group<T, NR, 3, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
-    case 4: launch_group<T, NR, 4, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
-    default: launch_group<T, NR, 0, FAM>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    case 2: launch_group<T, NR, 2, FAM>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
+    case 3: launch_group<T, NR, 3, FAM>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
+    case 4: launch_group<T, NR, 4, FAM>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
+    default: launch_group<T, NR, 0, FAM>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
     }
 }
 
 template <typename T, int NR>
 void launch_key(bool proj, const SkeletonPlan &p, const SkelLaunch &L, const T *x, double *t1, double *part,
-                const double *t2, T *y, hipStream_t s) {
+                const double *t2, double *ys, hipStream_t s) {
     if (L.end <= L.begin) return;
-    const size_t lds = sizeof(double) * kWave * (size_t)(L.key % 16 + NR);
+    size_t lds = sizeof(double) * kWave * (size_t)(L.key % 16 + NR);
+    if (lds < (size_t)p.lds_min) lds = (size_t)p.lds_min;
     switch (L.key / 16) {
-    case 0: launch_nd<T, NR, 0>(proj, p, L, lds, x, t1, part, t2, y, s); break;
-    case 1: launch_nd<T, NR, 1>(proj, p, L, lds, x, t1, part, t2, y, s); break;
-    default: launch_nd<T, NR, 2>(proj, p, L, lds, x, t1, part, t2, y, s); break;
+    case 0: launch_nd<T, NR, 0>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
+    case 1: launch_nd<T, NR, 1>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
+    default: launch_nd<T, NR, 2>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
     }
 }
 
 template <typename T, int NR>
-hipError_t launch_skeleton_k(const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2, T *y,
-                             hipStream_t s) {
-    for (int i = 0; i < p.nproj_l; i++) launch_key<T, NR>(true, p, p.proj_l[i], x, t1, part, t2, y, s);
+hipError_t launch_skeleton_k(const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2, double *ys,
+                             int64_t nys, hipStream_t s) {
+    skel_zero_kernel<<<stream_blocks(nys), 256, 0, s>>>(ys, nys);
+    for (int i = 0; i < p.nproj_l; i++) launch_key<T, NR>(true, p, p.proj_l[i], x, t1, part, t2, ys, s);
     if (p.nred) skel_reduce_kernel<NR><<<wave_grid(p.nred), 256, 0, s>>>(p.red, p.nred, part, t1);
     if (p.ncore) skel_core_kernel<NR><<<wave_grid(p.ncore), 256, 0, s>>>(p.core, p.ncore, p.G, t1, t2);
-    for (int i = 0; i < p.nex_l; i++) launch_key<T, NR>(false, p, p.ex_l[i], x, t1, part, t2, y, s);
+    for (int i = 0; i < p.nex_l; i++) launch_key<T, NR>(false, p, p.ex_l[i], x, t1, part, t2, ys, s);
     return hipGetLastError();
 }
 
 }  // namespace
 
 template <typename T>
-hipError_t launch_skeleton(int nrhs, const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2, T *y,
-                           hipStream_t s) {
+hipError_t launch_skeleton(int nrhs, const SkeletonPlan &p, const T *x, double *t1, double *part, double *t2,
+                           double *ys, int64_t nrows, hipStream_t s) {
+    const int64_t n = nrows * nrhs;
     switch (nrhs) {
-    case 1: return launch_skeleton_k<T, 1>(p, x, t1, part, t2, y, s);
-    case 2: return launch_skeleton_k<T, 2>(p, x, t1, part, t2, y, s);
-    case 4: return launch_skeleton_k<T, 4>(p, x, t1, part, t2, y, s);
-    case 8: return launch_skeleton_k<T, 8>(p, x, t1, part, t2, y, s);
-    case 16: return launch_skeleton_k<T, 16>(p, x, t1, part, t2, y, s);
+    case 1: return launch_skeleton_k<T, 1>(p, x, t1, part, t2, ys, n, s);
+    case 2: return launch_skeleton_k<T, 2>(p, x, t1, part, t2, ys, n, s);
+    case 4: return launch_skeleton_k<T, 4>(p, x, t1, part, t2, ys, n, s);
+    case 8: return launch_skeleton_k<T, 8>(p, x, t1, part, t2, ys, n, s);
+    case 16: return launch_skeleton_k<T, 16>(p, x, t1, part, t2, ys, n, s);
     case 32:
-        if constexpr (sizeof(T) == 4) return launch_skeleton_k<T, 32>(p, x, t1, part, t2, y, s);
+        if constexpr (sizeof(T) == 4) return launch_skeleton_k<T, 32>(p, x, t1, part, t2, ys, n, s);
         return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }
 
+template <typename T>
+hipError_t launch_skeleton_add(T *y, const double *ys, int64_t n, hipStream_t s) {
+    skel_add_kernel<T><<<stream_blocks(n), 256, 0, s>>>(y, ys, n);
+    return hipGetLastError();
+}
+
 template hipError_t launch_skeleton<double>(int, const SkeletonPlan &, const double *, double *, double *, double *,
-                                            double *, hipStream_t);
+                                            double *, int64_t, hipStream_t);
 template hipError_t launch_skeleton<float>(int, const SkeletonPlan &, const float *, double *, double *, double *,
-                                           float *, hipStream_t);
+                                           double *, int64_t, hipStream_t);
+template hipError_t launch_skeleton_add<double>(double *, const double *, int64_t, hipStream_t);
+template hipError_t launch_skeleton_add<float>(float *, const double *, int64_t, hipStream_t);
 
 }  // namespace hodlr

@@ -316,14 +316,34 @@ int hodlr_op_tune_matvec(hodlr_op *op, int prec_mask, int reps);
 int hodlr_op_set_matvec_variant(hodlr_op *op, int prec, int nrhs, int variant);
 int hodlr_op_get_matvec_variant(hodlr_op *op, int prec, int nrhs, int *variant);
 
+/* ---- skeleton schedule (no reference counterpart) ----
+ * Where a matvec of an operator with skeleton blocks (sys_size >= 0)
+ * evaluates them: 1 = on the calling stream after the stored blocks'
+ * passes; 2 = on a second stream of the operator, beside the stored passes
+ * (compute-bound work beside memory-bound work); 3 = as 2 with each
+ * skeleton workgroup holding 24 KiB of LDS, so at most 6 of them share a CU
+ * with the stored passes' workgroups; 0 (default) = the tuner's pick for
+ * that (prec, nrhs), else a cost model of stored bytes against skeleton
+ * entries.  Every schedule gives bit-identical results.  Products under HIP
+ * graph capture and hodlr_matvec_device_noalloc use 1.  get returns the
+ * schedule a (prec, nrhs) product takes; hodlr_op_tune_matvec times 1-3. */
+#define HODLR_SKEL_AUTO 0
+#define HODLR_SKEL_SERIAL 1
+#define HODLR_SKEL_OVERLAP 2
+#define HODLR_SKEL_CAPPED 3
+int hodlr_op_set_skeleton_schedule(hodlr_op *op, int mode);
+int hodlr_op_get_skeleton_schedule(hodlr_op *op, int prec, int nrhs, int *mode);
+
 /* ---- measurement ---- */
 /* Times `reps` back-to-back matvecs on the op's stream with HIP events per
  * kernel; ms[0] = whole matvec, ms[1] = projection kernel (V^T x),
  * ms[2] = panel kernel (near field + U t), ms[3] = skeleton blocks' on-the-fly
- * product (0 without skeleton blocks), averages per launch. */
+ * product from its first to its last kernel, on whichever stream it ran (0
+ * without skeleton blocks; beside the stored passes ms[0] < the sum),
+ * averages per launch. */
 int hodlr_matvec_timing(hodlr_op *op, int prec, int nrhs, int reps, double *ms);
 /* Live timing: while enabled, every matvec records HIP events around its
- * projection, panel and skeleton launches on the stream it runs on.  read
+ * projection, panel and skeleton launches on the streams they run on.  read
  * returns the summed milliseconds ms[6] = {f64 proj, f64 panel, f32 proj,
  * f32 panel, f64 skeleton blocks, f32 skeleton blocks} and counts[2] =
  * matvecs timed {f64, f32} since the last reset. */

@@ -222,6 +222,7 @@ struct SkeletonPlan {
     const int32_t *SK;    // row slots, then column slots at SK + nrow_slots
     int64_t nrow_slots;
     const double *G;
+    int lds_min;          // dynamic LDS of a proj / expand workgroup at least this many bytes
 };
 
 // rows of a batched vector that belong to one problem (never crosses a

@@ -18,7 +18,10 @@ Bars:
     at sys_size <= Nmax it is ~60x less accurate, in the oracle as here;
   * FP32 operator vs FP64: 1e-5; multi-RHS columns vs single-RHS: 1e-13;
   * a block-diagonal batch equals each problem assembled alone, bit for bit;
-  * solves reach the tolerance; operator files round-trip bit for bit.
+  * solves reach the tolerance; operator files round-trip bit for bit;
+  * every schedule of the skeleton phases (after the stored passes on the
+    calling stream, beside them on a second stream, beside them with capped
+    occupancy, the cost model's or the tuner's pick) gives the same bits.
 """
 import math
 import sys
@@ -192,3 +195,48 @@ def test_skeleton_operator_file(tmp_path):
     path2 = tmp_path / "sk2.op"
     B.save(path2)
     assert path.read_bytes() == path2.read_bytes()
+
+
+def test_skeleton_schedules_bit_identical():
+    say("case schedules")
+    n, nd = 6000, 3
+    pts = O.splitmix_uniform(81, n * nd).reshape(n, nd)
+    T = H.Tree(pts, -np.ones(nd), np.ones(nd), 32, 0)
+    A = H.Operator(T, H.Kernel(H.KERNEL_INV_R, nd, 1.0 / n, 1.0), 1e-8, sys_size=64)
+    assert A.stats()["n_skeleton"] > 0
+    X = np.stack([O.splitmix_uniform(82 + c, n) for c in range(32)], axis=1)
+    modes = (H.SKEL_SERIAL, H.SKEL_OVERLAP, H.SKEL_CAPPED, H.SKEL_AUTO)
+    for prec, ks in ((H.PREC_F64, (1, 4, 16)), (H.PREC_F32, (1, 8, 32))):
+        for k in ks:
+            Xk = X[:, :k] if k > 1 else X[:, 0]
+            ys = {}
+            for m in modes:
+                A.set_skeleton_schedule(m)
+                ys[m] = [A.matvec(Xk, prec=prec) for _ in range(2)]
+            A.set_skeleton_schedule(H.SKEL_AUTO)
+            say("prec", prec, "k", k, "auto schedule", A.skeleton_schedule(prec, k))
+            assert A.skeleton_schedule(prec, k) in (H.SKEL_OVERLAP, H.SKEL_CAPPED)
+            for m in modes:
+                for y in ys[m]:
+                    assert np.array_equal(y, ys[H.SKEL_SERIAL][0]), (prec, k, m)
+    b = O.splitmix_uniform(90, n)
+    sols = []
+    for m in (H.SKEL_SERIAL, H.SKEL_CAPPED):
+        A.set_skeleton_schedule(m)
+        x, info = A.gmres(b, tol=1e-8, mode=H.SOLVE_MIXED, x0=np.zeros(n))
+        assert info["converged"] and info["true_resid"] < 1e-8
+        sols.append((x, info["iterations"]))
+    assert sols[0][1] == sols[1][1] and np.array_equal(sols[0][0], sols[1][0])
+    A.set_skeleton_schedule(H.SKEL_AUTO)
+    A.tune(reps=1)
+    for prec, k in ((H.PREC_F64, 1), (H.PREC_F32, 1), (H.PREC_F32, 8)):
+        m = A.skeleton_schedule(prec, k)
+        say("tuned schedule prec", prec, "k", k, m)
+        assert m in (H.SKEL_SERIAL, H.SKEL_OVERLAP, H.SKEL_CAPPED)
+    assert np.array_equal(A.matvec(X[:, :8], prec=H.PREC_F32), A.matvec(X[:, :8], prec=H.PREC_F32))
+    A.set_skeleton_schedule(H.SKEL_SERIAL)
+    y_serial = A.matvec(X[:, :8], prec=H.PREC_F32)
+    A.set_skeleton_schedule(H.SKEL_AUTO)
+    assert np.array_equal(A.matvec(X[:, :8], prec=H.PREC_F32), y_serial)
+    with pytest.raises(H.HodlrError):
+        A.set_skeleton_schedule(7)
