@@ -1034,8 +1034,10 @@ def main():
     t_asm = time.time() - t0
     st = op.stats()
     log("assembled in %.2fs (sys_size %d): e_near %.3g e_far %.3g, %d skeleton blocks (%.3g entries on the fly), "
-        "%.1f GB FP64 + %.1f GB FP32" % (t_asm, sys_size, st["e_near"], st["e_far"], st["n_skeleton"], st["e_skel"],
-                                        st["bytes_f64"] * 1e-9, st["bytes_f32"] * 1e-9))
+        "%.1f GB FP64 + %.1f GB FP32; ACA rank pass %.2fs, %.3g entries (%.3g/s)"
+        % (t_asm, sys_size, st["e_near"], st["e_far"], st["n_skeleton"], st["e_skel"], st["bytes_f64"] * 1e-9,
+           st["bytes_f32"] * 1e-9, st["rank_seconds"], st["aca_entries"],
+           st["aca_entries"] / st["rank_seconds"] if st["rank_seconds"] else 0.0))
 
     if args.matvec_only:
         for prec in (H.PREC_F64, H.PREC_F32):
@@ -1198,6 +1200,10 @@ def main():
         "skeleton": skeleton,
         "assemble_s": t_asm,
         "aca_rank_pass_s": st["rank_seconds"],
+        # the rank pass's kernel entries (each row and column the ACA
+        # evaluates, retries included) over its wall time
+        "aca_entries": st["aca_entries"],
+        "aca_entries_per_s": st["aca_entries"] / st["rank_seconds"] if st["rank_seconds"] else None,
         "fill_s": st["fill_seconds"],
         "operator_gb": {"f64": st["bytes_f64"] * 1e-9, "f32": st["bytes_f32"] * 1e-9},
         "stored_form_gb": {"f64": (st["bytes_f64"] - st["bytes_core"] + 8.0 * st["e_skel"]) * 1e-9,

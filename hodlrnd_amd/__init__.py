@@ -59,7 +59,7 @@ class OpStats(C.Structure):
                 ("matvec_bytes_f64", C.c_double), ("matvec_bytes_f32", C.c_double),
                 ("assemble_seconds", C.c_double), ("rank_seconds", C.c_double),
                 ("fill_seconds", C.c_double), ("n_skeleton", C.c_int64), ("e_skel", C.c_double),
-                ("bytes_core", C.c_double)]
+                ("bytes_core", C.c_double), ("aca_entries", C.c_double)]
 
 
 class GmresParams(C.Structure):

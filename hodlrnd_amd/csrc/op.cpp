@@ -67,6 +67,18 @@ int aca_block_size(int m, int n) {
 
 bool aca_in_registers(int cls) { return cls < 64; }
 
+// LDS a workgroup ACA may take for its block's points ((m + n)(NDIM + 1) 8
+// bytes, plus 8 per column with weights): the 64 class always fits
+// (m + n <= 256); the 256 class up to 32 KiB (about 1000 points in 3-D,
+// four workgroups per CU); the 1024 class (max(m, n) > 4096) never.
+// Measured on the N = 10^6 grid and a 200k-point build (kernel time): 256
+// class 790 -> 772 ms (16 KiB: 774 ms), 64 class 99 -> 84 ms
+int64_t aca_pts_lds_limit(int cls) {
+    if (cls == 64) return 24576;
+    if (cls == 256) return 32768;
+    return 0;
+}
+
 // a block that may become a skeleton runs in aca_kernel in both passes
 // (only that kernel writes pivot positions)
 int aca_class(const hodlr_op &op, int64_t b) {
@@ -220,7 +232,7 @@ void core_pinv(int r, std::vector<double> &a, double *G) {
 // that the scratch pool stays within `budget` doubles.
 void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vector<int32_t> &cap, int fill,
                   DevBuf &pool, int64_t budget, DevBuf &d_rank, DevBuf &d_flags, const DevBuf *d_dest,
-                  hipStream_t s, const AcaStage &stage = AcaStage{nullptr, nullptr, 0, nullptr},
+                  hipStream_t s, const AcaStage &stage = AcaStage{nullptr, nullptr, 0, nullptr, nullptr, 0},
                   const std::vector<uint8_t> *stageable = nullptr) {
     // group by kernel class (largest class first), large blocks first
     // inside a class: bucket by class, then a counting sort on m + n
@@ -300,11 +312,25 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
         });
         if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double), s);
         upload(d_blocks, chunk, s, true);
+        // the workgroup forms stage a block's points in LDS when they fit
+        // in the chunk's allowance (the wave forms always do, statically)
+        AcaStage st = stage;
+        st.pts_lds = 0;
+        if (!aca_in_registers(bs)) {
+            int64_t want = 0;
+            for (size_t q = 0; q < chunk.size(); q++) {
+                const int p = chunk[q].prob;
+                const int64_t rec = (int64_t)(chunk[q].m + chunk[q].n) * (op.kds[p].ndim + 1) * 8 +
+                                    (op.w_off[p] >= 0 ? (int64_t)chunk[q].n * 8 : 0);
+                want = std::max(want, rec);
+            }
+            st.pts_lds = (int32_t)std::min<int64_t>(want, aca_pts_lds_limit(bs));
+        }
         HIP_CHECK(op.entries->aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), op.kdev.as<KernelDev>(), op.eps,
                              pool.as<double>(), d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
                              d_dest ? d_dest->as<FillDest>() : nullptr, op.leaf_start.as<int64_t>(),
                              op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), op.VP64.as<double>(),
-                             op.SK.as<int32_t>(), stage, s));
+                             op.SK.as<int32_t>(), st, s));
         HIP_CHECK(hipStreamSynchronize(s));
         if (verbose())
             std::fprintf(stderr, "[hodlr] aca %s pass: %zu/%zu blocks (workgroup %d, scratch %.2f GB)\n",
@@ -534,13 +560,20 @@ void assemble(hodlr_op &op) {
                 HIP_CHECK(hipMemsetAsync(stage_cur.p, 0, sizeof(unsigned long long), s));
                 HIP_CHECK(hipMemsetAsync(stage_off.p, 0xff, sizeof(int64_t) * (size_t)(nb > 0 ? nb : 1), s));
                 stage = AcaStage{stage_buf.as<double>(), stage_cur.as<unsigned long long>(), capd,
-                                 stage_off.as<int64_t>()};
+                                 stage_off.as<int64_t>(), nullptr, 0};
             } catch (const ArgError &) {  // no room: the fill pass recomputes every block
                 stage_buf.release();
                 stage_cur.release();
                 stage_off.release();
             }
         }
+    }
+    // entries each block's rank pass evaluates, added up over its retries
+    DevBuf d_evals;
+    if (!todo.empty()) {
+        d_evals.alloc(sizeof(int64_t) * (size_t)nb, s);
+        HIP_CHECK(hipMemsetAsync(d_evals.p, 0, sizeof(int64_t) * (size_t)nb, s));
+        stage.evals = d_evals.as<int64_t>();
     }
     // Retries (blocks whose rank outgrew the first budget of 128) are few
     // and large: 4-D d' = 3 puts 240 blocks of 4096 x 4096 with ranks up
@@ -646,6 +679,15 @@ void assemble(hodlr_op &op) {
         soff.resize(nb);
         HIP_CHECK(hipMemcpyAsync(soff.data(), stage.off, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
         HIP_CHECK(hipStreamSynchronize(s));
+    }
+    double aca_entries = 0.0;
+    if (d_evals.p) {
+        std::vector<int64_t> ev(nb);
+        HIP_CHECK(hipMemcpyAsync(ev.data(), d_evals.p, sizeof(int64_t) * nb, hipMemcpyDeviceToHost, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        for (int64_t v : ev) aca_entries += (double)v;
+        d_evals.release();
+        stage.evals = nullptr;
     }
     const double t_rank1 = now_s();
     if (verbose()) std::fprintf(stderr, "[hodlr] rank pass %.3f s, %lld blocks\n", t_rank1 - t_rank0, (long long)nb);
@@ -1472,6 +1514,7 @@ void assemble(hodlr_op &op) {
     op.e_far = st.e_far;
     st.n_skeleton = op.n_skel;
     st.e_skel = op.e_skel;
+    st.aca_entries = aca_entries;
     // cores (FP64, shared by both precisions) and pivot positions
     st.bytes_core = 8.0 * (double)op.n_g + 8.0 * (double)op.n_sk;
     st.bytes_f64 = 8.0 * (double)(op.n_np + op.n_up + op.n_vp) + st.bytes_core;

@@ -65,7 +65,7 @@ __device__ __forceinline__ double entry(const KernelDev &k, int64_t i, int64_t j
 
 // layout fingerprint of the descriptors passed to the launchers: the library
 // refuses a table built against other layouts
-constexpr uint32_t kEntryAbiVersion = 1;
+constexpr uint32_t kEntryAbiVersion = 2;
 constexpr uint32_t entry_abi_layout() {
     return (uint32_t)(sizeof(AcaBlock) | sizeof(FillDest) << 8 | sizeof(DenseBlock) << 16 | sizeof(NearLeaf) << 24) ^
            (uint32_t)(sizeof(KernelDev) * 2654435761u) ^ (uint32_t)(sizeof(AcaStage) << 4);

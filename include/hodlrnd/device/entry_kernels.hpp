@@ -257,6 +257,28 @@ __device__ __forceinline__ int mask_mid_gap(uint64_t mask) {
     return at + (int)(mx / 2);
 }
 
+// v - a_0 b_0 - a_1 b_1 - ... - a_{r-1} b_{r-1}, q ascending, each product
+// rounded before its difference (Eigen's row - u * v, LowRank_matrix.hpp:
+// 130,240), a_q = A[q sa], b_q = B[q sb].  The loads of eight terms are
+// issued before their eight updates, so a residual entry waits for r / 8
+// memory round trips instead of r.
+__device__ __forceinline__ double aca_residual(double v, const double *A, int64_t sa, const double *B, int64_t sb,
+                                               int r) {
+    int q = 0;
+    for (; q + 8 <= r; q += 8) {
+        double a[8], b[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            a[k] = A[(int64_t)(q + k) * sa];
+            b[k] = B[(int64_t)(q + k) * sb];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) v = v - a[k] * b[k];
+    }
+    for (; q < r; q++) v = v - A[(int64_t)q * sa] * B[(int64_t)q * sb];
+    return v;
+}
+
 template <class Fn, int BS>
 __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ blocks,
                                                  const KernelDev *__restrict__ Ks,
@@ -269,12 +291,34 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
                                                  double *__restrict__ UP, double *__restrict__ VP,
                                                  int32_t *__restrict__ SK, AcaStage S) {
     __shared__ Shared<BS> sh;
+    extern __shared__ double pts_lds[];
     const AcaBlock B = blocks[blockIdx.x];
     const KernelDev K = Ks[B.prob];
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     const int m = B.m, n = B.n, cap = B.cap;
     const int mn = m < n ? m : n;
+    // the block's points in LDS when they fit in S.pts_lds bytes: row then
+    // column coordinates, column weights, row then column point ids, so a
+    // residual row or column reads them without the order-map and
+    // coordinate gathers (the same values: the entries are bit-identical)
+    const int nd = K.ndim;
+    const bool staged = (int64_t)(m + n) * (nd + 1) * 8 + (K.W ? (int64_t)n * 8 : 0) <= (int64_t)S.pts_lds;
+    double *sxr = pts_lds, *sxc = sxr + (size_t)m * nd, *swc = sxc + (size_t)n * nd;
+    int64_t *sgr = reinterpret_cast<int64_t *>(swc + (K.W ? n : 0)), *sgc = sgr + m;
+    if (staged) {
+        for (int i = tid; i < m; i += BS) {
+            const int64_t p = B.ord[B.row0l + i];
+            sgr[i] = p;
+            for (int d = 0; d < nd; d++) sxr[(size_t)i * nd + d] = K.X[(p - K.pos0) * nd + d];
+        }
+        for (int j = tid; j < n; j += BS) {
+            const int64_t p = B.ord[B.col0l + j];
+            sgc[j] = p;
+            for (int d = 0; d < nd; d++) sxc[(size_t)j * nd + d] = K.X[(p - K.pos0) * nd + d];
+            if (K.W) swc[j] = K.W[p - K.pos0];
+        }
+    }
     double *U = pool + B.scratch;
     double *V = U + (size_t)m * cap;
     double *row = V + (size_t)n * cap;
@@ -292,18 +336,22 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     __syncthreads();
     int rrem = m - 1, crem = n, nri = 1, nci = 0;
     int r = 0, pivot = 0, flags = 0;
+    int64_t nrow_ev = 0, ncol_ev = 0;  // rows and columns evaluated
     double maxv = 0.0, mnorm = 0.0, rnorm = 0.0, cnorm = 0.0, gamma = 0.0;
     uint64_t rng = B.seed;
 
     // residual row ri of the block (LowRank_matrix.hpp:126-133); argmax over
     // the columns not yet used as pivots
     auto eval_row = [&](int ri) {
-        const int64_t pi = B.ord[B.row0l + ri];
+        nrow_ev++;
+        const int64_t pi = staged ? sgr[ri] : B.ord[B.row0l + ri];
+        const double *xa = sxr + (size_t)ri * nd;
         double bv = 0.0, ba = -1.0;
         int bi = INT_MAX;
         for (int j = tid; j < n; j += BS) {
-            double v = entry_t<Fn>(K, pi, B.ord[B.col0l + j]);
-            for (int q = 0; q < r; q++) v = v - U[(size_t)q * m + ri] * V[(size_t)q * n + j];
+            double v = staged ? entry_xy<Fn>(K, xa, sxc + (size_t)j * nd, pi, sgc[j], K.W ? swc[j] : 0.0)
+                              : entry_t<Fn>(K, pi, B.ord[B.col0l + j]);
+            v = aca_residual(v, U + ri, m, V + j, n, r);
             row[j] = v;
             if (!cused[j]) {
                 double a = fabs(v);
@@ -321,12 +369,16 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     };
     // residual column cj (LowRank_matrix.hpp:237-243)
     auto eval_col = [&](int cj) {
-        const int64_t pj = B.ord[B.col0l + cj];
+        ncol_ev++;
+        const int64_t pj = staged ? sgc[cj] : B.ord[B.col0l + cj];
+        const double *xb = sxc + (size_t)cj * nd;
+        const double wb = staged && K.W ? swc[cj] : 0.0;
         double bv = 0.0, ba = -1.0;
         int bi = INT_MAX;
         for (int i = tid; i < m; i += BS) {
-            double v = entry_t<Fn>(K, B.ord[B.row0l + i], pj);
-            for (int q = 0; q < r; q++) v = v - V[(size_t)q * n + cj] * U[(size_t)q * m + i];
+            double v = staged ? entry_xy<Fn>(K, sxr + (size_t)i * nd, xb, sgr[i], pj, wb)
+                              : entry_t<Fn>(K, B.ord[B.row0l + i], pj);
+            v = aca_residual(v, V + cj, n, U + i, m, r);
             col[i] = v;
             if (!rused[i]) {
                 double a = fabs(v);
@@ -443,9 +495,37 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
         mnorm += fabs(gamma * gamma * rs * cs);
         for (int q0 = 0; q0 < r; q0 += kWave) {
             const int qn = min(kWave, r - q0);
-            for (int qq = 0; qq < qn; qq++) {
-                const double *uq = U + (size_t)(q0 + qq) * m, *ur = U + (size_t)r * m;
-                const double *vq = V + (size_t)(q0 + qq) * n, *vr = V + (size_t)r * n;
+            const double *ur = U + (size_t)r * m, *vr = V + (size_t)r * n;
+            int qq = 0;
+            // eight q at a time: the same per-thread sums (i ascending from
+            // tid) and wave trees as one at a time, their loads and
+            // shuffles interleaved
+            for (; qq + 8 <= qn; qq += 8) {
+                double du[8], dv[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) du[k] = dv[k] = 0.0;
+                for (int i = tid; i < m; i += BS) {
+                    const double x = ur[i];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) du[k] = fma(U[(size_t)(q0 + qq + k) * m + i], x, du[k]);
+                }
+                for (int j = tid; j < n; j += BS) {
+                    const double x = vr[j];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) dv[k] = fma(V[(size_t)(q0 + qq + k) * n + j], x, dv[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    du[k] = wave_sum(du[k]);
+                    dv[k] = wave_sum(dv[k]);
+                }
+                if (lane == 0)
+#pragma unroll
+                    for (int k = 0; k < 8; k++) { sh.du[wid][qq + k] = du[k]; sh.dv[wid][qq + k] = dv[k]; }
+            }
+            for (; qq < qn; qq++) {
+                const double *uq = U + (size_t)(q0 + qq) * m;
+                const double *vq = V + (size_t)(q0 + qq) * n;
                 double du = 0.0, dv = 0.0;
                 for (int i = tid; i < m; i += BS) du = fma(uq[i], ur[i], du);
                 for (int j = tid; j < n; j += BS) dv = fma(vq[j], vr[j], dv);
@@ -470,6 +550,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     if (tid == 0) {
         out_rank[B.id] = r;
         out_flags[B.id] = flags | (dense ? 1 : 0);
+        if (S.evals) S.evals[B.id] += nrow_ev * n + ncol_ev * m;
     }
     if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors
         const int64_t len = (int64_t)r * (m + n);
@@ -573,10 +654,12 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
     int r = 0, pivot = 0, flags = 0;
     double maxv = 0.0, mnorm = 0.0, rnorm = 0.0, cnorm = 0.0, gamma = 0.0;
     double rowv = 0.0, colv = 0.0;                 // residual row (lane j) and column (lane i)
+    int64_t nrow_ev = 0, ncol_ev = 0;              // rows and columns evaluated
     uint64_t rng = B.seed;
 
     // residual row ri (LowRank_matrix.hpp:126-133)
     auto eval_row = [&](int ri) {
+        nrow_ev++;
         const double *a = xr + ri * kMaxDimDev;
         double v = l < n ? entry_xy<Fn>(K, a, xc + l * kMaxDimDev, prow[ri], pj, wj) : 0.0;
 #pragma unroll
@@ -599,6 +682,7 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
     };
     // residual column cj (LowRank_matrix.hpp:237-243)
     auto eval_col = [&](int cj) {
+        ncol_ev++;
         const double *b = xc + cj * kMaxDimDev;
         const int64_t pjc = __shfl(pj, cj, kWave);
         const double wc = __shfl(wj, cj, kWave);
@@ -714,6 +798,7 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
     if (l == 0) {
         out_rank[B.id] = r;
         out_flags[B.id] = flags | (dense ? 1 : 0);
+        if (S.evals) S.evals[B.id] += nrow_ev * n + ncol_ev * m;
     }
     if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors
         const int64_t len = (int64_t)r * (m + n);
@@ -826,6 +911,7 @@ hipError_t launch_aca_t(int bs, const AcaBlock *blocks, int64_t nblocks, const K
     using namespace entry_detail;
     if (nblocks == 0) return hipSuccess;
     dim3 g((unsigned)nblocks);
+    const size_t lds = S.pts_lds > 0 ? (size_t)S.pts_lds : 0;
     switch (bs) {
     case 8:
         aca_wave_kernel<Fn, 8><<<g, 64, 0, s>>>(blocks, K, tol, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP, S);
@@ -839,16 +925,16 @@ hipError_t launch_aca_t(int bs, const AcaBlock *blocks, int64_t nblocks, const K
                                                  S);
         break;
     case 64:
-        aca_kernel<Fn, 64><<<g, 64, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP, VP,
-                                            SK, S);
-        break;
-    case 256:
-        aca_kernel<Fn, 256><<<g, 256, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP,
+        aca_kernel<Fn, 64><<<g, 64, lds, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP,
                                               VP, SK, S);
         break;
-    default:
-        aca_kernel<Fn, 1024><<<g, 1024, 0, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP,
+    case 256:
+        aca_kernel<Fn, 256><<<g, 256, lds, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt, UP,
                                                 VP, SK, S);
+        break;
+    default:
+        aca_kernel<Fn, 1024><<<g, 1024, lds, s>>>(blocks, K, tol, pool, rank, flags, fill, dest, leaf_start, leaf_cnt,
+                                                  UP, VP, SK, S);
         break;
     }
     return hipGetLastError();

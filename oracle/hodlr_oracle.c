@@ -500,6 +500,7 @@ static int aca_grow(const or_kernel_t *k, const double *pts,
     int nri = 0, nci = 0;
     uint64_t rng = seed;
     int retries = 0;
+    int64_t rows_ev = 0, cols_ev = 0; /* rows and columns evaluated */
 
     row_ind[nri++] = 0; /* :96-97 */
     rused[0] = 1;
@@ -510,6 +511,7 @@ static int aca_grow(const or_kernel_t *k, const double *pts,
 
 #define EVAL_ROW(ri)                                                          \
     do {                                                                      \
+        rows_ev++;                                                            \
         for (int j_ = 0; j_ < n; j_++) {                                      \
             double v_ = kernel_value(k, pts, rows[(ri)], cols[j_]);           \
             for (int q_ = 0; q_ < r; q_++)                                    \
@@ -519,6 +521,7 @@ static int aca_grow(const or_kernel_t *k, const double *pts,
     } while (0)
 #define EVAL_COL(cj)                                                          \
     do {                                                                      \
+        cols_ev++;                                                            \
         for (int i_ = 0; i_ < m; i_++) {                                      \
             double v_ = kernel_value(k, pts, rows[i_], cols[(cj)]);           \
             for (int q_ = 0; q_ < r; q_++)                                    \
@@ -616,6 +619,7 @@ static int aca_grow(const or_kernel_t *k, const double *pts,
         info->dense = dense;
         info->rank = dense ? mn : r;
         info->retries = retries;
+        info->entries = rows_ev * n + cols_ev * m;
     }
     free(rused); free(cused); free(row_ind); free(col_ind); free(tmp); free(row); free(col);
     return dense ? mn : r;
@@ -744,6 +748,7 @@ typedef struct {
     int64_t *rb, *cb;     /* skeleton bases (point ids) */
     or_cpqr_t qr;
     int built;
+    int64_t aca_entries;  /* entries its ACA evaluated */
 } or_block_t;
 
 typedef struct {
@@ -795,6 +800,7 @@ static void build_block(or_op_t *op, or_block_t *b) {
     b->m = m;
     b->n = n;
     b->rank = info.rank;
+    b->aca_entries = info.entries;
     b->L = b->R = NULL;
     b->rb = b->cb = NULL;
     if (info.dense) {
@@ -1054,6 +1060,7 @@ void or_op_get_stats(const or_op_t *op, or_op_stats_t *s) {
         const or_block_t *b = &op->blk[i];
         if (b->built != 1) continue;
         s->n_blocks++;
+        s->aca_entries += (double)b->aca_entries;
         s->n_flop_ref += (double)b->rank * (double)(b->m + b->n);
         if ((int64_t)b->rank > s->max_rank) s->max_rank = b->rank;
         if (b->kind == BLK_DENSE) { s->n_dense++; s->e_far += (double)b->m * (double)b->n; }

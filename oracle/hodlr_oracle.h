@@ -90,6 +90,7 @@ typedef struct {
     int32_t dense;      /* 1 if the block fell back to dense storage */
     int32_t steps;      /* computed_rank before the fallback decision */
     int32_t retries;    /* row + column retry iterations taken */
+    int64_t entries;    /* kernel entries evaluated: rows evaluated x n + columns x m */
 } or_aca_info_t;
 
 uint64_t or_block_seed(int level, int64_t row_node, int64_t col_node);
@@ -112,6 +113,7 @@ typedef struct {
     int64_t n_dense;       /* of which dense fallback */
     int64_t n_skeleton;    /* of which skeleton (sys_size) blocks */
     double init_seconds;
+    double aca_entries;    /* kernel entries the blocks' ACA evaluated (or_aca_info_t.entries summed) */
 } or_op_stats_t;
 
 /* sys_size < 0: store every factor (the device path's form);

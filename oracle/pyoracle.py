@@ -23,13 +23,13 @@ class Kernel(C.Structure):
 
 class AcaInfo(C.Structure):
     _fields_ = [("rank", C.c_int32), ("dense", C.c_int32), ("steps", C.c_int32),
-                ("retries", C.c_int32)]
+                ("retries", C.c_int32), ("entries", C.c_int64)]
 
 
 class OpStats(C.Structure):
     _fields_ = [("n_flop_ref", C.c_double), ("max_rank", C.c_int64), ("e_near", C.c_double),
                 ("e_far", C.c_double), ("n_blocks", C.c_int64), ("n_dense", C.c_int64),
-                ("n_skeleton", C.c_int64), ("init_seconds", C.c_double)]
+                ("n_skeleton", C.c_int64), ("init_seconds", C.c_double), ("aca_entries", C.c_double)]
 
 
 class GmresInfo(C.Structure):
@@ -238,7 +238,7 @@ def aca(k, pts, rows, cols, eps, seed=0):
     lib.or_aca(k.ref, _p(f64(pts)), _p(rows), m, _p(cols), n, eps, seed, cap,
                _p(U), _p(V), _p(rp), _p(cp), C.byref(info))
     r = info.steps
-    return dict(rank=info.rank, dense=bool(info.dense), steps=r, retries=info.retries,
+    return dict(rank=info.rank, dense=bool(info.dense), steps=r, retries=info.retries, entries=info.entries,
                 U=U[:r].T.copy(), V=V[:r].T.copy(), row_piv=rp[:r].copy(), col_piv=cp[:r].copy())
 
 

@@ -224,6 +224,11 @@ def test_aca_separable_kernel_rank_one():
     assert a["rank"] == 1 and not a["dense"]
     K = O.dense_matrix(k, pts, rows, cols)
     assert np.abs(K - a["U"] @ a["V"].T).max() < 1e-14 * np.abs(K).max()
+    # entries evaluated (LowRank_matrix.hpp:126-225): row 0 and its pivot
+    # column; the next pivot row's residual is below the floor, and its 10
+    # retries (:144-216) each evaluate a row before the loop breaks: 12 rows
+    # of n = 50 and 1 column of m = 40
+    assert a["retries"] == 10 and a["entries"] == 12 * 50 + 40
 
 
 def test_aca_cross_reproduces_pivots_and_error():
@@ -289,6 +294,7 @@ def test_aca_dense_fallback_and_tiny_blocks():
     assert a["dense"] and a["rank"] == 5  # r >= min(m, n) - 1
     a1 = O.aca(k, pts, np.arange(1), np.arange(5, 10), 1e-14)
     assert a1["dense"] and a1["rank"] == 1  # one row: no column pivot step
+    assert a1["entries"] == 5  # that row only (no row left to pivot: :220-225)
 
 
 def test_aca_all_below_floor_gives_rank_zero():
@@ -299,6 +305,7 @@ def test_aca_all_below_floor_gives_rank_zero():
     k = O.KernelDesc(O.K_INV, 3, 1e-14, 0.0)
     a = O.aca(k, pts, np.arange(50), np.arange(50, 100), 1e-8)
     assert a["rank"] == 0 and not a["dense"] and a["retries"] == 10
+    assert a["entries"] == 11 * 50  # row 0 and its 10 retries, no column
 
 
 # ---------------------------------------------------------------- operator
