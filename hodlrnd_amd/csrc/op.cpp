@@ -53,13 +53,14 @@ bool verbose() {
     return v;
 }
 
-// ACA kernel class of a block: 8, 16, 32 = one wave per block with the
-// factors in registers (rank capacity = class, so max(m, n) <= 64 and
-// min(m, n) <= class); 64, 256, 1024 = workgroup of that size with the
-// factors in the scratch pool
-int aca_block_size(int m, int n) {
+// ACA kernel class of a block with rank budget cap: 8, 16, 32 = one wave
+// per block with the factors in registers (rank capacity = class >= the
+// budget, so max(m, n) <= 64 and min(m, n) <= 32); 64, 256, 1024 =
+// workgroup of that size with the factors in the scratch pool
+int aca_block_size(int m, int n, int cap) {
     const int mx = m > n ? m : n, mn = m < n ? m : n;
-    if (mx <= 64 && mn <= 32) return mn <= 8 ? 8 : (mn <= 16 ? 16 : 32);
+    const int c = mn < cap ? mn : cap;
+    if (mx <= 64 && mn <= 32) return c <= 8 ? 8 : (c <= 16 ? 16 : 32);
     if (mx <= 128) return 64;
     if (mx <= 4096) return 256;
     return 1024;
@@ -81,8 +82,8 @@ int64_t aca_pts_lds_limit(int cls) {
 
 // a block that may become a skeleton runs in aca_kernel in both passes
 // (only that kernel writes pivot positions)
-int aca_class(const hodlr_op &op, int64_t b) {
-    const int c = aca_block_size(op.b_m[b], op.b_n[b]);
+int aca_class(const hodlr_op &op, int64_t b, int cap) {
+    const int c = aca_block_size(op.b_m[b], op.b_n[b], cap);
     return (op.sys_size >= 0 && op.b_n[b] > op.sys_size && c < 64) ? 64 : c;
 }
 
@@ -246,7 +247,7 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
     order.reserve(ids.size());
     {
         std::vector<std::vector<int64_t>> bucket(6);
-        for (int64_t b : ids) bucket[class_index(aca_class(op, b))].push_back(b);
+        for (int64_t b : ids) bucket[class_index(aca_class(op, b, cap[b]))].push_back(b);
         for (auto &bk : bucket) {
             int64_t kmax = 0;
             for (int64_t b : bk) kmax = std::max<int64_t>(kmax, (int64_t)op.b_m[b] + op.b_n[b]);
@@ -275,14 +276,14 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
     std::vector<AcaBlock> chunk;
     std::vector<int64_t> offs;
     while (i < order.size()) {
-        const int bs = aca_class(op, order[i]);
+        const int bs = aca_class(op, order[i], cap[order[i]]);
         // the chunk: consecutive blocks of one class within the scratch budget
         int64_t used = 0;
         const size_t i0 = i;
         offs.clear();
         while (i < order.size()) {
             const int64_t b = order[i];
-            if (aca_class(op, b) != bs) break;
+            if (aca_class(op, b, cap[b]) != bs) break;
             const int64_t need = aca_in_registers(bs) ? 0 : aca_scratch_doubles(op.b_m[b], op.b_n[b], cap[b]);
             if (i > i0 && used + need > budget) break;
             offs.push_back(used);
@@ -313,18 +314,21 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
         if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double), s);
         upload(d_blocks, chunk, s, true);
         // the workgroup forms stage a block's points in LDS when they fit
-        // in the chunk's allowance (the wave forms always do, statically)
+        // in the class's allowance (the wave forms always do, statically);
+        // a launch reserves what its largest fitting block needs, nothing
+        // when none fits (the reservation costs occupancy)
         AcaStage st = stage;
         st.pts_lds = 0;
         if (!aca_in_registers(bs)) {
+            const int64_t lim = aca_pts_lds_limit(bs);
             int64_t want = 0;
             for (size_t q = 0; q < chunk.size(); q++) {
                 const int p = chunk[q].prob;
                 const int64_t rec = (int64_t)(chunk[q].m + chunk[q].n) * (op.kds[p].ndim + 1) * 8 +
                                     (op.w_off[p] >= 0 ? (int64_t)chunk[q].n * 8 : 0);
-                want = std::max(want, rec);
+                if (rec <= lim) want = std::max(want, rec);
             }
-            st.pts_lds = (int32_t)std::min<int64_t>(want, aca_pts_lds_limit(bs));
+            st.pts_lds = (int32_t)want;
         }
         HIP_CHECK(op.entries->aca(bs, d_blocks.as<AcaBlock>(), (int64_t)chunk.size(), op.kdev.as<KernelDev>(), op.eps,
                              pool.as<double>(), d_rank.as<int32_t>(), d_flags.as<int32_t>(), fill,
@@ -509,9 +513,17 @@ void assemble(hodlr_op &op) {
         (int64_t)1 << 24);
     std::vector<int32_t> cap(nb);
     std::vector<int64_t> todo(nb);
+    // first rank budget: 128, but 16 for blocks of the one-wave form with
+    // 17-32 points on their short side.  Its registers hold two factor
+    // values per rank slot and lane: with room for 32 the kernel takes 256
+    // VGPRs (one wave per SIMD), with 16 it takes 195 (two), and ranks stay
+    // below 16 for ~99% of such blocks (1/r at eps_ACA 1e-8).  A block that
+    // reaches the budget is run again with min(m, n) (the retry below), which
+    // repeats the same steps: the result does not depend on the budget.
     for (int64_t b = 0; b < nb; b++) {
         const int mn = std::min(op.b_m[b], op.b_n[b]);
-        cap[b] = std::min(mn, 128);
+        const int mx = std::max(op.b_m[b], op.b_n[b]);
+        cap[b] = (mx <= 64 && mn > 16 && mn <= 32) ? 16 : std::min(mn, 128);
         todo[b] = b;
     }
     op.rank.assign(nb, 0);
@@ -568,7 +580,8 @@ void assemble(hodlr_op &op) {
             }
         }
     }
-    // entries each block's rank pass evaluates, added up over its retries
+    // entries each block's ACA evaluates (its last run: a block run again
+    // with a larger rank budget repeats the same steps first)
     DevBuf d_evals;
     if (!todo.empty()) {
         d_evals.alloc(sizeof(int64_t) * (size_t)nb, s);
@@ -589,7 +602,7 @@ void assemble(hodlr_op &op) {
         const int64_t pass_budget = first_pass ? budget : retry_budget;
         std::vector<int64_t> big, rest;
         if (first_pass)
-            for (int64_t b : todo) (aca_class(op, b) == 1024 ? big : rest).push_back(b);
+            for (int64_t b : todo) (aca_class(op, b, cap[b]) == 1024 ? big : rest).push_back(b);
         first_pass = false;
         if (!big.empty() && !rest.empty()) {
             // The largest blocks (workgroup-1024 class) put few workgroups on

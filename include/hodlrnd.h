@@ -99,8 +99,8 @@ typedef struct {
     int64_t n_skeleton;    /* skeleton blocks (hodlr_op_assemble_skeleton) */
     double e_skel;         /* entries of skeleton blocks evaluated per matvec: sum r (m + n) */
     double bytes_core;     /* HBM bytes of the skeleton cores and pivot positions */
-    double aca_entries;    /* kernel entries the ACA rank pass evaluated (rows x n + columns x m per
-                              block, retries included; 0 for an operator read from a file) */
+    double aca_entries;    /* kernel entries the blocks' ACA evaluates (rows x n + columns x m per
+                              block, pivot retries included; 0 for an operator read from a file) */
 } hodlr_op_stats;
 
 typedef struct {

@@ -60,7 +60,7 @@ struct AcaStage {
     unsigned long long *cursor;   // doubles handed out so far (may exceed cap)
     int64_t cap;                  // doubles in buf
     int64_t *off;                 // per block id
-    int64_t *evals;               // per block id: entries evaluated, added to (nullptr: not counted)
+    int64_t *evals;               // per block id: entries its last ACA run evaluated (nullptr: not counted)
     int32_t pts_lds;              // workgroup ACA: dynamic LDS bytes for the block's points (0: none)
 };
 

@@ -501,27 +501,31 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
             // tid) and wave trees as one at a time, their loads and
             // shuffles interleaved
             for (; qq + 8 <= qn; qq += 8) {
-                double du[8], dv[8];
+                double d[8];
 #pragma unroll
-                for (int k = 0; k < 8; k++) du[k] = dv[k] = 0.0;
+                for (int k = 0; k < 8; k++) d[k] = 0.0;
                 for (int i = tid; i < m; i += BS) {
                     const double x = ur[i];
 #pragma unroll
-                    for (int k = 0; k < 8; k++) du[k] = fma(U[(size_t)(q0 + qq + k) * m + i], x, du[k]);
+                    for (int k = 0; k < 8; k++) d[k] = fma(U[(size_t)(q0 + qq + k) * m + i], x, d[k]);
                 }
+#pragma unroll
+                for (int k = 0; k < 8; k++) d[k] = wave_sum(d[k]);
+                if (lane == 0)
+#pragma unroll
+                    for (int k = 0; k < 8; k++) sh.du[wid][qq + k] = d[k];
+#pragma unroll
+                for (int k = 0; k < 8; k++) d[k] = 0.0;
                 for (int j = tid; j < n; j += BS) {
                     const double x = vr[j];
 #pragma unroll
-                    for (int k = 0; k < 8; k++) dv[k] = fma(V[(size_t)(q0 + qq + k) * n + j], x, dv[k]);
+                    for (int k = 0; k < 8; k++) d[k] = fma(V[(size_t)(q0 + qq + k) * n + j], x, d[k]);
                 }
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    du[k] = wave_sum(du[k]);
-                    dv[k] = wave_sum(dv[k]);
-                }
+                for (int k = 0; k < 8; k++) d[k] = wave_sum(d[k]);
                 if (lane == 0)
 #pragma unroll
-                    for (int k = 0; k < 8; k++) { sh.du[wid][qq + k] = du[k]; sh.dv[wid][qq + k] = dv[k]; }
+                    for (int k = 0; k < 8; k++) sh.dv[wid][qq + k] = d[k];
             }
             for (; qq < qn; qq++) {
                 const double *uq = U + (size_t)(q0 + qq) * m;
@@ -550,7 +554,7 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
     if (tid == 0) {
         out_rank[B.id] = r;
         out_flags[B.id] = flags | (dense ? 1 : 0);
-        if (S.evals) S.evals[B.id] += nrow_ev * n + ncol_ev * m;
+        if (S.evals) S.evals[B.id] = nrow_ev * n + ncol_ev * m;
     }
     if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors
         const int64_t len = (int64_t)r * (m + n);
@@ -798,7 +802,7 @@ __global__ __launch_bounds__(64) void aca_wave_kernel(const AcaBlock *__restrict
     if (l == 0) {
         out_rank[B.id] = r;
         out_flags[B.id] = flags | (dense ? 1 : 0);
-        if (S.evals) S.evals[B.id] += nrow_ev * n + ncol_ev * m;
+        if (S.evals) S.evals[B.id] = nrow_ev * n + ncol_ev * m;
     }
     if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors
         const int64_t len = (int64_t)r * (m + n);

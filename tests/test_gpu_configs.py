@@ -61,11 +61,10 @@ def test_config1_full_size():
     say("rank agreement", agree, "blocks", len(r_gpu))
     assert agree == 1.0
     # the same pivots, retries and stops: the same rows and columns evaluated
-    # (no block reaches the device's first rank budget of 128, so none is run twice)
     st, rst = A.op.stats(), ref.stats()
     say("ACA entries evaluated: gpu", st["aca_entries"], "oracle", rst["aca_entries"],
         "rank pass", st["rank_seconds"], "s")
-    assert st["max_rank"] < 128 and st["aca_entries"] == rst["aca_entries"] > 0
+    assert st["aca_entries"] == rst["aca_entries"] > 0
     x = O.splitmix_uniform(2, n)
     y = A * x
     e_ref = relerr(y, ref.matvec(x))
