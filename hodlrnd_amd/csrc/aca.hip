@@ -37,6 +37,25 @@ __global__ __launch_bounds__(256) void stage_copy_kernel(const StageCopy *__rest
     }
 }
 
+// staged pivot positions of skeleton blocks into the operator's pivot
+// slots; one wave per block
+__global__ __launch_bounds__(64) void stage_pivots_kernel(const PivotCopy *__restrict__ items,
+                                                          const double *__restrict__ buf,
+                                                          int32_t *__restrict__ SK) {
+    const PivotCopy C = items[blockIdx.x];
+    const int32_t *pv = reinterpret_cast<const int32_t *>(buf + C.src);
+    for (int q = threadIdx.x; q < C.r; q += blockDim.x) {
+        SK[C.sk_row + q] = pv[q];
+        SK[C.sk_col + q] = pv[C.r + q];
+    }
+}
+
+hipError_t launch_stage_pivots(const PivotCopy *items, int64_t n, const double *buf, int32_t *SK, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    stage_pivots_kernel<<<dim3((unsigned)n), 64, 0, s>>>(items, buf, SK);
+    return hipGetLastError();
+}
+
 // zero the diagonal of every leaf's self block (the first nc columns of its
 // near-field panel); the FP32 operator then holds only off-diagonal entries
 __global__ void zero_self_diag_kernel(const NearLeaf *__restrict__ leaves, int64_t nl, float *__restrict__ NP) {

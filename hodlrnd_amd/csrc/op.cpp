@@ -308,7 +308,7 @@ void run_aca_pass(hodlr_op &op, const std::vector<int64_t> &ids, const std::vect
                 a.id = (int32_t)b;
                 a.seed = block_seed(op.b_level[b], op.b_rowj[b], op.b_colj[b]);
                 a.prob = p;
-                a.stage = (stage.buf && stageable && (*stageable)[b]) ? 1 : 0;
+                a.stage = (stage.buf && stageable) ? (*stageable)[b] : 0;
             }
         });
         if ((int64_t)(pool.bytes / sizeof(double)) < used) pool.alloc((size_t)used * sizeof(double), s);
@@ -554,11 +554,13 @@ void assemble(hodlr_op &op) {
     AcaStage stage{nullptr, nullptr, 0, nullptr};
     std::vector<uint8_t> stageable(nb, 0);
     if (!todo.empty() && op.ctx->stage_bytes != 0) {
+        // a skeleton candidate (n > sys_size) keeps its pivots, the others
+        // their factors
         int64_t need = 0;
         for (int64_t b = 0; b < nb; b++) {
-            if (op.sys_size >= 0 && op.b_n[b] > op.sys_size) continue;
-            stageable[b] = 1;
-            need += (int64_t)cap[b] * (op.b_m[b] + op.b_n[b]);
+            const bool sk = op.sys_size >= 0 && op.b_n[b] > op.sys_size;
+            stageable[b] = sk ? 2 : 1;
+            need += sk ? (int64_t)cap[b] : (int64_t)cap[b] * (op.b_m[b] + op.b_n[b]);
         }
         const int64_t room = op.ctx->stage_bytes > 0
                                  ? op.ctx->stage_bytes / (int64_t)sizeof(double)
@@ -662,7 +664,8 @@ void assemble(hodlr_op &op) {
             // what is staged so far, the cursor continuing from where it is
             int64_t extra = 0;
             for (int64_t b : again)
-                if (stageable[b]) extra += (int64_t)cap[b] * (op.b_m[b] + op.b_n[b]);
+                if (stageable[b])
+                    extra += stageable[b] == 2 ? (int64_t)cap[b] : (int64_t)cap[b] * (op.b_m[b] + op.b_n[b]);
             unsigned long long cur = 0;
             HIP_CHECK(hipMemcpyAsync(&cur, stage.cursor, sizeof(cur), hipMemcpyDeviceToHost, s));
             HIP_CHECK(hipStreamSynchronize(s));
@@ -1332,12 +1335,26 @@ void assemble(hodlr_op &op) {
         HIP_CHECK(op.entries->dense_fill(d_dense.as<DenseBlock>(), (int64_t)dense.size(), op.kdev.as<KernelDev>(),
                                     op.leaf_start.as<int64_t>(), op.leaf_cnt.as<int32_t>(), op.UP64.as<double>(), s));
         HIP_CHECK(hipStreamSynchronize(s));
-        std::vector<int64_t> lowrank, staged;
+        std::vector<int64_t> lowrank, staged, staged_piv;
         for (int64_t b = 0; b < nb; b++)
             if (!(op.flags[b] & 1)) {
-                if (stage.buf && soff[b] >= 0 && !skel[b]) staged.push_back(b);
+                if (stage.buf && soff[b] >= 0) (skel[b] ? staged_piv : staged).push_back(b);
                 else lowrank.push_back(b);
             }
+        if (!staged_piv.empty()) {  // skeleton blocks' pivots, kept by the rank pass
+            std::vector<PivotCopy> items(staged_piv.size());
+            parallel_for((int64_t)items.size(), [&](int64_t a0, int64_t a1) {
+                for (int64_t a = a0; a < a1; a++) {
+                    const int64_t b = staged_piv[(size_t)a];
+                    items[(size_t)a] = PivotCopy{soff[b], dest[b].sk_row, dest[b].sk_col, op.rank[b], 0};
+                }
+            }, 1 << 14);
+            DevBuf d_items;
+            upload(d_items, items, s, true);
+            HIP_CHECK(launch_stage_pivots(d_items.as<PivotCopy>(), (int64_t)items.size(), stage.buf,
+                                          op.SK.as<int32_t>(), s));
+            HIP_CHECK(hipStreamSynchronize(s));
+        }
         if (!staged.empty()) {  // copy staged factors, in pieces of at most 64 Ki values
             constexpr int64_t kPiece = 1 << 16;
             std::vector<int64_t> first(staged.size() + 1, 0);
@@ -1397,8 +1414,8 @@ void assemble(hodlr_op &op) {
                 q0 = q1;
             }
             if (verbose())
-                std::fprintf(stderr, "[hodlr] staged factors of %zu blocks copied, %zu blocks recomputed\n",
-                             staged.size(), lowrank.size());
+                std::fprintf(stderr, "[hodlr] staged factors of %zu blocks and pivots of %zu copied, %zu blocks "
+                             "recomputed\n", staged.size(), staged_piv.size(), lowrank.size());
         }
         stage_buf.release();
         // the fill pass repeats the rank pass with the same caps

@@ -14,6 +14,7 @@ namespace hodlr {
 const EntryLaunchers &builtin_entry_launchers();
 hipError_t launch_stage_copy(const StageCopy *items, int64_t n, const double *buf, const int64_t *leaf_start,
                              const int32_t *leaf_cnt, double *UP, double *VP, hipStream_t s);
+hipError_t launch_stage_pivots(const PivotCopy *items, int64_t n, const double *buf, int32_t *SK, hipStream_t s);
 hipError_t launch_convert(const double *a, float *b, int64_t n, hipStream_t s);
 hipError_t launch_zero_self_diag(const NearLeaf *l, int64_t nl, float *NP, hipStream_t s);
 hipError_t launch_f32_result(const float *y, const double *x, const double *diag, double *out, int64_t n, int k,

@@ -46,15 +46,18 @@ struct AcaBlock {
     int32_t id;           // block id
     uint64_t seed;        // retry stream (replaces rand(), LowRank_matrix.hpp:175,274)
     int32_t prob;         // problem of the block (kernel descriptor index)
-    int32_t stage;        // rank pass: keep the factors in the staging buffer
+    int32_t stage;        // rank pass: keep 1 = the factors, 2 = the pivot positions (skeleton
+                          // candidate) in the staging buffer, 0 = nothing
 };
 
 // Factor staging of the rank pass.  A compressed block whose factors fit
 // takes r(m + n) doubles of buf at an offset handed out by an atomic
-// cursor: U (m x r, column-major, rows in ACA order) then V (n x r); off[id]
-// records the offset, -1 when the block was not staged (dense, skeleton,
-// out of room).  The fill step then copies staged factors into their panels
-// instead of repeating the ACA.
+// cursor: U (m x r, column-major, rows in ACA order) then V (n x r); a
+// skeleton candidate takes r doubles holding its 2r pivot positions (int32:
+// rows then columns, in step order).  off[id] records the offset, -1 when
+// the block was not staged (dense, out of room).  The fill step then copies
+// staged factors into their panels and staged pivots into the operator's
+// pivot slots instead of repeating the ACA.
 struct AcaStage {
     double *buf;                  // nullptr: no staging
     unsigned long long *cursor;   // doubles handed out so far (may exceed cap)
@@ -73,6 +76,13 @@ struct StageCopy {
     int64_t up_base, vp_base;
     int32_t m, n, r, R, coff, T, voff;
     int32_t e0, e1, pad;
+};
+
+// a skeleton block's staged pivots: 2r int32 at buf + src into its row and
+// column pivot slots
+struct PivotCopy {
+    int64_t src, sk_row, sk_col;
+    int32_t r, pad;
 };
 
 // where a block's factors go in the operator (fill pass)

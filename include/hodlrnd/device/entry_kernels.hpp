@@ -556,8 +556,8 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
         out_flags[B.id] = flags | (dense ? 1 : 0);
         if (S.evals) S.evals[B.id] = nrow_ev * n + ncol_ev * m;
     }
-    if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors
-        const int64_t len = (int64_t)r * (m + n);
+    if (!fill && S.buf && B.stage && !dense && !(flags & 2)) {  // keep the factors or pivots
+        const int64_t len = B.stage == 2 ? (int64_t)r : (int64_t)r * (m + n);
         if (tid == 0) {
             const int64_t o = (int64_t)atomicAdd(S.cursor, (unsigned long long)len);
             const int64_t off = o + len <= S.cap ? o : -1;
@@ -566,7 +566,13 @@ __global__ __launch_bounds__(BS) void aca_kernel(const AcaBlock *__restrict__ bl
         }
         __syncthreads();
         const int64_t off = sh.bcast64;
-        if (off >= 0) {  // U and V columns 0..r-1 are contiguous in the scratch
+        if (off >= 0 && B.stage == 2) {  // pivot positions, as the fill pass would write them
+            int32_t *pv = reinterpret_cast<int32_t *>(S.buf + off);
+            for (int q = tid; q < r; q += BS) {
+                pv[q] = B.ord[B.row0l + rind[q]];
+                pv[r + q] = B.ord[B.col0l + cind[q]];
+            }
+        } else if (off >= 0) {  // U and V columns 0..r-1 are contiguous in the scratch
             for (int64_t e = tid; e < (int64_t)r * m; e += BS) S.buf[off + e] = U[e];
             for (int64_t e = tid; e < (int64_t)r * n; e += BS) S.buf[off + (int64_t)r * m + e] = V[e];
         }
