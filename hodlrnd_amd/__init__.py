@@ -39,6 +39,9 @@ MV_PROJ_MFMA, MV_PANEL_MFMA, MV_PANEL_FLAT, MV_PANEL_VEC, MV_PROJ_PACK, MV_PANEL
 # calling stream, beside them on a second stream, or beside them with capped
 # occupancy; AUTO = the tuner's pick, else a cost model (same bits for all)
 SKEL_AUTO, SKEL_SERIAL, SKEL_OVERLAP, SKEL_CAPPED = 0, 1, 2, 3
+# Arnoldi orthogonalisation of single-problem solves: two classical
+# Gram-Schmidt passes (default) or modified Gram-Schmidt as the reference
+ORTHO_CGS2, ORTHO_MGS = 0, 1
 
 
 class KernelDesc(C.Structure):
@@ -66,7 +69,7 @@ class GmresParams(C.Structure):
     _fields_ = [("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
                 ("mode", C.c_int32), ("x0_is_b", C.c_int32), ("inner_tol", C.c_double),
                 ("max_outer", C.c_int32), ("verify", C.c_int32), ("hist_cap", C.c_int32),
-                ("hist", C.c_void_p)]
+                ("hist", C.c_void_p), ("ortho", C.c_int32)]
 
 
 class GmresInfo(C.Structure):
@@ -498,13 +501,13 @@ class Operator:
         return Y[:, 0].copy() if single else Y
 
     def gmres(self, b, x0=None, restart=100, max_iter=None, tol=1e-10, mode=SOLVE_F64,
-              inner_tol=0.0, max_outer=20, verify=True, hist_cap=512):
+              inner_tol=0.0, max_outer=20, verify=True, hist_cap=512, ortho=ORTHO_CGS2):
         b = _f64(b)
         x = b.copy() if x0 is None else _f64(x0).copy()
         hist = np.zeros(hist_cap)
         p = GmresParams(restart, self.n if max_iter is None else max_iter, tol, mode,
                         1 if x0 is None else 0, inner_tol, max_outer, 1 if verify else 0,
-                        hist_cap, hist.ctypes.data)
+                        hist_cap, hist.ctypes.data, ortho)
         info = GmresInfo()
         rc = lib.hodlr_gmres(self.h, _p(b), _p(x), C.byref(p), C.byref(info))
         if rc not in (OK, E_NOCONV):
@@ -514,12 +517,12 @@ class Operator:
         return x, out
 
     def gmres_device(self, b_dev, x_dev, restart=100, max_iter=None, tol=1e-10, mode=SOLVE_F64,
-                     inner_tol=0.0, max_outer=20, verify=False, x0="b"):
+                     inner_tol=0.0, max_outer=20, verify=False, x0="b", ortho=ORTHO_CGS2):
         """Solve with b and x resident on the device (tree order, double).
         x0: "b" (the reference's choice), "zero", or "given" (x holds it)."""
         x0code = {"b": 1, "zero": 2, "given": 0}[x0]
         p = GmresParams(restart, self.n if max_iter is None else max_iter, tol, mode,
-                        x0code, inner_tol, max_outer, 1 if verify else 0, 0, None)
+                        x0code, inner_tol, max_outer, 1 if verify else 0, 0, None, ortho)
         info = GmresInfo()
         rc = lib.hodlr_gmres_device(self.h, b_dev, x_dev, C.byref(p), C.byref(info))
         if rc not in (OK, E_NOCONV):

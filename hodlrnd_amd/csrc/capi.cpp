@@ -577,7 +577,7 @@ int hodlr_gmres(hodlr_op *op, const double *b, double *x, const hodlr_gmres_para
 
 int hodlr_gmres_batch_device(hodlr_op *op, int nrhs, const double *b, double *x, const hodlr_gmres_params *params,
                              hodlr_gmres_info *infos) {
-    if (!op || !b || !x || !params || !infos) return HODLR_E_ARG;
+    if (!op || !b || !x || !params || !infos || params->ortho != HODLR_ORTHO_CGS2) return HODLR_E_ARG;
     int rc = guard(op->ctx, [&] {
         if (nrhs != 1 && nrhs != 2 && nrhs != 4 && nrhs != 8 && nrhs != 16)
             throw ArgError{HODLR_E_ARG, "nrhs must be 1, 2, 4, 8 or 16"};
@@ -592,7 +592,7 @@ int hodlr_gmres_batch_device(hodlr_op *op, int nrhs, const double *b, double *x,
 
 int hodlr_gmres_batch(hodlr_op *op, int nrhs, const double *b, double *x, const hodlr_gmres_params *params,
                       hodlr_gmres_info *infos) {
-    if (!op || !b || !x || !params || !infos || nrhs < 1) return HODLR_E_ARG;
+    if (!op || !b || !x || !params || !infos || nrhs < 1 || params->ortho != HODLR_ORTHO_CGS2) return HODLR_E_ARG;
     int rc = guard(op->ctx, [&] {
         const int K = padded_rhs(nrhs);
         if (K < 0 || K > 16) throw ArgError{HODLR_E_ARG, "at most 16 right-hand sides per problem"};

@@ -50,7 +50,7 @@ double Workspace::norm2(const double *v, hipStream_t s) {
 }
 
 int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int max_iter, double tol,
-                 bool x0_zero, hipStream_t s, GmresResult &res, std::vector<double> *hist) {
+                 bool x0_zero, hipStream_t s, GmresResult &res, std::vector<double> *hist, bool mgs) {
     const int64_t n = ws.n;
     const int m = ws.restart;
     const int ld = m + 1;
@@ -91,11 +91,23 @@ int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int 
         double *vi = V + (size_t)i * n;
         A(vi, w);
         res.matvecs++;
-        // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
-        HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd, 0, s));
-        HIP_CHECK(launch_gemv(V, n, i + 1, hd, -1.0, w, n, s));
-        HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd + 128, 0, s));
-        HIP_CHECK(launch_gemv(V, n, i + 1, hd + 128, -1.0, w, n, s));
+        if (mgs) {
+            // MGS (GMRES.hpp:236-240): h_k = <w, v_k>, w -= h_k v_k for k =
+            // 0..i in turn, each coefficient read by the update on the
+            // device; h2 = 0
+            for (int k = 0; k <= i; k++) {
+                const double *vk = V + (size_t)k * n;
+                HIP_CHECK(launch_mdot(vk, n, 1, w, n, ws.partial.as<double>(), hd + k, 0, s));
+                HIP_CHECK(launch_gemv(vk, n, 1, hd + k, -1.0, w, n, s));
+            }
+            HIP_CHECK(hipMemsetAsync(hd + 128, 0, sizeof(double) * (size_t)(i + 1), s));
+        } else {
+            // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
+            HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd, 0, s));
+            HIP_CHECK(launch_gemv(V, n, i + 1, hd, -1.0, w, n, s));
+            HIP_CHECK(launch_mdot(V, n, i + 1, w, n, ws.partial.as<double>(), hd + 128, 0, s));
+            HIP_CHECK(launch_gemv(V, n, i + 1, hd + 128, -1.0, w, n, s));
+        }
         HIP_CHECK(launch_mdot(w, n, 1, w, n, ws.partial.as<double>(), hd + 256, 0, s));
         HIP_CHECK(launch_gmres_step(H, cs, sn, sv, hd, i, ld, st, s));
         HIP_CHECK(launch_scale_by(w, &st->scale, V + (size_t)(i + 1) * n, n, s));
@@ -182,10 +194,12 @@ void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p
     const bool x0_zero = p.x0_is_b == 2;
     if (x0_zero) HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * n, s));
 
+    if (p.ortho != HODLR_ORTHO_CGS2 && p.ortho != HODLR_ORTHO_MGS) throw ArgError{HODLR_E_ARG, "bad ortho"};
+    const bool mgs = p.ortho == HODLR_ORTHO_MGS;
     Apply A64 = [&](const double *in, double *out) { op.matvec_dev(HODLR_PREC_F64, 1, in, out, s); };
     GmresResult gr;
     if (p.mode == HODLR_SOLVE_F64) {
-        gmres_device(A64, ws, b, x, max_iter, p.tol, x0_zero, s, gr, &hist);
+        gmres_device(A64, ws, b, x, max_iter, p.tol, x0_zero, s, gr, &hist, mgs);
         info.iterations = gr.iterations;
         info.matvecs_f64 = gr.matvecs;
         info.converged = gr.converged;
@@ -236,7 +250,7 @@ void solve(hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p
             // never tighter than the caller's floor inner_tol)
             const double target = std::min(0.5, std::max(inner_floor, 0.3 * p.tol / std::max(res, 1e-300)));
             gmres_device(A32, ws, rr.as<double>(), d.as<double>(), max_iter - total, target, true, s, g,
-                         nullptr);
+                         nullptr, mgs);
             total += g.iterations;
             info.matvecs_f32 += g.matvecs;
             HIP_CHECK(launch_add(x, d.as<double>(), n, s));

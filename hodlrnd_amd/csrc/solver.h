@@ -32,8 +32,9 @@ struct GmresResult {
     double resid = 0.0;
 };
 
+// mgs: modified Gram-Schmidt (GMRES.hpp:236-240) instead of CGS2
 int gmres_device(const Apply &A, Workspace &ws, const double *b, double *x, int max_iter, double tol,
-                 bool x0_zero, hipStream_t s, GmresResult &res, std::vector<double> *hist);
+                 bool x0_zero, hipStream_t s, GmresResult &res, std::vector<double> *hist, bool mgs = false);
 
 void solve(struct ::hodlr_op &op, const double *b, double *x, const hodlr_gmres_params &p,
            hodlr_gmres_info &info);

@@ -118,7 +118,15 @@ typedef struct {
     int32_t verify;        /* F64 mode: recompute ||b - A x|| / ||b|| at exit */
     int32_t hist_cap;      /* length of hist (may be 0) */
     double *hist;          /* residual history, one entry per iteration */
+    int32_t ortho;         /* Arnoldi orthogonalisation of single-problem solves: HODLR_ORTHO_CGS2
+                              (default: two classical Gram-Schmidt passes, two multi-dot kernels per
+                              step) or HODLR_ORTHO_MGS (modified Gram-Schmidt, one basis vector at a
+                              time in the reference's order, GMRES.hpp:236-240); batched solves
+                              accept CGS2 only */
 } hodlr_gmres_params;
+
+#define HODLR_ORTHO_CGS2 0
+#define HODLR_ORTHO_MGS 1
 
 typedef struct {
     int32_t iterations;    /* return value of iterSolver::GMRES (GMRES.hpp:212,267,294,302) */

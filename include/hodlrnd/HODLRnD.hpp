@@ -790,6 +790,7 @@ public:
         p.x0_is_b = 0;
         p.hist_cap = (int)hist.size();
         p.hist = hist.data();
+        p.ortho = HODLR_ORTHO_MGS;  // the reference's Arnoldi order (GMRES.hpp:236-240)
         hodlr_gmres_info info{};
         int rc = hodlr_gmres(A->op_, b.data(), x.data(), &p, &info);
         A->check(rc, "gmres");

@@ -124,6 +124,36 @@ def test_gmres_f64_vs_oracle():
     assert relerr(x, xr) < 1e-8
 
 
+MGS_CASES = [
+    # nd, n, nmax, eps, family, tol
+    (3, 4000, 64, 1e-10, O.K_INV, 1e-10),
+    (2, 3000, 32, 1e-10, O.K_INV, 1e-10),
+    (3, 3000, 64, 1e-8, O.K_EXP, 1e-8),
+    (2, 2500, 32, 1e-10, O.K_LOG, 1e-10),
+]
+
+
+@pytest.mark.parametrize("nd,n,nmax,eps,family,tol", MGS_CASES)
+def test_gmres_mgs_follows_the_reference(nd, n, nmax, eps, family, tol):
+    """Modified Gram-Schmidt in the reference's order (GMRES.hpp:236-240):
+    the same iteration count as the oracle, not just within one."""
+    say("case gmres_mgs", nd, n, family)
+    pts, fam, sc, dg, w = problem(nd, n, 4300 + n, family=family)
+    A, ref, ok = build_pair(pts, fam, sc, dg, w, nmax, 0, eps)
+    b = O.splitmix_uniform(7, n)
+    x, info = A.op.gmres(b, tol=tol, ortho=H.ORTHO_MGS)
+    xc, ic = A.op.gmres(b, tol=tol)
+    xr, ir = ref.gmres(b, tol=tol)
+    say("mgs", info["iterations"], info["resid"], "cgs2", ic["iterations"], "oracle", ir["iterations"], ir["resid"])
+    assert info["converged"] and info["iterations"] == ir["iterations"]
+    assert abs(info["resid"] - ir["resid"]) < 1e-3 * ir["resid"]
+    assert relerr(x, xr) < 100 * tol and relerr(xc, xr) < 100 * tol
+    xm, im = A.op.gmres(b, tol=tol, mode=H.SOLVE_MIXED, ortho=H.ORTHO_MGS, x0=np.zeros(n))
+    assert im["converged"] and im["true_resid"] < tol
+    with pytest.raises(H.HodlrError):
+        A.op.gmres(b, tol=tol, ortho=5)
+
+
 def test_gmres_mixed_reaches_fp64_residual():
     say("case gmres_mixed")
     n = 4000
