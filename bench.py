@@ -49,11 +49,13 @@ HBM_PEAK_GBS = 8000.0
 # FP64 vector issue rate of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
 # (78.6 TFLOP/s counts each of these FMAs as 2 flops)
 VALU_F64_LANE_OPS = 256 * 4 * 16 * 2.4e9
-# issue slots per on-the-fly entry of the 3-D 1/r skeleton kernels (from
-# their gfx950 code): 3 v_add_f64 + 3 v_fma_f64 (distance), v_rsq_f64 (a
-# quarter-rate transcendental: 4 slots), 5 for the refinement (cubic step,
-# FP64 operator) or 4 (Newton step, FP32 operator), 1 FMA per RHS
-SKEL_SLOTS = {"f64": 16, "f32": 15}
+# issue slots per on-the-fly entry of the 3-D 1/r skeleton kernels: FP64
+# operator 3 v_add_f64 + 3 v_fma_f64 (distance from differences), FP32
+# operator 1 v_add_f64 + 3 v_fma_f64 (distance from centred coordinates and
+# norms); v_rsq_f64 (a quarter-rate transcendental: 4 slots); 5 for the
+# refinement (cubic step, FP64 operator) or 4 (Newton step, FP32 operator);
+# 1 FMA per RHS
+SKEL_SLOTS = {"f64": 16, "f32": 13}
 
 
 def log(*a):

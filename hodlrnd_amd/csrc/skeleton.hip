@@ -32,8 +32,10 @@
 // cancels large terms, so entries rounded to FP32 would lose the accuracy
 // that FP32 storage of ACA factors keeps.  1/r is the hardware reciprocal
 // square root refined by Newton steps (two for the FP64 operator: about
-// 1 ulp; one for the FP32 operator: ~1e-14); this path does not have to
-// round like the CPU restatement.  The phases are compute bound: about 15-20
+// 1 ulp; one for the FP32 operator: ~1e-14), and the FP32 operator forms
+// squared distances from centred coordinates (see tile_sum); this path does
+// not have to round like the CPU restatement.  The phases are compute
+// bound: about 13-20
 // FP64 vector instructions per entry, plus one FMA per entry and RHS.
 #include "../../include/hodlrnd/device/aca_layout.h"
 #include "../../include/hodlrnd/device/device_common.h"
@@ -71,33 +73,59 @@ constexpr bool kCubic = sizeof(T) == 8;
 //     acc_t += f(|a_t - b_s|^2) v_s.
 // The sources are staged 64 at a time into LDS (lane l loads source l:
 // coalesced), then every lane reads each source by a broadcast LDS read that
-// feeds both of its targets.  Source record: ND coordinates then NR values,
-// S = ND + NR doubles.  One wave per workgroup, so the barriers around a
-// tile only order this wave's LDS traffic.
+// feeds both of its targets.  One wave per workgroup, so the barriers around
+// a tile only order this wave's LDS traffic.
+//
+// FP64 operator: |a - b|^2 from the differences (ND subtractions and ND
+// FMAs per target and source); source record: ND coordinates then NR
+// values, S = ND + NR doubles.  FP32 operator: coordinates relative to a
+// point c of the item's compact side (the column node of a projection, the
+// row node of an expansion), |a'|^2 + |b'|^2 - 2 a'.b' (one addition and
+// ND FMAs); source record: -2 b', |b'|^2, then the values, S = ND + 1 + NR.
+// The skeleton blocks are admissible, so |a - b| is at least about the
+// compact node's size and (|a'|^2 + |b'|^2) / |a - b|^2 stays below ~10:
+// the squared distance keeps ~1e-15 relative accuracy, far inside the
+// FP32 operator's 1e-14 reciprocal square root.
 template <int NR, int ND, int FAM, bool CUBIC, int TPL>
 __device__ __forceinline__ void tile_sum(const double *__restrict__ lds, int S, int cnt, const double *a0,
-                                         const double *a1, int nd, double *acc0, double *acc1) {
+                                         const double *a1, double ta0, double ta1, int nd, double *acc0,
+                                         double *acc1) {
     constexpr int MD = ND > 0 ? ND : kMaxDimDev;
+    constexpr bool kNorm = !CUBIC;
     // four sources per iteration: their LDS reads issue together (84-90
     // VGPRs, 5-6 waves per SIMD; two sources kept 8 waves but measured 2.5%
     // slower, one 1% slower still)
 #pragma unroll 4
     for (int s = 0; s < cnt; s++) {
         const double *src = lds + s * S;
-        double d0 = 0.0, d1 = 0.0;
+        double d0, d1;
+        if constexpr (kNorm) {
+            const double tb = src[ND > 0 ? ND : nd];
+            d0 = ta0 + tb;
+            d1 = ta1 + tb;
 #pragma unroll
-        for (int d = 0; d < MD; d++)
-            if (d < nd) {
-                const double b = src[d];
-                const double t0 = a0[d] - b;
-                d0 = fma(t0, t0, d0);
-                if constexpr (TPL == 2) {
-                    const double t1 = a1[d] - b;
-                    d1 = fma(t1, t1, d1);
+            for (int d = 0; d < MD; d++)
+                if (d < nd) {
+                    const double q = src[d];
+                    d0 = fma(a0[d], q, d0);
+                    if constexpr (TPL == 2) d1 = fma(a1[d], q, d1);
                 }
-            }
+        } else {
+            d0 = d1 = 0.0;
+#pragma unroll
+            for (int d = 0; d < MD; d++)
+                if (d < nd) {
+                    const double b = src[d];
+                    const double t0 = a0[d] - b;
+                    d0 = fma(t0, t0, d0);
+                    if constexpr (TPL == 2) {
+                        const double t1 = a1[d] - b;
+                        d1 = fma(t1, t1, d1);
+                    }
+                }
+        }
         const double f0 = kfun<FAM, CUBIC>(d0);
-        const double *v = src + (ND > 0 ? ND : nd);
+        const double *v = src + (ND > 0 ? ND : nd) + (kNorm ? 1 : 0);
         if constexpr (TPL == 2) {
             const double f1 = kfun<FAM, CUBIC>(d1);
 #pragma unroll
@@ -129,11 +157,12 @@ __device__ __forceinline__ T gload(const T *p) {
 // summed, so its load latency overlaps the arithmetic.
 template <int NR, int ND, int FAM, bool CUBIC, class V, class Fetch>
 __device__ __forceinline__ void source_loop(int nsrc, int nd, int lane, double *lds, const double *a0,
-                                            const double *a1, double *acc0, double *acc1, bool two,
-                                            Fetch fetch) {
+                                            const double *a1, double ta0, double ta1, const double *ctr,
+                                            double *acc0, double *acc1, bool two, Fetch fetch) {
     constexpr int MD = ND > 0 ? ND : kMaxDimDev;
     constexpr bool kPrefetch = MD + NR <= 12;
-    const int S = nd + NR;
+    constexpr bool kNorm = !CUBIC;
+    const int S = nd + NR + (kNorm ? 1 : 0);
     const int ntiles = (nsrc + kWave - 1) / kWave;
     double c[MD], w = 1.0;
     V v[NR];
@@ -144,16 +173,30 @@ __device__ __forceinline__ void source_loop(int nsrc, int nd, int lane, double *
         if (lane < cnt) {
             if (!kPrefetch) fetch(t, c, v, w);
             double *dst = lds + lane * S;
+            if constexpr (kNorm) {
+                double tb = 0.0;
 #pragma unroll
-            for (int d = 0; d < MD; d++)
-                if (d < nd) dst[d] = c[d];
+                for (int d = 0; d < MD; d++)
+                    if (d < nd) {
+                        const double bd = c[d] - ctr[d];
+                        dst[d] = -2.0 * bd;
+                        tb = fma(bd, bd, tb);
+                    }
+                dst[nd] = tb;
 #pragma unroll
-            for (int k = 0; k < NR; k++) dst[nd + k] = w * (double)v[k];
+                for (int k = 0; k < NR; k++) dst[nd + 1 + k] = w * (double)v[k];
+            } else {
+#pragma unroll
+                for (int d = 0; d < MD; d++)
+                    if (d < nd) dst[d] = c[d];
+#pragma unroll
+                for (int k = 0; k < NR; k++) dst[nd + k] = w * (double)v[k];
+            }
         }
         __syncthreads();
         if (kPrefetch && t + 1 < ntiles && lane < min(kWave, nsrc - (t + 1) * kWave)) fetch(t + 1, c, v, w);
-        if (two) tile_sum<NR, ND, FAM, CUBIC, 2>(lds, S, cnt, a0, a1, nd, acc0, acc1);
-        else tile_sum<NR, ND, FAM, CUBIC, 1>(lds, S, cnt, a0, a1, nd, acc0, acc1);
+        if (two) tile_sum<NR, ND, FAM, CUBIC, 2>(lds, S, cnt, a0, a1, ta0, ta1, nd, acc0, acc1);
+        else tile_sum<NR, ND, FAM, CUBIC, 1>(lds, S, cnt, a0, a1, ta0, ta1, nd, acc0, acc1);
     }
 }
 
@@ -169,6 +212,23 @@ __device__ __forceinline__ void load_point(const KernelDev &K, int64_t pos, int 
 // the in-flight prefetch
 __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0); }
 
+// FP32 operator: the targets relative to ctr and their squared norms
+template <int MD, bool CUBIC>
+__device__ __forceinline__ void center_targets(const double *ctr, int nd, double *a0, double *a1, double &ta0,
+                                               double &ta1) {
+    ta0 = ta1 = 0.0;
+    if constexpr (!CUBIC) {
+#pragma unroll
+        for (int d = 0; d < MD; d++)
+            if (d < nd) {
+                a0[d] -= ctr[d];
+                a1[d] -= ctr[d];
+                ta0 = fma(a0[d], a0[d], ta0);
+                ta1 = fma(a1[d], a1[d], ta1);
+            }
+    }
+}
+
 template <typename T, int NR, int ND, int FAM>
 __device__ __forceinline__ void proj_item(const SkelProj &P, const KernelDev &K, const int32_t *__restrict__ SK,
                                           const T *__restrict__ x, double *__restrict__ t1,
@@ -176,14 +236,18 @@ __device__ __forceinline__ void proj_item(const SkelProj &P, const KernelDev &K,
     constexpr int MD = ND > 0 ? ND : kMaxDimDev;
     const int nd = ND > 0 ? ND : K.ndim;
     const bool act0 = lane < P.ntgt, act1 = lane + kWave < P.ntgt;
-    double a0[MD], a1[MD];
+    double a0[MD], a1[MD], ctr[MD];
     load_point<MD>(K, SK[P.t0 + (act0 ? lane : 0)], nd, a0);
     load_point<MD>(K, SK[P.t0 + (act1 ? lane + kWave : 0)], nd, a1);
+    if constexpr (!kCubic<T>) load_point<MD>(K, P.src0, nd, ctr);  // a point of the column node
     wait_loads();
+    double ta0, ta1;
+    center_targets<MD, kCubic<T>>(ctr, nd, a0, a1, ta0, ta1);
     double acc0[NR], acc1[NR];
 #pragma unroll
     for (int c = 0; c < NR; c++) acc0[c] = acc1[c] = 0.0;
-    source_loop<NR, ND, FAM, kCubic<T>, T>(P.nsrc, nd, lane, lds, a0, a1, acc0, acc1, P.ntgt > kWave,
+    source_loop<NR, ND, FAM, kCubic<T>, T>(P.nsrc, nd, lane, lds, a0, a1, ta0, ta1, ctr, acc0, acc1,
+                                            P.ntgt > kWave,
                                             [&](int t, double *c, T *v, double &w) {
                                                 const int64_t j = P.src0 + (int64_t)t * kWave + lane;
                                                 const double *xs = K.X + (j - K.pos0) * nd;
@@ -208,7 +272,8 @@ __device__ __forceinline__ void proj_item(const SkelProj &P, const KernelDev &K,
 }
 
 // one item per single-wave workgroup, one (family, NDIM) per launch (ND = 0:
-// any NDIM up to 8); dynamic LDS: 64 (NDIM + NR) doubles
+// any NDIM up to 8); dynamic LDS: 64 (NDIM + NR) doubles, one more per source
+// for the FP32 operator
 template <typename T, int NR, int ND, int FAM>
 __global__ __launch_bounds__(64) void skel_proj_kernel(const SkelProj *__restrict__ items,
                                                        const KernelDev *__restrict__ Ks,
@@ -274,14 +339,18 @@ __device__ __forceinline__ void expand_item(const SkelExpand &E, const KernelDev
     const int nd = ND > 0 ? ND : K.ndim;
     const bool act0 = lane < E.h, act1 = lane + kWave < E.h;
     const int64_t ip0 = E.y0 + (act0 ? lane : 0), ip1 = E.y0 + (act1 ? lane + kWave : 0);
-    double a0[MD], a1[MD];
+    double a0[MD], a1[MD], ctr[MD];
     load_point<MD>(K, ip0, nd, a0);
     load_point<MD>(K, ip1, nd, a1);
+    if constexpr (!kCubic<T>) load_point<MD>(K, E.y0, nd, ctr);  // a point of the row node
     wait_loads();
+    double ta0, ta1;
+    center_targets<MD, kCubic<T>>(ctr, nd, a0, a1, ta0, ta1);
     double acc0[NR], acc1[NR];
 #pragma unroll
     for (int c = 0; c < NR; c++) acc0[c] = acc1[c] = 0.0;
-    source_loop<NR, ND, FAM, kCubic<T>, double>(E.nc, nd, lane, lds, a0, a1, acc0, acc1, E.h > kWave,
+    source_loop<NR, ND, FAM, kCubic<T>, double>(E.nc, nd, lane, lds, a0, a1, ta0, ta1, ctr, acc0, acc1,
+                                                 E.h > kWave,
                                                  [&](int t, double *c, double *v, double &w) {
                                                      const int64_t q = E.c0 + (int64_t)t * kWave + lane;  // slot
                                                      const int64_t cp = gload(SKC + q) - K.pos0;
@@ -363,7 +432,9 @@ template <typename T, int NR>
 void launch_key(bool proj, const SkeletonPlan &p, const SkelLaunch &L, const T *x, double *t1, double *part,
                 const double *t2, double *ys, hipStream_t s) {
     if (L.end <= L.begin) return;
-    size_t lds = sizeof(double) * kWave * (size_t)(L.key % 16 + NR);
+    // one record per source: NDIM coordinates, its squared norm (FP32
+    // operator), NR values
+    size_t lds = sizeof(double) * kWave * (size_t)(L.key % 16 + NR + (sizeof(T) == 4 ? 1 : 0));
     if (lds < (size_t)p.lds_min) lds = (size_t)p.lds_min;
     switch (L.key / 16) {
     case 0: launch_nd<T, NR, 0>(proj, p, L, lds, x, t1, part, t2, ys, s); break;
