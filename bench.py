@@ -818,8 +818,8 @@ def bench_gl4d(args, H):
 def bench_serve(args, H):
     """Serving stream: Poisson arrivals of solve requests, NDIM = 3, N
     log-uniform in [1000, 200000], points uniform in [-1,1]^3, A = I + (1/N)/r
-    (A_ii = 1), eps_ACA 1e-8, Nmax 64, d' 0 (Nmax and d' are not fixed by the
-    workload; these are the library defaults).  30% of requests bring new
+    (A_ii = 1), eps_ACA 1e-8, Nmax 128 (the --nmax default), d' 0 (Nmax and d'
+    are not fixed by the workload).  30% of requests bring new
     points (build + solve), 70% reuse an operator built earlier in the stream
     (uniformly), each with 1..32 right-hand sides.  Every column is solved to
     a FP64-certified relative residual of 1e-8.
