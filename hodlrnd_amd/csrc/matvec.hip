@@ -414,6 +414,128 @@ __global__ __launch_bounds__(256) void panel_vec_kernel(const PanelItem *__restr
     }
 }
 
+// lane s's value (s wave-uniform): a scalar read, no LDS round trip
+__device__ __forceinline__ int rdlane(int v, int s) { return __builtin_amdgcn_readlane(v, s); }
+__device__ __forceinline__ int64_t rdlane64(int64_t v, int s) {
+    const int lo = __builtin_amdgcn_readlane((int)(uint32_t)v, s);
+    const int hi = __builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), s);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Vector-ALU panel product with 16-byte loads and a flattened walk, for
+// leaves whose segment lists hold many short segments (small leaves, dense
+// fallback blocks).  Lane (o, g) owns rows R o .. R o + R - 1 (R = 16 /
+// sizeof(T)) in column group g of G = 64 / ceil(h / R); the leaf's segments
+// are cut into steps of G columns and walked as one list, the 4 waves taking
+// groups of kU steps in turn.  Each wave keeps up to 64 segment descriptors
+// in registers (lane i: segment i of the current chunk, with the prefix sum
+// of the steps), so finding a step's segment is a ballot and a few scalar
+// lane reads, not a dependent load.  Reads past a column's last row go to rows
+// >= h, which the reduction drops (the arenas carry tail padding).
+template <typename T, int K>
+__global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__restrict__ items,
+                                                            const Segment *__restrict__ segs,
+                                                            const T *__restrict__ NP, const T *__restrict__ UP,
+                                                            const T *__restrict__ x, const T *__restrict__ t,
+                                                            T *__restrict__ y) {
+    constexpr int R = 16 / (int)sizeof(T), NW = 4, kU = K <= 2 ? 8 : (K <= 4 ? 4 : 2);
+    typedef T vec_t __attribute__((ext_vector_type(R), aligned(sizeof(T))));
+    __shared__ T red[NW][kWave][R][K];
+    const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const PanelItem P = items[xcd_remap(blockIdx.x, gridDim.x)];
+    const int h = P.h;
+    const int hl = (h + R - 1) / R;  // lanes per column
+    const int G = kWave / hl;
+    const int o = lane % hl, g = lane / hl;
+    const bool active = g < G;
+    const int64_t ld = P.ld;
+    T acc[R][K];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[r][k] = T(0);
+    // the current chunk of descriptors: lane i holds segment c0 + i
+    int c0 = 0, cbase = 0, cend = 0;  // chunk's first step and one past its last
+    int64_t d_mat = 0, d_vec = 0;
+    int d_len = 0, d_arena = 0, d_qend = 0;
+    auto load_chunk = [&]() {
+        const int i = c0 + lane;
+        int steps = 0;
+        if (i < P.nseg) {
+            const Segment S = segs[P.seg0 + i];
+            d_mat = S.mat;
+            d_vec = S.vec;
+            d_len = S.len;
+            d_arena = S.arena;
+            steps = (S.len + G - 1) / G;
+        }
+        int incl = steps;  // inclusive prefix sum over the wave
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int v = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += v;
+        }
+        d_qend = i < P.nseg ? cbase + incl : INT_MAX;
+        cend = cbase + __shfl(incl, kWave - 1, kWave);
+    };
+    load_chunk();
+    int nq = cend;  // steps of all segments (more chunks only past 64 segments)
+    for (int cc = kWave; cc < P.nseg; cc += kWave) {
+        const int i = cc + lane;
+        int steps = i < P.nseg ? (segs[P.seg0 + i].len + G - 1) / G : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) steps += __shfl_xor(steps, off, kWave);
+        nq += steps;
+    }
+    for (int g0 = wid * kU; g0 < nq; g0 += NW * kU) {
+        vec_t a[kU];
+        T vv[kU][K];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int step = g0 + u;
+            while (step >= cend && c0 + kWave < P.nseg) {  // next chunk (uniform)
+                c0 += kWave;
+                cbase = cend;
+                load_chunk();
+            }
+            const bool in = step < nq;
+            // the segment holding this step: lanes whose segment ends at or
+            // before it come first
+            const int s = in ? __builtin_amdgcn_readfirstlane(__popcll(__ballot(d_qend <= step))) : 0;
+            const int len = rdlane(d_len, s);
+            const int arena = rdlane(d_arena, s);
+            const int64_t mat = rdlane64(d_mat, s), vec = rdlane64(d_vec, s);
+            const int qb = rdlane(d_qend, s) - (len + G - 1) / G;
+            const int col = (step - qb) * G + g;
+            const bool ok = in && active && col < len;
+            const T *M = (arena == 0 ? NP : UP) + mat + P.r0 + (int64_t)o * R;
+            const T *v = (arena == 1 ? t : x) + vec * K;
+            a[u] = ok ? *reinterpret_cast<const vec_t *>(M + (int64_t)col * ld) : vec_t(T(0));
+#pragma unroll
+            for (int k = 0; k < K; k++) vv[u][k] = ok ? v[(int64_t)col * K + k] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int k = 0; k < K; k++) acc[r][k] = fma(a[u][r], vv[u][k], acc[r][k]);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int k = 0; k < K; k++) red[wid][lane][r][k] = active ? acc[r][k] : T(0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < h * K; e += blockDim.x) {
+        const int row = e / K, k = e % K;
+        const int ol = row / R, r = row % R;
+        T sum = T(0);
+        for (int w = 0; w < NW; w++)
+            for (int gi = 0; gi < G; gi++) sum += red[w][gi * hl + ol][r][k];
+        y[(P.y0 + row) * K + k] = sum;
+    }
+}
+
 // Vector-ALU panel product, flattened walk.  Lane (o, g) of a wave owns row o of the tile
 // and column group g (G = 64 / h groups of h lanes).  The leaf's segments
 // are walked as one list of G-column steps (ceil(len / G) per segment); the
@@ -856,7 +978,10 @@ hipError_t launch_matvec_k(const ProjItem *pi, int64_t npi, const ProjItem *pw, 
                 }
             if constexpr (K <= 8)
                 if (!done && (variant & kMatvecPanelVec)) {
-                    panel_vec_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                    if (variant & kMatvecPanelFlat)
+                        panel_vecflat_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
+                    else
+                        panel_vec_kernel<T, K><<<dim3((unsigned)npa), 256, 0, s>>>(pa, segs, NP, UP, x, t, y);
                     done = true;
                 }
             if (!done) {

@@ -68,6 +68,15 @@ int aca_block_size(int m, int n, int cap) {
 
 bool aca_in_registers(int cls) { return cls < 64; }
 
+// ratio of per-segment to flattened load rounds at or above which the 1-2 RHS
+// panel product defaults to the flattened 16-byte form.  Measured: the
+// flattened walk wins at 8.4x and above (16^4 Gauss-Legendre with 16-row
+// leaves 15.7-19x, 3000 random points 8.4-11.4x) and loses at 5.3x and
+// below (the 100^3 grid 1.4-2.0x, 10^4-2 10^5 random points 1.1-4.2x, 2-D
+// random 3.4-5.3x), its per-step segment lookup costing more than the
+// rounds it saves
+constexpr double kFlatGain = 6.0;
+
 // LDS a workgroup ACA may take for its block's points ((m + n)(NDIM + 1) 8
 // bytes, plus 8 per column with weights): the 64 class always fits
 // (m + n <= 256); the 256 class up to 32 KiB (about 1000 points in 3-D,
@@ -1513,6 +1522,38 @@ void assemble(hodlr_op &op) {
     op.n_panel = (int64_t)panel.size();
     op.n_seg = (int64_t)segs.size();
     op.n_quad = (int64_t)quads.size();
+    {
+        // 1-2 RHS default panel form, from the load rounds of one wave the
+        // two walks need (kU = 8 column steps per round, 4 waves, G = 64 /
+        // ceil(h / R) columns per step, R = 16 / sizeof(T)): one segment at a
+        // time, a segment takes ceil(len / (4 kU G)) rounds; over the
+        // flattened walk a tile takes ceil(sum ceil(len / G) / (4 kU)).
+        // Short segments under short tiles leave the per-segment walk's
+        // rounds mostly empty (4-D 16^4 Gauss-Legendre, 16-row leaves: FP32
+        // panel 1.45 -> 0.90 ms, FP64 1.59 -> 1.41); elsewhere the flattened
+        // walk's per-step lookup costs more than it saves.
+        for (int pi = 0; pi < 2; pi++) {
+            const int R = pi == 0 ? 2 : 4;  // FP64, FP32
+            double seg_rounds = 0.0, flat_rounds = 0.0;
+            for (size_t pq = 0; pq < panel.size(); pq++) {
+                const PanelItem &it = panel[pq];
+                const int hl = (it.h + R - 1) / R, G = kWave / hl;
+                int64_t steps = 0;
+                for (int64_t q = it.seg0; q < it.seg0 + it.nseg; q++) {
+                    seg_rounds += (double)((segs[q].len + 32 * G - 1) / (32 * G));
+                    steps += (segs[q].len + G - 1) / G;
+                }
+                flat_rounds += (double)((steps + 31) / 32);
+            }
+            const double gain = flat_rounds > 0.0 ? seg_rounds / flat_rounds : 0.0;
+            if (gain >= kFlatGain)
+                for (int lg = 0; lg < 2; lg++)
+                    if (op.mv_variant[pi][lg] == kMatvecPanelVec) op.mv_variant[pi][lg] |= kMatvecPanelFlat;
+            if (verbose())
+                std::fprintf(stderr, "[hodlr] panel %s: %zu segments, rounds per segment %.0f, flattened %.0f (%.2fx)\n",
+                             pi == 0 ? "f64" : "f32", (size_t)segs.size(), seg_rounds, flat_rounds, gain);
+        }
+    }
     {  // per-position diagonal (a batch may mix diagonals) for the FP32 path
         std::vector<double> dg(N > 0 ? N : 1, 0.0);
         for (int p = 0; p < P; p++)
@@ -1791,6 +1832,7 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
             std::vector<int> cands;
             if (k <= 8) cands = {0, kMatvecPanelFlat};
             if (k <= 8) cands.push_back(kMatvecPanelVec);
+            if (k <= 8) cands.push_back(kMatvecPanelVec | kMatvecPanelFlat);
             if (k <= 2) cands.push_back(kMatvecProjPack | kMatvecPanelVec);
             if (k >= 4) cands.push_back(kMatvecProjMfma | kMatvecPanelMfma);
             if (k >= 16) cands.push_back(kMatvecPanelMfma);  // VALU projection, MFMA panel
@@ -1858,7 +1900,8 @@ void hodlr_op::tune_matvec(int prec_mask, int reps) {
                                  : (pick[0] & kMatvecProjWave ? "wave" : (pick[0] & kMatvecProjPack ? "pack" : "valu")),
                              best[0] / reps,
                              pick[1] & kMatvecPanelMfma ? (pick[1] & kMatvecPanelWide ? "mfma-wide" : "mfma")
-                                 : (pick[1] & kMatvecPanelVec ? "vec" : (pick[1] ? "flat" : "seg")),
+                                 : (pick[1] & kMatvecPanelVec ? (pick[1] & kMatvecPanelFlat ? "vec-flat" : "vec")
+                                                              : (pick[1] ? "flat" : "seg")),
                              best[1] / reps);
             if (verbose() && skel_best)
                 std::fprintf(stderr, "[hodlr] tune %s K=%d: skeleton schedule %d (whole product: one stream %.3f ms, "

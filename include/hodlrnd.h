@@ -306,15 +306,20 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * flattened column list (else one segment at a time; suits many short
  * segments), 8 = vector-ALU panel product reading 16 bytes per lane and
  * column (two FP64 / four FP32 rows; up to 8 RHS; MFMA takes precedence
- * when also set, and 8 over 4), 16 = vector-ALU projection packing several
+ * when also set; 8 | 4 = the same 16-byte loads over the flattened walk of
+ * 4, for leaves with many short segments), 16 = vector-ALU projection packing several
  * rows of a narrow V-panel tile into one wave load (1-2 RHS), 32 = with 2,
  * the matrix-core panel product staging each group of columns through LDS
  * with 16-byte loads (4-32 RHS), 64 = vector-ALU projection taking items of
  * at most 64 rows one wave each, four per workgroup (1-32 RHS, FP64 to 16;
  * MFMA takes precedence).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
- * results agree up to rounding order.  get/set read or force the choice for
- * one (prec, nrhs). */
+ * results agree up to rounding order.  The 1-2 RHS default is 8, or 8 | 4
+ * where the leaves' segment lists are short enough that one segment at a
+ * time would leave most of each load round idle; each operator decides from
+ * its own lists, so two operators (a batch and its problems alone) give
+ * bit-identical rows only under the same forms.  get/set read or force the
+ * choice for one (prec, nrhs). */
 #define HODLR_MV_PROJ_MFMA 1
 #define HODLR_MV_PANEL_MFMA 2
 #define HODLR_MV_PANEL_FLAT 4

@@ -348,8 +348,11 @@ def test_matvec_kernel_forms_agree(nd, n, nmax):
             a = op.matvec(Xk, prec=prec)
             op.set_variant(prec, k, H.MV_PANEL_VEC)
             b = op.matvec(Xk, prec=prec)
+            op.set_variant(prec, k, H.MV_PANEL_VEC | H.MV_PANEL_FLAT)  # 16-byte loads, flattened walk
+            bf = op.matvec(Xk, prec=prec)
             op.set_variant(prec, k, 0)
             assert relerr(b, a) < tol, (prec, k, relerr(b, a))
+            assert relerr(bf, a) < tol, (prec, k, relerr(bf, a))
             if k <= 2:  # row-packed projection
                 op.set_variant(prec, k, H.MV_PROJ_PACK)
                 c = op.matvec(Xk, prec=prec)
@@ -453,3 +456,32 @@ def test_assembly_staging_identical(nd, n, nmax, eps, sys_size, cap):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     for u, v in zip(a[2:], b[2:]):
         assert np.array_equal(u, v), relerr(u, v)
+
+
+def test_panel_forms_small_leaves_many_segments():
+    """8^4 Gauss-Legendre nodes, Nmax 64: 16-point leaves whose segment lists
+    (80 neighbour blocks, dense-fallback blocks) run past 64 segments, so the
+    flattened 16-byte panel form walks more than one chunk of descriptors."""
+    say("case panel_small_leaves")
+    gx, gw = O.gauss_legendre(8)
+    pts = O.tensor_grid(4, gx)
+    wts = O.tensor_weights(4, gw)
+    n = len(pts)
+    T = H.Tree(pts, -np.ones(4), np.ones(4), 64, 0)
+    op = H.Operator(T, H.Kernel(H.KERNEL_INV_R, 4, 1.0, 1.0, wts), 1e-8)
+    X = np.stack([O.splitmix_uniform(60 + c, n) for c in range(8)], axis=1)
+    ok = O.KernelDesc(O.K_INV, 4, 1.0, 1.0, wts)
+    yd = O.dense_matvec(ok, pts, X[:, 0])
+    for prec, tol in ((H.PREC_F64, 1e-14), (H.PREC_F32, 1e-6)):
+        for k in (1, 2, 4, 8):
+            Xk = X[:, :k] if k > 1 else X[:, 0]
+            op.set_variant(prec, k, 0)
+            a = op.matvec(Xk, prec=prec)
+            op.set_variant(prec, k, H.MV_PANEL_VEC | H.MV_PANEL_FLAT)
+            b = op.matvec(Xk, prec=prec)
+            op.set_variant(prec, k, 0)
+            say(" prec", prec, "k", k, "vec-flat vs per-segment", relerr(b, a))
+            assert relerr(b, a) < tol, (prec, k)
+    e = relerr(op.matvec(X[:, 0]), yd)
+    say(" vs dense", e)
+    assert e < 1e-6

@@ -165,6 +165,10 @@ def test_skeleton_batch_equals_alone():
         kerns.append(H.Kernel(fam, nd, 1.0 / n, 1.0 + 1.0 / n))
         alone.append(H.Operator(trees[-1], kerns[-1], 1e-8, sys_size=64))
     B = H.Operator(trees, kerns, 1e-8, sys_size=64)
+    for a in alone:  # each operator picks its default forms from its own segment lists
+        for prec in (H.PREC_F64, H.PREC_F32):
+            for k in (1, 4):
+                a.set_variant(prec, k, B.get_variant(prec, k))
     off = B.offsets()
     assert B.stats()["n_skeleton"] == sum(a.stats()["n_skeleton"] for a in alone) > 0
     for prec in (H.PREC_F64, H.PREC_F32):
