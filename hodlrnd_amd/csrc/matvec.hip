@@ -428,10 +428,13 @@ __device__ __forceinline__ int64_t rdlane64(int64_t v, int s) {
 // sizeof(T)) in column group g of G = 64 / ceil(h / R); the leaf's segments
 // are cut into steps of G columns and walked as one list, the 4 waves taking
 // groups of kU steps in turn.  Each wave keeps up to 64 segment descriptors
-// in registers (lane i: segment i of the current chunk, with the prefix sum
-// of the steps), so finding a step's segment is a ballot and a few scalar
-// lane reads, not a dependent load.  Reads past a column's last row go to rows
-// >= h, which the reduction drops (the arenas carry tail padding).
+// in registers (lane i: segment i of the current chunk -- its matrix and
+// vector base addresses, its length and its first and end step), so a
+// step's segment is a ballot, a popcount and six lane reads: no dependent
+// load and no division per step (a run-time divide is a long scalar
+// sequence, and the scalar unit, one per CU, is what bounds this walk).
+// Reads past a column's last row go to rows >= h, which the reduction drops
+// (the arenas carry tail padding).
 template <typename T, int K>
 __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__restrict__ items,
                                                             const Segment *__restrict__ segs,
@@ -456,17 +459,16 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
         for (int k = 0; k < K; k++) acc[r][k] = T(0);
     // the current chunk of descriptors: lane i holds segment c0 + i
     int c0 = 0, cbase = 0, cend = 0;  // chunk's first step and one past its last
-    int64_t d_mat = 0, d_vec = 0;
-    int d_len = 0, d_arena = 0, d_qend = 0;
+    int64_t d_m = 0, d_v = 0;         // the segment's matrix (row r0) and vector addresses
+    int d_len = 0, d_qbeg = 0, d_qend = 0;
     auto load_chunk = [&]() {
         const int i = c0 + lane;
         int steps = 0;
         if (i < P.nseg) {
             const Segment S = segs[P.seg0 + i];
-            d_mat = S.mat;
-            d_vec = S.vec;
+            d_m = (int64_t)((S.arena == 0 ? NP : UP) + S.mat + P.r0);
+            d_v = (int64_t)((S.arena == 1 ? t : x) + S.vec * K);
             d_len = S.len;
-            d_arena = S.arena;
             steps = (S.len + G - 1) / G;
         }
         int incl = steps;  // inclusive prefix sum over the wave
@@ -475,6 +477,7 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
             const int v = __shfl_up(incl, off, kWave);
             if (lane >= off) incl += v;
         }
+        d_qbeg = cbase + incl - steps;
         d_qend = i < P.nseg ? cbase + incl : INT_MAX;
         cend = cbase + __shfl(incl, kWave - 1, kWave);
     };
@@ -487,6 +490,7 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
         for (int off = 32; off > 0; off >>= 1) steps += __shfl_xor(steps, off, kWave);
         nq += steps;
     }
+    const int64_t lrow = (int64_t)o * R;
     for (int g0 = wid * kU; g0 < nq; g0 += NW * kU) {
         vec_t a[kU];
         T vv[kU][K];
@@ -503,16 +507,19 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
             // before it come first
             const int s = in ? __builtin_amdgcn_readfirstlane(__popcll(__ballot(d_qend <= step))) : 0;
             const int len = rdlane(d_len, s);
-            const int arena = rdlane(d_arena, s);
-            const int64_t mat = rdlane64(d_mat, s), vec = rdlane64(d_vec, s);
-            const int qb = rdlane(d_qend, s) - (len + G - 1) / G;
-            const int col = (step - qb) * G + g;
+            const int col = (step - rdlane(d_qbeg, s)) * G + g;
             const bool ok = in && active && col < len;
-            const T *M = (arena == 0 ? NP : UP) + mat + P.r0 + (int64_t)o * R;
-            const T *v = (arena == 1 ? t : x) + vec * K;
-            a[u] = ok ? *reinterpret_cast<const vec_t *>(M + (int64_t)col * ld) : vec_t(T(0));
+            // unpredicated global loads: a lane without a column reads the
+            // arena's and x's first elements (always allocated) and drops them
+            const T *M = ok ? reinterpret_cast<const T *>(rdlane64(d_m, s)) + lrow + (int64_t)col * ld : NP;
+            const T *v = ok ? reinterpret_cast<const T *>(rdlane64(d_v, s)) + (int64_t)col * K : x;
+            const vec_t av = *(const __attribute__((address_space(1))) vec_t *)M;
+            a[u] = ok ? av : vec_t(T(0));
 #pragma unroll
-            for (int k = 0; k < K; k++) vv[u][k] = ok ? v[(int64_t)col * K + k] : T(0);
+            for (int k = 0; k < K; k++) {
+                const T vk = ((const __attribute__((address_space(1))) T *)v)[k];
+                vv[u][k] = ok ? vk : T(0);
+            }
         }
 #pragma unroll
         for (int u = 0; u < kU; u++)

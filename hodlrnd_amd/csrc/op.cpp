@@ -68,14 +68,15 @@ int aca_block_size(int m, int n, int cap) {
 
 bool aca_in_registers(int cls) { return cls < 64; }
 
-// ratio of per-segment to flattened load rounds at or above which the 1-2 RHS
-// panel product defaults to the flattened 16-byte form.  Measured: the
-// flattened walk wins at 8.4x and above (16^4 Gauss-Legendre with 16-row
-// leaves 15.7-19x, 3000 random points 8.4-11.4x) and loses at 5.3x and
-// below (the 100^3 grid 1.4-2.0x, 10^4-2 10^5 random points 1.1-4.2x, 2-D
-// random 3.4-5.3x), its per-step segment lookup costing more than the
-// rounds it saves
-constexpr double kFlatGain = 6.0;
+// ratio of per-segment to flattened load rounds at or above which the 1-4 RHS
+// panel product defaults to the flattened 16-byte form.  Measured (FP32 / FP64
+// panel ms, per-segment vs flattened): it wins or ties from 2.6x (16^4
+// Gauss-Legendre with 16-row leaves, 16-19x: 1.67 / 1.92 vs 0.66 / 1.08;
+// 10^4 random points, 2.6-4.2x: 0.041 / 0.055 vs 0.027 / 0.053; 2-D random,
+// 3.4-5.3x: 0.016 / 0.018 vs 0.012 / 0.018) and loses at 2.0x and below
+// (the 100^3 grid 1.67 / 3.18 vs 1.77 / 3.31; 2 10^4 random points 0.052 /
+// 0.097 vs 0.063 / 0.120); the 16^4 d' = 3 operator (2.0-2.5x) ties
+constexpr double kFlatGain = 2.5;
 
 // LDS a workgroup ACA may take for its block's points ((m + n)(NDIM + 1) 8
 // bytes, plus 8 per column with weights): the 64 class always fits
@@ -1523,15 +1524,14 @@ void assemble(hodlr_op &op) {
     op.n_seg = (int64_t)segs.size();
     op.n_quad = (int64_t)quads.size();
     {
-        // 1-2 RHS default panel form, from the load rounds of one wave the
+        // 1-4 RHS default panel form, from the load rounds of one wave the
         // two walks need (kU = 8 column steps per round, 4 waves, G = 64 /
         // ceil(h / R) columns per step, R = 16 / sizeof(T)): one segment at a
         // time, a segment takes ceil(len / (4 kU G)) rounds; over the
         // flattened walk a tile takes ceil(sum ceil(len / G) / (4 kU)).
         // Short segments under short tiles leave the per-segment walk's
-        // rounds mostly empty (4-D 16^4 Gauss-Legendre, 16-row leaves: FP32
-        // panel 1.45 -> 0.90 ms, FP64 1.59 -> 1.41); elsewhere the flattened
-        // walk's per-step lookup costs more than it saves.
+        // rounds mostly empty; where they do not, the flattened walk's
+        // per-step segment lookup costs more than it saves.
         for (int pi = 0; pi < 2; pi++) {
             const int R = pi == 0 ? 2 : 4;  // FP64, FP32
             double seg_rounds = 0.0, flat_rounds = 0.0;
@@ -1547,7 +1547,7 @@ void assemble(hodlr_op &op) {
             }
             const double gain = flat_rounds > 0.0 ? seg_rounds / flat_rounds : 0.0;
             if (gain >= kFlatGain)
-                for (int lg = 0; lg < 2; lg++)
+                for (int lg = 0; lg < 3; lg++)
                     if (op.mv_variant[pi][lg] == kMatvecPanelVec) op.mv_variant[pi][lg] |= kMatvecPanelFlat;
             if (verbose())
                 std::fprintf(stderr, "[hodlr] panel %s: %zu segments, rounds per segment %.0f, flattened %.0f (%.2fx)\n",

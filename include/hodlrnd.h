@@ -314,7 +314,7 @@ int hodlr_vec_load(const char *path, int64_t cap, double *v, int64_t *n);
  * at most 64 rows one wave each, four per workgroup (1-32 RHS, FP64 to 16;
  * MFMA takes precedence).  tune times the forms of each pass on this operator for
  * nrhs = 1 .. 32 and the precisions in prec_mask and keeps the fastest;
- * results agree up to rounding order.  The 1-2 RHS default is 8, or 8 | 4
+ * results agree up to rounding order.  The 1-4 RHS default is 8, or 8 | 4
  * where the leaves' segment lists are short enough that one segment at a
  * time would leave most of each load round idle; each operator decides from
  * its own lists, so two operators (a batch and its problems alone) give
