@@ -441,7 +441,10 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
                                                             const T *__restrict__ NP, const T *__restrict__ UP,
                                                             const T *__restrict__ x, const T *__restrict__ t,
                                                             T *__restrict__ y) {
-    constexpr int R = 16 / (int)sizeof(T), NW = 4, kU = K <= 2 ? 8 : (K <= 4 ? 4 : 2);
+    // column steps in flight per lane: fewer for more RHS (registers); at 8
+    // RHS FP32 keeps 4 (108 VGPRs: 256-problem batch panel 6.43 -> 5.95 ms),
+    // FP64 2 (4 took 140 VGPRs and measured slower, 13.6 -> 14.0 ms)
+    constexpr int R = 16 / (int)sizeof(T), NW = 4, kU = K <= 2 ? 8 : (K <= 4 || sizeof(T) == 4 ? 4 : 2);
     typedef T vec_t __attribute__((ext_vector_type(R), aligned(sizeof(T))));
     __shared__ T red[NW][kWave][R][K];
     const int lane = threadIdx.x & (kWave - 1), wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -515,10 +518,19 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
             const T *v = ok ? reinterpret_cast<const T *>(rdlane64(d_v, s)) + (int64_t)col * K : x;
             const vec_t av = *(const __attribute__((address_space(1))) vec_t *)M;
             a[u] = ok ? av : vec_t(T(0));
+            if constexpr (K % R == 0) {  // the column's K values as K/R 16-byte loads
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const T vk = ((const __attribute__((address_space(1))) T *)v)[k];
-                vv[u][k] = ok ? vk : T(0);
+                for (int kk = 0; kk < K / R; kk++) {
+                    const vec_t w = ((const __attribute__((address_space(1))) vec_t *)v)[kk];
+#pragma unroll
+                    for (int r = 0; r < R; r++) vv[u][kk * R + r] = ok ? w[r] : T(0);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const T vk = ((const __attribute__((address_space(1))) T *)v)[k];
+                    vv[u][k] = ok ? vk : T(0);
+                }
             }
         }
 #pragma unroll
