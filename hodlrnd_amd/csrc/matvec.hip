@@ -513,7 +513,9 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
             const int col = (step - rdlane(d_qbeg, s)) * G + g;
             const bool ok = in && active && col < len;
             // unpredicated global loads: a lane without a column reads the
-            // arena's and x's first elements (always allocated) and drops them
+            // arena's and x's first elements (always allocated); its matrix
+            // values are zeroed, so it adds zeros times x's first (finite)
+            // values -- a non-finite x reaches every row of the product anyway
             const T *M = ok ? reinterpret_cast<const T *>(rdlane64(d_m, s)) + lrow + (int64_t)col * ld : NP;
             const T *v = ok ? reinterpret_cast<const T *>(rdlane64(d_v, s)) + (int64_t)col * K : x;
             const vec_t av = *(const __attribute__((address_space(1))) vec_t *)M;
@@ -523,14 +525,11 @@ __global__ __launch_bounds__(256) void panel_vecflat_kernel(const PanelItem *__r
                 for (int kk = 0; kk < K / R; kk++) {
                     const vec_t w = ((const __attribute__((address_space(1))) vec_t *)v)[kk];
 #pragma unroll
-                    for (int r = 0; r < R; r++) vv[u][kk * R + r] = ok ? w[r] : T(0);
+                    for (int r = 0; r < R; r++) vv[u][kk * R + r] = w[r];
                 }
             } else {
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const T vk = ((const __attribute__((address_space(1))) T *)v)[k];
-                    vv[u][k] = ok ? vk : T(0);
-                }
+                for (int k = 0; k < K; k++) vv[u][k] = ((const __attribute__((address_space(1))) T *)v)[k];
             }
         }
 #pragma unroll
